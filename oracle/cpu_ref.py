@@ -1,0 +1,655 @@
+"""CPU oracle for the 2D CSR wake step -- TEST INFRASTRUCTURE ONLY.
+
+This module is the checker, never the product. Only ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may
+import it. The product path (``pydfcsr_amd``) never imports anything under
+``oracle/`` and fails loudly when its HIP library is missing.
+
+It is a float64 numpy restatement of pyDFCSR's hot path, written from the
+reference's source text (paths relative to the reference root
+``pyDFCSR_2D/``).  Each function cites the lines it follows.  numpy/scipy
+routines that the reference itself calls (``savgol_filter``, ``np.gradient``,
+``np.trapz``, ``RegularGridInterpolator``, ``np.polyfit``) are called here the
+same way, so the oracle inherits their exact arithmetic.
+
+Pinning (what checks each function, see tests/test_oracle_*.py):
+  histogram_cic_2d      brute force on hand-built particles, serial-loop form
+                        bit-exact, golden vectors from the reference module.
+  interpolate3D/1D      exact on multilinear functions, truncation/clamp/zero
+                        special cases, golden vectors from the reference module.
+  reference_trajectory  closed-form arcs, unit/orthogonal tau and n.
+  get_DF                scipy savgol_filter / np.gradient composition; golden
+                        vectors from the reference module.
+  DFTracker             RegularGridInterpolator re-grid, window pop rule,
+                        golden vectors from the reference module.
+  csr_integrand/wake    zero-density and linearity invariants, scalar loop
+                        form, golden vectors from the reference module.
+  apply_wakes           RegularGridInterpolator closed form on bilinear fields.
+"""
+from __future__ import annotations
+
+import math
+from collections import deque
+
+import numpy as np
+from scipy.interpolate import RegularGridInterpolator
+from scipy.signal import savgol_filter
+
+_trapz = getattr(np, "trapezoid", None) or np.trapz
+
+# CSR_scaling = 8.98755e3 * charge[C]   (CSR.py:80)
+CSR_SCALING_PER_COULOMB = 8.98755e3
+
+
+# ---------------------------------------------------------------------------
+# a3  histogram_cic_2d   (deposit.py:42-87)
+# ---------------------------------------------------------------------------
+def cic_cells(q1, q2, nbins_1, bins_start_1, bins_end_1, nbins_2, bins_start_2, bins_end_2):
+    """Per-particle lower-cell indices and CIC fractions (deposit.py:55-75).
+
+    The FP64 expressions are the reference's: ``cell = (q - start) * inv``
+    with ``inv = 1 / ((end - start) / nbins)`` and ``i = floor(cell)``.
+    """
+    q1 = np.asarray(q1, dtype=np.float64)
+    q2 = np.asarray(q2, dtype=np.float64)
+    inv1 = 1.0 / ((bins_end_1 - bins_start_1) / nbins_1)
+    inv2 = 1.0 / ((bins_end_2 - bins_start_2) / nbins_2)
+    c1 = (q1 - bins_start_1) * inv1
+    c2 = (q2 - bins_start_2) * inv2
+    f1 = np.floor(c1)
+    f2 = np.floor(c2)
+    S1 = 1.0 - (c1 - f1)
+    S2 = 1.0 - (c2 - f2)
+    return f1.astype(np.int64), f2.astype(np.int64), S1, S2
+
+
+def histogram_cic_2d(q1, q2, w, nbins_1, bins_start_1, bins_end_1,
+                     nbins_2, bins_start_2, bins_end_2):
+    """CIC deposition with the reference's serial accumulation order.
+
+    deposit.py:65-85 adds, for each particle in turn, its four corner
+    contributions ``w*S1*S2`` etc.  ``np.add.at`` with particle-major,
+    corner-minor ordering reproduces the same FP64 sum order exactly.
+    """
+    i1, i2, S1, S2 = cic_cells(q1, q2, nbins_1, bins_start_1, bins_end_1,
+                               nbins_2, bins_start_2, bins_end_2)
+    w = np.asarray(w, dtype=np.float64)
+    n = w.shape[0]
+    vals = np.empty((n, 4))
+    vals[:, 0] = w * S1 * S2
+    vals[:, 1] = w * S1 * (1.0 - S2)
+    vals[:, 2] = w * (1.0 - S1) * S2
+    vals[:, 3] = w * (1.0 - S1) * (1.0 - S2)
+    a = np.stack([i1, i1, i1 + 1, i1 + 1], axis=1)
+    b = np.stack([i2, i2 + 1, i2, i2 + 1], axis=1)
+    ok = (a >= 0) & (a < nbins_1) & (b >= 0) & (b < nbins_2)
+    hist = np.zeros((nbins_1, nbins_2), dtype=np.float64)
+    flat = (a * nbins_2 + b)[ok]
+    np.add.at(hist.reshape(-1), flat, vals[ok])
+    return hist
+
+
+def histogram_cic_2d_loop(q1, q2, w, nbins_1, bins_start_1, bins_end_1,
+                          nbins_2, bins_start_2, bins_end_2):
+    """Literal scalar-loop form of deposit.py:54-87, for tiny inputs."""
+    bin_spacing_1 = (bins_end_1 - bins_start_1) / nbins_1
+    inv_spacing_1 = 1. / bin_spacing_1
+    bin_spacing_2 = (bins_end_2 - bins_start_2) / nbins_2
+    inv_spacing_2 = 1. / bin_spacing_2
+    hist = np.zeros((nbins_1, nbins_2))
+    for i in range(len(w)):
+        c1 = (q1[i] - bins_start_1) * inv_spacing_1
+        c2 = (q2[i] - bins_start_2) * inv_spacing_2
+        i1 = int(math.floor(c1))
+        i2 = int(math.floor(c2))
+        S1 = 1. - (c1 - i1)
+        S2 = 1. - (c2 - i2)
+        if 0 <= i1 < nbins_1:
+            if 0 <= i2 < nbins_2:
+                hist[i1, i2] += w[i] * S1 * S2
+            if 0 <= i2 + 1 < nbins_2:
+                hist[i1, i2 + 1] += w[i] * S1 * (1. - S2)
+        if 0 <= i1 + 1 < nbins_1:
+            if 0 <= i2 < nbins_2:
+                hist[i1 + 1, i2] += w[i] * (1. - S1) * S2
+            if 0 <= i2 + 1 < nbins_2:
+                hist[i1 + 1, i2 + 1] += w[i] * (1. - S1) * (1. - S2)
+    return hist
+
+
+def cell_counts(i1, i2, nbins_1, nbins_2):
+    """Integer count of particles whose lower cell (i1, i2) is on the grid."""
+    ok = (i1 >= 0) & (i1 < nbins_1) & (i2 >= 0) & (i2 < nbins_2)
+    cnt = np.zeros(nbins_1 * nbins_2, dtype=np.int64)
+    np.add.at(cnt, (i1 * nbins_2 + i2)[ok], 1)
+    return cnt.reshape(nbins_1, nbins_2)
+
+
+# ---------------------------------------------------------------------------
+# a1  beam moments   (beams.py:88-98, 137-215; deposit.py:147-159)
+# ---------------------------------------------------------------------------
+def beam_moments(x, z):
+    """sigma/mean/slope/x_transform statistics the step reads.
+
+    beams.py:201-204 ``slope = np.polyfit(z, x, 1)``; beams.py:206-211
+    ``x_transform = x - np.polyval(slope, z)``; CSR.py:371-372 std/mean of
+    x_transform; deposit.py:157-159 the |z| < 0.1 sigma_z slice std.
+    """
+    x = np.asarray(x, dtype=np.float64)
+    z = np.asarray(z, dtype=np.float64)
+    sigma_x = np.std(x)
+    sigma_z = np.std(z)
+    p = np.polyfit(z, x, deg=1)
+    xt = x - np.polyval(p, z)
+    slice_ind = np.argwhere(np.abs(z) < 0.1 * sigma_z)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        slice_sig = np.std(x[slice_ind]) if slice_ind.size else np.nan
+    return dict(mean_x=np.mean(x), mean_z=np.mean(z), sigma_x=sigma_x, sigma_z=sigma_z,
+                slope=p, mean_xt=np.mean(xt), sigma_xt=np.std(xt), slice_sigma_x=slice_sig,
+                n_slice=int(slice_ind.shape[0]))
+
+
+# ---------------------------------------------------------------------------
+# a4-a6  DF_tracker.get_DF   (deposit.py:145-245)
+# ---------------------------------------------------------------------------
+def _sg2(a, window, order):
+    """deposit.py:195-196: savgol along axis 0, then axis 1 (mode='interp')."""
+    return savgol_filter(savgol_filter(a, window_length=window, polyorder=order, axis=0),
+                         window_length=window, polyorder=order, axis=1)
+
+
+def get_DF(x, z, px, t, xbins=100, zbins=100, xlim=5, zlim=5, filter_order=0,
+           filter_window=0, velocity_threhold=5, **_unused):
+    """Deposition + density-function post-processing (deposit.py:145-245)."""
+    x = np.asarray(x, dtype=np.float64)
+    z = np.asarray(z, dtype=np.float64)
+    px = np.asarray(px, dtype=np.float64)
+    sigma_x = np.std(x)
+    sigma_z = np.std(z)
+    xmean = np.mean(x)
+    zmean = np.mean(z)
+    slice_ind = np.argwhere(np.abs(z) < 0.1 * sigma_z)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        slice_sigX = np.std(x[slice_ind]) if slice_ind.size else np.nan
+        frac = sigma_x / slice_sigX
+    if frac > 5:                                    # deposit.py:160-167
+        xb, zb, window = xbins, zbins, filter_window
+    else:
+        xb, zb, window = 100, 100, 5
+    x0, x1 = xmean - xlim * sigma_x, xmean + xlim * sigma_x
+    z0, z1 = zmean - zlim * sigma_z, zmean + zlim * sigma_z
+    x_grids = np.linspace(x0, x1, xb)
+    z_grids = np.linspace(z0, z1, zb)
+    density = histogram_cic_2d(x, z, np.ones(x.shape), xb, x0, x1, zb, z0, z1)
+    vx = histogram_cic_2d(x, z, px, xb, x0, x1, zb, z0, z1)
+    threshold = np.max(density) / velocity_threhold
+    m = density > threshold
+    vx[m] /= density[m]
+    density = _sg2(density, window, filter_order)
+    vx = _sg2(vx, window, filter_order)
+    dsum = _trapz(_trapz(density, x_grids, axis=0), z_grids)
+    density /= dsum
+    vx[density <= threshold] = 0                    # raw-count threshold (deposit.py:204)
+    density_x, density_z = np.gradient(density, x_grids, z_grids)
+    vx_x, _vx_z = np.gradient(vx, x_grids, z_grids)
+    density_x = _sg2(density_x, window, filter_order)
+    density_z = _sg2(density_z, window, filter_order)
+    vx_x = _sg2(vx_x, window, filter_order)
+    threshold = np.max(density) / velocity_threhold * 8
+    with np.errstate(invalid="ignore"):
+        fill = np.mean(vx_x[density > threshold]) if np.any(density > threshold) else np.nan
+    vx_x[density < threshold] = fill
+    return dict(x_grids=x_grids, z_grids=z_grids, density=density, vx=vx,
+                density_x=density_x, density_z=density_z, vx_x=vx_x, t=t,
+                sigma_x=sigma_x, sigma_z=sigma_z, xmean=xmean, zmean=zmean,
+                frac=frac, nbins=(xb, zb), window=window,
+                x_range=(x0, x1), z_range=(z0, z1))
+
+
+FIELDS = ("density", "density_x", "density_z", "vx", "vx_x")
+
+
+# ---------------------------------------------------------------------------
+# a8-a10  DF history   (deposit.py:247-426)
+# ---------------------------------------------------------------------------
+class DFTracker:
+    """History of DF slices, re-gridding and 3D stacking (deposit.py:90-426)."""
+
+    def __init__(self, xbins=100, zbins=100, xlim=5, zlim=5, filter_order=0,
+                 filter_window=0, velocity_threhold=5, upper_limit=None):
+        self.cfg = dict(xbins=xbins, zbins=zbins, xlim=xlim, zlim=zlim,
+                        filter_order=filter_order, filter_window=filter_window,
+                        velocity_threhold=velocity_threhold)
+        self.upper_limit = upper_limit
+        self.start_time = 0.0
+        self.interp_start = 0
+        self.DF_log = deque()
+        self.time_log = deque()
+        self.sigma_x_log = deque()
+        self.sigma_z_log = deque()
+        self.time_interp = deque()
+        self.interp = {f: deque() for f in FIELDS}
+        self.sigma_x_interp = None
+        self.sigma_z_interp = None
+        self.x_grid_interp = None
+        self.z_grid_interp = None
+        self.cur = None
+        self.n_regrid = 0
+
+    def get_DF(self, x, z, px, t):
+        self.cur = get_DF(x, z, px, t, **self.cfg)
+        return self.cur
+
+    def append_DF(self):                                 # deposit.py:247-256
+        c = self.cur
+        self.DF_log.append(c)
+        self.time_log.append(c["t"])
+        self.sigma_x_log.append(c["sigma_x"])
+        self.sigma_z_log.append(c["sigma_z"])
+        self.end_time = c["t"]
+
+    def pop_left_DF(self, new_start_time):               # deposit.py:258-280
+        while self.start_time < new_start_time:
+            self.DF_log.popleft()
+            self.time_log.popleft()
+            self.sigma_x_log.popleft()
+            self.sigma_z_log.popleft()
+            self.start_time = self.time_log[0]
+        while self.interp_start < new_start_time:
+            for f in FIELDS:
+                self.interp[f].popleft()
+            self.time_interp.popleft()
+            self.interp_start = self.time_interp[0]
+
+    def DF_interp(self, DF, x_grids, z_grids, fill_value=0.0):   # deposit.py:296-309
+        X, Z = np.meshgrid(self.x_grid_interp, self.z_grid_interp, indexing="ij")
+        f = RegularGridInterpolator((x_grids, z_grids), DF, method="linear",
+                                    fill_value=fill_value, bounds_error=False)
+        return f((X, Z))
+
+    def _interp_slice(self, c):
+        out = {}
+        for f in FIELDS:
+            fill = np.mean(c["vx_x"]) if f == "vx_x" else 0.0
+            out[f] = self.DF_interp(c[f], c["x_grids"], c["z_grids"], fill_value=fill)
+        return out
+
+    def append_interpolant(self, formation_length, n_formation_length):   # deposit.py:312-392
+        start_point = np.amax(a=(0, self.end_time - n_formation_length * formation_length))
+        self.pop_left_DF(new_start_time=start_point)
+        c = self.cur
+        if (self.sigma_x_interp and self.sigma_z_interp
+                and 2 > c["sigma_x"] / self.sigma_x_interp > 1 / 2
+                and 2 > c["sigma_z"] / self.sigma_z_interp > 1 / 2):
+            self.time_interp.append(c["t"])
+            sl = self._interp_slice(c)
+            for f in FIELDS:
+                self.interp[f].append(sl[f])
+            return False
+        max_sx, min_sx = np.max(self.sigma_x_log), np.min(self.sigma_x_log)
+        max_sz, min_sz = np.max(self.sigma_z_log), np.min(self.sigma_z_log)
+        xbins = int(500 * (max_sx / min_sx))
+        zbins = int(500 * (max_sz / min_sz))
+        if isinstance(self.upper_limit, int):
+            xbins = min(xbins, self.upper_limit)
+            zbins = min(zbins, self.upper_limit)
+        self.sigma_x_interp = max_sx
+        self.sigma_z_interp = max_sz
+        self.x_grid_interp = np.linspace(c["xmean"] - 5 * max_sx, c["xmean"] + 5 * max_sx, xbins)
+        self.z_grid_interp = np.linspace(c["zmean"] - 5 * max_sz, c["zmean"] + 5 * max_sz, zbins)
+        self.interp = {f: deque() for f in FIELDS}
+        self.time_interp = self.time_log.copy()
+        for d in self.DF_log:
+            sl = self._interp_slice(d)
+            for f in FIELDS:
+                self.interp[f].append(sl[f])
+        self.n_regrid += 1
+        return True
+
+    def build_interpolant(self):                         # deposit.py:395-426
+        self.min_x, self.max_x = self.time_interp[0], self.time_interp[-1]
+        self.min_y, self.max_y = self.x_grid_interp[0], self.x_grid_interp[-1]
+        self.min_z, self.max_z = self.z_grid_interp[0], self.z_grid_interp[-1]
+        with np.errstate(invalid="ignore", divide="ignore"):
+            self.delta_x = (self.max_x - self.min_x) / (len(self.time_interp) - 1)
+        self.delta_y = (self.max_y - self.min_y) / (self.x_grid_interp.shape[0] - 1)
+        self.delta_z = (self.max_z - self.min_z) / (self.z_grid_interp.shape[0] - 1)
+        self.data = {f: np.array(self.interp[f]) for f in FIELDS}
+
+    def axes(self):
+        return dict(min_t=self.min_x, min_x=self.min_y, min_z=self.min_z,
+                    dt=self.delta_x, dx=self.delta_y, dz=self.delta_z)
+
+
+# ---------------------------------------------------------------------------
+# a11  interpolate3D   (interp3D.py:18-66)   a12  interpolate1D (interp1D.py:13-36)
+# ---------------------------------------------------------------------------
+def _trunc_index(u, size):
+    """int(u) truncation with the reference's validity rule.
+
+    interp3D.py:30-52: ``i0 = int(u)``, ``i1 = i0`` at the last plane else
+    ``i0 + 1``; valid iff ``i0 >= 0 and i1 < size``.  NaN is invalid.
+    """
+    with np.errstate(invalid="ignore"):
+        f = np.trunc(u)
+        ok = (f >= 0) & (f <= size - 1)
+    i0 = np.where(ok, f, 0).astype(np.int64)
+    i1 = np.where(i0 == size - 1, i0, i0 + 1)
+    d = np.where(ok, u - i0, 0.0)
+    return i0, i1, d, ok
+
+
+def interpolate3D(xval, yval, zval, data, min_x, min_y, min_z, delta_x, delta_y, delta_z):
+    """Vectorised interp3D.py:18-66 (same arithmetic, same order)."""
+    xval = np.asarray(xval, dtype=np.float64)
+    nx, ny, nz = data.shape
+    with np.errstate(invalid="ignore", divide="ignore"):
+        u = (xval - min_x) / delta_x
+        v = (np.asarray(yval, dtype=np.float64) - min_y) / delta_y
+        w = (np.asarray(zval, dtype=np.float64) - min_z) / delta_z
+    x0, x1, xd, okx = _trunc_index(u, nx)
+    y0, y1, yd, oky = _trunc_index(v, ny)
+    z0, z1, zd, okz = _trunc_index(w, nz)
+    ok = okx & oky & okz
+    c00 = data[x0, y0, z0] * (1 - xd) + data[x1, y0, z0] * xd
+    c01 = data[x0, y0, z1] * (1 - xd) + data[x1, y0, z1] * xd
+    c10 = data[x0, y1, z0] * (1 - xd) + data[x1, y1, z0] * xd
+    c11 = data[x0, y1, z1] * (1 - xd) + data[x1, y1, z1] * xd
+    c0 = c00 * (1 - yd) + c10 * yd
+    c1 = c01 * (1 - yd) + c11 * yd
+    return np.where(ok, c0 * (1 - zd) + c1 * zd, 0.0)
+
+
+def interpolate3D_loop(xval, yval, zval, data, min_x, min_y, min_z, delta_x, delta_y, delta_z):
+    """Literal scalar-loop form of interp3D.py:18-66, for tiny inputs."""
+    result = np.zeros(len(xval))
+    xs, ys, zs = data.shape
+    for i in range(len(xval)):
+        x = (xval[i] - min_x) / delta_x
+        y = (yval[i] - min_y) / delta_y
+        z = (zval[i] - min_z) / delta_z
+        if not (np.isfinite(x) and np.isfinite(y) and np.isfinite(z)):
+            continue
+        x0 = int(x); x1 = x0 if x0 == xs - 1 else x0 + 1
+        y0 = int(y); y1 = y0 if y0 == ys - 1 else y0 + 1
+        z0 = int(z); z1 = z0 if z0 == zs - 1 else z0 + 1
+        xd, yd, zd = x - x0, y - y0, z - z0
+        if x0 >= 0 and y0 >= 0 and z0 >= 0 and x1 < xs and y1 < ys and z1 < zs:
+            c00 = data[x0, y0, z0] * (1 - xd) + data[x1, y0, z0] * xd
+            c01 = data[x0, y0, z1] * (1 - xd) + data[x1, y0, z1] * xd
+            c10 = data[x0, y1, z0] * (1 - xd) + data[x1, y1, z0] * xd
+            c11 = data[x0, y1, z1] * (1 - xd) + data[x1, y1, z1] * xd
+            c0 = c00 * (1 - yd) + c10 * yd
+            c1 = c01 * (1 - yd) + c11 * yd
+            result[i] = c0 * (1 - zd) + c1 * zd
+    return result
+
+
+def interpolate1D(xval, data, min_x, delta_x):
+    """Vectorised interp1D.py:13-36 (out-of-range -> 0)."""
+    u = (np.asarray(xval, dtype=np.float64) - min_x) / delta_x
+    x0, x1, xd, ok = _trunc_index(u, data.shape[0])
+    return np.where(ok, data[x0] * (1 - xd) + data[x1] * xd, 0.0)
+
+
+# ---------------------------------------------------------------------------
+# a12  lattice tables   (lattice.py:4-110, 136-173)
+# ---------------------------------------------------------------------------
+def reference_trajectory(elements, Nsample=2000):
+    """Reference orbit tables; ``elements`` is the ordered list of element
+    dicts (the lattice YAML without its leading ``step_size`` key).
+
+    Returns s, rho, distance, nsep, coords, n_vec, tau_vec (lattice.py:4-110).
+    """
+    Nel = len(elements)
+    distance = np.zeros(Nel)
+    rho = np.zeros(Nel)
+    nsep = np.zeros(Nel)
+    for k, e in enumerate(elements):
+        L = e["L"]
+        nsep[k] = e["nsep"]
+        if e["type"] == "dipole":
+            rho[k] = e["angle"] / L
+        distance[k] = L if k == 0 else L + distance[k - 1]
+    s = np.linspace(0, distance[-1], Nsample)
+    coords = np.zeros((Nsample, 2))
+    tau = np.zeros((Nsample, 2))
+    nv = np.zeros((Nsample, 2))
+    th = 0.0
+    tau[0] = (np.cos(th), np.sin(th))
+    nv[0] = (np.sin(th), -1 * np.cos(th))
+    ei = 0
+    for cnt in range(1, Nsample):
+        st = s[cnt]
+        if st > distance[ei]:
+            ei += 1
+        ds = s[cnt] - s[cnt - 1]
+        e = elements[ei]
+        if e["type"] == "dipole":
+            L, ang = e["L"], e["angle"]
+            phi = ds / L * ang
+            r = L / ang
+            x0 = coords[cnt - 1, 0] - r * np.sin(th)
+            y0 = coords[cnt - 1, 1] + r * np.cos(th)
+            coords[cnt, 0] = x0 + r * np.sin(phi + th)
+            coords[cnt, 1] = y0 - r * np.cos(phi + th)
+        else:
+            phi = 0
+            coords[cnt, 0] = coords[cnt - 1, 0] + ds * np.cos(th)
+            coords[cnt, 1] = coords[cnt - 1, 1] + ds * np.sin(th)
+        th += phi
+        tau[cnt] = (np.cos(th), np.sin(th))
+        nv[cnt] = (np.sin(th), -1 * np.cos(th))
+    return s, rho, distance, nsep, coords, nv, tau
+
+
+class LatticeTables:
+    """Lattice tables + 1D interpolant axis (lattice.py:136-150)."""
+
+    def __init__(self, elements, Nsample=2000):
+        (self.s, self.rho, self.distance, self.nsep, self.coords, self.n_vec,
+         self.tau_vec) = reference_trajectory(elements, Nsample)
+        self.Nelement = len(elements)
+        self.min_x, self.max_x = self.s[0], self.s[-1]
+        self.delta_x = (self.max_x - self.min_x) / (self.s.shape[0] - 1)
+
+
+# ---------------------------------------------------------------------------
+# a13  get_CSR_integrand   (CSR.py:605-782)
+# ---------------------------------------------------------------------------
+def csr_integrand(s, x, t, sp, xp, lat, hist):
+    """Integrand grids (I_z, I_x) for one observer (CSR.py:605-782).
+
+    ``hist`` is a DFTracker after build_interpolant(); ``lat`` a LatticeTables.
+    """
+    a = hist.axes()
+
+    def i3(f, tv, xv, zv):
+        return interpolate3D(tv, xv, zv, hist.data[f], a["min_t"], a["min_x"], a["min_z"],
+                             a["dt"], a["dx"], a["dz"])
+
+    vx = i3("vx", np.array([t]), np.array([x]), np.array([s - t]))[0]
+    spf = sp.ravel()
+    xpf = xp.ravel()
+
+    def i1(col, v):
+        return interpolate1D(v, col, lat.min_x, lat.delta_x)
+
+    sv = np.array([s])
+    X0_s, X0_sp = i1(lat.coords[:, 0], sv)[0], i1(lat.coords[:, 0], spf)
+    Y0_s, Y0_sp = i1(lat.coords[:, 1], sv)[0], i1(lat.coords[:, 1], spf)
+    nsx, nspx = i1(lat.n_vec[:, 0], sv)[0], i1(lat.n_vec[:, 0], spf)
+    nsy, nspy = i1(lat.n_vec[:, 1], sv)[0], i1(lat.n_vec[:, 1], spf)
+    tsx, tspx = i1(lat.tau_vec[:, 0], sv)[0], i1(lat.tau_vec[:, 0], spf)
+    tsy, tspy = i1(lat.tau_vec[:, 1], sv)[0], i1(lat.tau_vec[:, 1], spf)
+
+    rx = X0_s - X0_sp + x * nsx - xpf * nspx
+    ry = Y0_s - Y0_sp + x * nsy - xpf * nspy
+    R = np.sqrt(rx ** 2 + ry ** 2)
+
+    rho_sp = np.zeros(spf.shape)                     # CSR.py:651-656
+    for k in range(lat.Nelement):
+        if k == 0:
+            rho_sp[spf < lat.distance[k]] = lat.rho[k]
+        else:
+            rho_sp[(spf < lat.distance[k]) & (spf >= lat.distance[k - 1])] = lat.rho[k]
+
+    t_ret = t - R
+    zq = spf - t_ret
+    dens = i3("density", t_ret, xpf, zq)
+    dens_x = i3("density_x", t_ret, xpf, zq)
+    dens_z = i3("density_z", t_ret, xpf, zq)
+    vx_ret = i3("vx", t_ret, xpf, zq)
+    vx_x_ret = i3("vx_x", t_ret, xpf, zq)
+
+    vs, vs_ret, vs_s_ret = 1, 1, 0
+    scale = 1 + xpf * rho_sp
+    vel_x = vs * tsx + vx * nsx
+    vel_y = vs * tsy + vx * nsy
+    vr_x = vs_ret * tspx + vx_ret * nspx
+    vr_y = vs_ret * tspy + vx_ret * nspy
+    with np.errstate(invalid="ignore", divide="ignore"):
+        nab_x = dens_x * nspx + dens_z / scale * tspx
+        nab_y = dens_x * nspy + dens_z / scale * tspy
+        div_v = vs_s_ret + vx_x_ret
+        part1 = vel_x * vr_x + vel_y * vr_y
+        num1 = scale * ((vel_x - part1 * vr_x) * nab_x + (vel_y - part1 * vr_y) * nab_y)
+        num2 = -scale * part1 * dens * div_v
+        Iz = num1 / R + num2 / R
+        nmx = nsx - nspx
+        nmy = nsy - nspy
+        p1 = rx * nmx + ry * nmy
+        p2 = nsx * tspx + nsy * tspy
+        pd = -(vr_x * nab_x + vr_y * nab_y) - dens * div_v
+        W1 = scale * p1 / (R * R * R) * dens
+        W2 = scale * p1 / (R * R) * pd
+        W3 = -scale * p2 / R * pd
+        Ix = W1 + W2 + W3
+    return Iz.reshape(xp.shape), Ix.reshape(xp.shape)
+
+
+# ---------------------------------------------------------------------------
+# a14  get_CSR_wake   (CSR.py:454-602)
+# ---------------------------------------------------------------------------
+def wake_regions(s, x, t, sigma_z, sigma_x, slope0, xmean, formation_length,
+                 n_formation_length, Nx, Nz):
+    """Integration regions (xp axis, sp axis) for one observer (CSR.py:456-553).
+
+    Returns a list of (xp_1d, sp_1d) in the reference's summation order.
+    """
+    tan_theta = slope0
+    x0 = (s - t) * slope0
+    if np.abs(tan_theta) <= 1:
+        s2 = s - 500 * sigma_z
+        s3 = s - 20 * sigma_z
+        s4 = s + 5 * sigma_z
+        s1 = np.max((0, s2 - n_formation_length * formation_length))
+        sp1 = np.linspace(s1, s2, Nz)
+        sp2 = np.linspace(s2, s3, Nz)
+        sp3 = np.linspace(s3, s4, Nz)
+        xp_w = np.linspace(x0 - 20 * sigma_x, x0 + 20 * sigma_x, 2 * Nx)
+        xp_n = np.linspace(x0 - 10 * sigma_x, x0 + 10 * sigma_x, Nx)
+        return [(xp_w, sp1), (xp_n, sp2), (xp_n, sp3)]
+    if tan_theta > 0:
+        tan_alpha = -2 * tan_theta / (1 - tan_theta ** 2)
+        d = (10 * sigma_x + xmean - x) / tan_alpha
+        x1_l, x1_r = x + 0.1 * sigma_x, x + 10 * sigma_x
+        x2_l, x2_r = x - 3 * sigma_x, x1_l
+    else:
+        tan_alpha = 2 * tan_theta / (1 - tan_theta ** 2)
+        d = -(xmean - x - 10 * sigma_x) / tan_alpha
+        x1_l, x1_r = x - 10 * sigma_x, x - 1 * sigma_x
+        x2_l, x2_r = x1_r, x + 3 * sigma_x
+    s4 = s + 3 * sigma_z
+    s3 = np.max((0, s - d))
+    s2 = s3 - 200 * sigma_z
+    x3_l, x3_r = x0 - 5 * sigma_x, x0 + 5 * sigma_x
+    x4_l, x4_r = x0 - 20 * sigma_x, x0 + 20 * sigma_x
+    s1 = np.max((0, s2 - n_formation_length * formation_length))
+    sp1 = np.linspace(s1, s2, Nz)
+    sp2 = np.linspace(s2, s3, Nz)
+    sp3 = np.linspace(s3, s4, Nz)
+    xp1 = np.linspace(x1_l, x1_r, Nx)
+    xp2 = np.linspace(x2_l, x2_r, Nx)
+    xp3 = np.linspace(x3_l, x3_r, Nx)
+    xp4 = np.linspace(x4_l, x4_r, 2 * Nx)
+    return [(xp4, sp1), (xp3, sp2), (xp1, sp3), (xp2, sp3)]
+
+
+def csr_wake(s, x, t, beam, formation_length, n_formation_length, Nx, Nz, lat, hist,
+             csr_scaling):
+    """(dE_dct, x_kick) at one observer: CSR.py:454-602 with nested trapz."""
+    regs = wake_regions(s, x, t, beam["sigma_z"], beam["sigma_x"], beam["slope"][0],
+                        beam["mean_x"], formation_length, n_formation_length, Nx, Nz)
+    dE = 0.0
+    xk = 0.0
+    for xp1, sp1 in regs:
+        xpm, spm = np.meshgrid(xp1, sp1, indexing="ij")
+        Iz, Ix = csr_integrand(s, x, t, spm, xpm, lat, hist)
+        dE = dE + (-csr_scaling * _trapz(_trapz(Iz, xp1, axis=0), sp1))
+        xk = xk + csr_scaling * _trapz(_trapz(Ix, xp1, axis=0), sp1)
+    return dE, xk
+
+
+# ---------------------------------------------------------------------------
+# a2  get_CSR_mesh (CSR.py:361-394)   a15  calculate_2D_CSR (CSR.py:397-418)
+# ---------------------------------------------------------------------------
+def csr_mesh(x, z, xbins, zbins, xlim, zlim):
+    x = np.asarray(x, dtype=np.float64)
+    z = np.asarray(z, dtype=np.float64)
+    p = np.polyfit(z, x, deg=1)
+    xt = x - np.polyval(p, z)
+    sig_x = np.std(xt)
+    mean_x = np.mean(xt)
+    sig_z = np.std(z)
+    mean_z = np.mean(z)
+    zrange = np.linspace(mean_z - zlim * sig_z, mean_z + zlim * sig_z, zbins)
+    xrange = np.linspace(mean_x - xlim * sig_x, mean_x + xlim * sig_x, xbins)
+    xm_t, zm = np.meshgrid(xrange, zrange, indexing="ij")
+    xm_t = xm_t.flatten()
+    zm = zm.flatten()
+    xmesh = xm_t + np.polyval(p, zm)
+    return dict(xmesh=xmesh, zmesh=zm, zrange=zrange, xrange=xrange, slope=p)
+
+
+def calculate_2D_CSR(position, mesh, beam, formation_length, n_formation_length, Nx, Nz,
+                     lat, hist, csr_scaling, xbins, zbins, rows=None):
+    """Loop over mesh points (CSR.py:397-418).  ``rows`` restricts the loop
+    to a subset of flattened mesh indices (others are left as NaN)."""
+    Mtot = xbins * zbins
+    dE = np.full(Mtot, np.nan)
+    xk = np.full(Mtot, np.nan)
+    idx = range(Mtot) if rows is None else rows
+    for i in idx:
+        s = position + mesh["zmesh"][i]
+        x = mesh["xmesh"][i]
+        dE[i], xk[i] = csr_wake(s, x, position, beam, formation_length, n_formation_length,
+                                Nx, Nz, lat, hist, csr_scaling)
+    return dE.reshape(xbins, zbins), xk.reshape(xbins, zbins)
+
+
+# ---------------------------------------------------------------------------
+# a16  Beam.apply_wakes   (beams.py:108-131)
+# ---------------------------------------------------------------------------
+def apply_wakes(x, z, px, pz, dE_dct, x_kick, xrange, zrange, step_size, init_energy,
+                transverse_on):
+    """Returns new (px, pz) after the CSR kick (beams.py:108-131)."""
+    x = np.asarray(x, dtype=np.float64)
+    z = np.asarray(z, dtype=np.float64)
+    p = np.polyfit(z, x, deg=1)
+    xt = x - np.polyval(p, z)
+    dE_E1 = step_size * dE_dct * 1e6 / init_energy
+    f = RegularGridInterpolator((xrange, zrange), dE_E1, fill_value=0.0, bounds_error=False)
+    pz_new = pz + f(np.array([xt, z]).T)
+    if transverse_on:
+        dxp = step_size * x_kick * 1e6 / init_energy
+        f = RegularGridInterpolator((xrange, zrange), dxp, fill_value=0.0, bounds_error=False)
+        px_new = px + f(np.array([xt, z]).T)
+    else:
+        px_new = np.array(px, dtype=np.float64, copy=True)
+    return px_new, pz_new
+
+
+def formation_length_bend(R, sigma_z):
+    """CSR.py:140-142 (called with sigma_z = 5 * beam sigma_z, CSR.py:256,265)."""
+    return (24 * (R ** 2) * sigma_z) ** (1 / 3)

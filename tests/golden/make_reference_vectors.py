@@ -1,0 +1,271 @@
+"""Write golden input/output vectors from the reference's own hot-path functions.
+
+Run ONLY in the build container, where the reference source tree is mounted:
+
+    python tests/golden/make_reference_vectors.py /root/reference/pyDFCSR_2D
+
+The reference package cannot be imported as shipped (numba, bmadx, mpi4py,
+h5py, distgen and pmd_beamphysics are absent).  The functions captured here
+are plain numpy/scipy code once numba's decorators are identities, so this
+script registers import-only placeholders for those absent modules before
+loading the reference modules from their files.  None of the placeholders is
+called by a captured function except numba's ``jit``/``jitclass``/``prange``
+(identity / ``range``), whose pure-Python semantics equal the JIT's for these
+functions on finite inputs.
+
+Nothing of the reference travels: the outputs are plain ``.npz`` arrays
+(inputs and expected outputs) under ``tests/golden/``.
+"""
+from __future__ import annotations
+
+import collections
+import importlib.util
+import os
+import sys
+import tempfile
+import types
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _install_placeholders():
+    def jit(*a, **k):
+        if a and callable(a[0]):
+            return a[0]
+        return lambda f: f
+
+    numba = types.ModuleType("numba")
+    numba.jit = jit
+    numba.njit = jit
+    numba.prange = range
+    class _Type:                      # numba type objects are only subscripted in specs
+        def __getitem__(self, item):
+            return self
+
+    numba.double = _Type()
+    exp = types.ModuleType("numba.experimental")
+    exp.jitclass = lambda spec=None: (lambda c: c)
+    numba.experimental = exp
+    sys.modules["numba"] = numba
+    sys.modules["numba.experimental"] = exp
+
+    bmadx = types.ModuleType("bmadx")
+    bmadx.Particle = collections.namedtuple("Particle", "x px y py z pz s p0c mc2")
+    bmadx.M_ELECTRON = 510998.95
+    for name in ("Drift", "SBend", "Quadrupole", "Sextupole", "track_element"):
+        setattr(bmadx, name, None)
+    bmadx.__path__ = []
+    consts = types.ModuleType("bmadx.constants")
+    consts.C_LIGHT, consts.M_ELECTRON, consts.E_CHARGE = 299792458.0, 510998.95, 1.602176634e-19
+    bmadx.constants = consts
+    sys.modules["bmadx"] = bmadx
+    sys.modules["bmadx.constants"] = consts
+    for name in ("mpi4py", "h5py", "distgen", "pmd_beamphysics"):
+        m = types.ModuleType(name)
+        sys.modules[name] = m
+    sys.modules["mpi4py"].MPI = None
+    sys.modules["distgen"].Generator = None
+    sys.modules["pmd_beamphysics"].ParticleGroup = None
+
+
+def _load_reference(ref_dir):
+    _install_placeholders()
+    pkg = types.ModuleType("pyDFCSR_2D")
+    pkg.__path__ = [ref_dir]
+    sys.modules["pyDFCSR_2D"] = pkg
+    # beams.py imports the openPMD converters of interfaces.py (bmadx/torch
+    # glue for beam I/O); none of the captured functions uses them.
+    itf = types.ModuleType("pyDFCSR_2D.interfaces")
+    itf.openpmd_to_bmadx_particles = itf.bmadx_particles_to_openpmd = None
+    sys.modules["pyDFCSR_2D.interfaces"] = itf
+    mods = {}
+    for name in ("tools", "yaml_parser", "params", "interp1D", "interp3D", "deposit",
+                 "lattice", "twiss", "r_gen6", "twiss_R", "physical_constants",
+                 "beams", "CSR"):
+        spec = importlib.util.spec_from_file_location(f"pyDFCSR_2D.{name}",
+                                                      os.path.join(ref_dir, f"{name}.py"))
+        mod = importlib.util.module_from_spec(spec)
+        sys.modules[f"pyDFCSR_2D.{name}"] = mod
+        spec.loader.exec_module(mod)
+        mods[name] = mod
+    return mods
+
+
+LATTICE_YAML = """step_size: 0.05
+D0:
+  type: drift
+  L: 0.1
+  nsep: 1
+B1:
+  type: dipole
+  L: 0.3
+  angle: 0.05
+  nsep: 1
+D1:
+  type: drift
+  L: 0.4
+  nsep: 2
+B2:
+  type: dipole
+  L: 0.3
+  angle: -0.05
+  nsep: 1
+"""
+
+
+def make_beam(rng, n, sx, sz, chirp=0.0, corr=0.0):
+    z = rng.normal(0.0, sz, n)
+    x = rng.normal(0.0, sx, n) + corr * z
+    px = rng.normal(0.0, 1e-4, n) + 0.02 * x
+    pz = chirp * z + rng.normal(0.0, 1e-4, n)
+    return x, px, z, pz
+
+
+def main(ref_dir):
+    R = _load_reference(ref_dir)
+    dep, i3, i1, lat_mod, CSR, params, beams = (R["deposit"], R["interp3D"], R["interp1D"],
+                                                R["lattice"], R["CSR"], R["params"], R["beams"])
+    rng = np.random.default_rng(12345)
+    out = {}
+
+    # a3 histogram_cic_2d (deposit.py:42-87), particles straddling the grid edges
+    q1 = rng.normal(0.0, 1.0, 3000)
+    q2 = rng.normal(0.0, 2.0, 3000)
+    w = rng.normal(0.5, 0.3, 3000)
+    h1 = dep.histogram_cic_2d(q1, q2, np.ones(q1.shape), 16, -2.5, 2.5, 12, -5.0, 5.0)
+    h2 = dep.histogram_cic_2d(q1, q2, w, 16, -2.5, 2.5, 12, -5.0, 5.0)
+    np.savez(os.path.join(HERE, "ref_histogram_cic_2d.npz"), q1=q1, q2=q2, w=w,
+             args=np.array([16, -2.5, 2.5, 12, -5.0, 5.0]), h_ones=h1, h_w=h2)
+
+    # a11 interpolate3D (interp3D.py:18-66) incl. truncation / clamp / out-of-range
+    data = rng.normal(size=(5, 7, 9))
+    mins = np.array([0.1, -0.3, 0.2])
+    dels = np.array([0.5, 0.25, 0.125])
+    sizes = np.array(data.shape)
+    u = rng.uniform(-1.6, sizes[None, :] + 0.6, size=(600, 3))
+    u[:40, 0] = rng.uniform(-0.99, -0.01, 40)          # (-1, 0): negative fraction
+    u[40:80, 1] = rng.uniform(sizes[1] - 1, sizes[1] - 0.01, 40)   # clamp plane
+    vals = mins[None, :] + u * dels[None, :]
+    r3 = i3.interpolate3D(vals[:, 0].copy(), vals[:, 1].copy(), vals[:, 2].copy(), data,
+                          mins[0], mins[1], mins[2], dels[0], dels[1], dels[2])
+    d1 = rng.normal(size=23)
+    u1 = rng.uniform(-1.5, 23.5, 300)
+    v1 = -0.4 + u1 * 0.05
+    r1 = i1.interpolate1D(v1.copy(), d1, -0.4, 0.05)
+    np.savez(os.path.join(HERE, "ref_interpolate.npz"), data3=data, mins=mins, dels=dels,
+             q3=vals, r3=r3, data1=d1, q1=v1, r1=r1, min1=-0.4, del1=0.05)
+
+    # a12 lattice tables (lattice.py:4-173)
+    tmpdir = tempfile.mkdtemp()
+    lat_path = os.path.join(tmpdir, "lat.yaml")
+    with open(lat_path, "w") as f:
+        f.write(LATTICE_YAML)
+    lat = lat_mod.Lattice({"lattice_input_file": lat_path})
+    np.savez(os.path.join(HERE, "ref_lattice.npz"), s=lat.s, rho=lat.rho, distance=lat.distance,
+             nsep=lat.nsep, coords=lat.coords, n_vec=lat.n_vec, tau_vec=lat.tau_vec,
+             positions_record=lat._positions_record, steps_per_element=lat.steps_per_element,
+             min_x=lat.min_x, delta_x=lat.delta_x)
+
+    # a4 get_DF (deposit.py:145-245): correlated beam (configured grid) and
+    # uncorrelated beam (100x100 / window 5 fallback)
+    cfg = dict(xbins=40, zbins=48, xlim=5, zlim=5, filter_order=2, filter_window=7,
+               velocity_threhold=1000)
+    for tag, corr in (("corr", 4.0), ("fallback", 0.0)):
+        x, px, z, pz = make_beam(rng, 12000, 30e-6, 60e-6, corr=corr)
+        tr = dep.DF_tracker(dict(cfg))
+        tr.get_DF(x=x, z=z, px=px, t=0.25)
+        np.savez(os.path.join(HERE, f"ref_get_df_{tag}.npz"), x=x, z=z, px=px, t=0.25,
+                 cfg_xbins=40, cfg_zbins=48, cfg_order=2, cfg_window=7, cfg_vt=1000.0,
+                 x_grids=tr.x_grids, z_grids=tr.z_grids, density=tr.density, vx=tr.vx,
+                 density_x=tr.density_x, density_z=tr.density_z, vx_x=tr.vx_x)
+
+    # a8-a10 history: 7 slices with a sigma jump (forces a full re-grid), a
+    # window pop, capped interp grid (upper_limit) to keep the fixture small
+    tr = dep.DF_tracker(dict(cfg, upper_limit=40))
+    hist_inputs = []
+    times = [0.0, 0.05, 0.1, 0.15, 0.2, 0.25, 0.3]
+    sig = [30e-6, 31e-6, 32e-6, 90e-6, 91e-6, 92e-6, 93e-6]
+    fls = [float("inf"), 0.2, 0.2, 0.2, 0.06, 0.06, 0.06]
+    regrid_flags = []
+    for k, (t, sx, fl) in enumerate(zip(times, sig, fls)):
+        x, px, z, pz = make_beam(rng, 3000, sx, 50e-6 + 2e-6 * k, corr=0.5)
+        hist_inputs.append(np.stack([x, z, px]))
+        tr.get_DF(x=x, z=z, px=px, t=t)
+        tr.append_DF()
+        grid_before = tr.x_grid_interp
+        tr.append_interpolant(formation_length=fl, n_formation_length=1.0)
+        regrid_flags.append(int(tr.x_grid_interp is not grid_before))   # new interp grid
+        if k >= 1:          # one slice: deposit.py:419 divides by zero
+            tr.build_interpolant()
+    np.savez(os.path.join(HERE, "ref_history.npz"), inputs=np.stack(hist_inputs),
+             times=np.array(times), fls=np.array(fls), n_fl=1.0,
+             cfg_xbins=40, cfg_zbins=48, cfg_order=2, cfg_window=7, cfg_vt=1000.0,
+             upper_limit=40, time_interp=np.array(tr.time_interp),
+             regrid_flags=np.array(regrid_flags),
+             x_grid_interp=tr.x_grid_interp, z_grid_interp=tr.z_grid_interp,
+             axes=np.array([tr.min_x, tr.min_y, tr.min_z, tr.delta_x, tr.delta_y, tr.delta_z]),
+             density=tr.data_density_interp, density_x=tr.data_density_x_interp,
+             density_z=tr.data_density_z_interp, vx=tr.data_vx_interp,
+             vx_x=tr.data_vx_x_interp)
+
+    # a13-a15 integrand, wake and mesh loop (CSR.py:361-782) on that history
+    class _Beam:
+        pass
+
+    xl, pxl, zl, pzl = hist_inputs[-1][0], hist_inputs[-1][2], hist_inputs[-1][1], None
+    obj = CSR.CSR2D.__new__(CSR.CSR2D)
+    obj.DF_tracker = tr
+    obj.lattice = lat
+    obj.integration_params = params.Integration_params({"n_formation_length": 1.0,
+                                                        "zbins": 16, "xbins": 12})
+    obj.CSR_params = params.CSR_params({"xbins": 3, "zbins": 4, "xlim": 3, "zlim": 3,
+                                        "workdir": tmpdir})
+    obj.CSR_scaling = 8.98755e3 * 1e-9
+    obj.formation_length = 0.05
+    Bm = beams.Beam.__new__(beams.Beam)
+    Particle = sys.modules["bmadx"].Particle
+    Bm.particle = Particle(xl, pxl, np.zeros_like(xl), np.zeros_like(xl), zl,
+                           np.zeros_like(xl), 0.0, 2e9, 510998.95)
+    Bm._init_energy = 2e9
+    Bm.position = 0.3
+    Bm.update_status()
+    obj.beam = Bm
+    obj.get_CSR_mesh()
+    obj.calculate_2D_CSR()
+    chirp_runs = []
+    for slope_override in (2.5, -2.5):
+        Bm._slope = np.array([slope_override, Bm._slope[1]])
+        res = [obj.get_CSR_wake(0.3 + obj.CSR_zmesh[i], obj.CSR_xmesh[i]) for i in (0, 5, 11)]
+        chirp_runs.append(np.array(res))
+    Bm.update_status()
+    dbg = obj.get_CSR_wake(0.3 + obj.CSR_zmesh[6], obj.CSR_xmesh[6], debug=True)
+    np.savez(os.path.join(HERE, "ref_wake.npz"),
+             beam_x=xl, beam_z=zl, beam_px=pxl, position=0.3,
+             sigma_x=Bm._sigma_x, sigma_z=Bm._sigma_z, slope=Bm._slope, mean_x=Bm._mean_x,
+             mean_z=Bm._mean_z, formation_length=0.05, n_fl=1.0, Nx=12, Nz=16,
+             csr_scaling=obj.CSR_scaling, mesh_xbins=3, mesh_zbins=4, mesh_xlim=3, mesh_zlim=3,
+             xmesh=obj.CSR_xmesh, zmesh=obj.CSR_zmesh, zrange=obj.CSR_zrange,
+             xrange=obj.CSR_xrange_transformed, dE_dct=obj.dE_dct, x_kick=obj.x_kick,
+             chirp_pos=chirp_runs[0], chirp_neg=chirp_runs[1], chirp_rows=np.array([0, 5, 11]),
+             dbg_xp_w=dbg[0], dbg_xp_n=dbg[1], dbg_sp1=dbg[2], dbg_sp2=dbg[3], dbg_sp3=dbg[4],
+             dbg_Iz1=dbg[5], dbg_Ix1=dbg[6], dbg_Iz2=dbg[7], dbg_Ix2=dbg[8],
+             dbg_Iz3=dbg[9], dbg_Ix3=dbg[10])
+
+    # a16 apply_wakes (beams.py:108-131)
+    pz0 = rng.normal(0, 1e-4, xl.shape[0])
+    px0 = np.array(pxl)
+    Bm.particle = Particle(xl, px0, np.zeros_like(xl), np.zeros_like(xl), zl, pz0, 0.0, 2e9,
+                           510998.95)
+    Bm.apply_wakes(obj.dE_dct, obj.x_kick, obj.CSR_xrange_transformed, obj.CSR_zrange,
+                   0.05, 1)
+    np.savez(os.path.join(HERE, "ref_apply_wakes.npz"), x=xl, z=zl, px0=px0, pz0=pz0,
+             dE_dct=obj.dE_dct, x_kick=obj.x_kick, xrange=obj.CSR_xrange_transformed,
+             zrange=obj.CSR_zrange, step_size=0.05, energy=2e9, px1=Bm.particle.px,
+             pz1=Bm.particle.pz)
+    print("golden vectors written to", HERE)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference/pyDFCSR_2D")

@@ -1,0 +1,205 @@
+"""Pins the CPU oracle against mathematics that fixes its answers.
+
+Closed forms, invariants, brute force on tiny inputs and library routines,
+each chosen so that a plausible slip (a dropped term, a wrong sign or index,
+a transposed operand) fails at least one of them.
+"""
+import numpy as np
+import pytest
+from scipy.interpolate import RegularGridInterpolator
+
+from oracle import cpu_ref as O
+
+
+# -- a3 CIC deposition ------------------------------------------------------
+def test_cic_particle_on_node_single_cell():
+    # grid 4 x 5 cells over [0, 4) x [0, 10): a particle exactly on a node
+    h = O.histogram_cic_2d(np.array([2.0]), np.array([4.0]), np.array([3.0]), 4, 0.0, 4.0, 5, 0.0, 10.0)
+    exp = np.zeros((4, 5))
+    exp[2, 2] = 3.0
+    assert np.array_equal(h, exp)
+
+
+def test_cic_off_node_split_and_edges():
+    # cell (1.25, 0.5) -> S1 = 0.75, S2 = 0.5
+    h = O.histogram_cic_2d(np.array([1.25]), np.array([1.0]), np.array([2.0]), 4, 0.0, 4.0, 5, 0.0, 10.0)
+    exp = np.zeros((4, 5))
+    exp[1, 0] = 2 * 0.75 * 0.5
+    exp[1, 1] = 2 * 0.75 * 0.5
+    exp[2, 0] = 2 * 0.25 * 0.5
+    exp[2, 1] = 2 * 0.25 * 0.5
+    assert np.allclose(h, exp, rtol=0, atol=1e-15)
+    # particle in the last cell loses its (i+1) neighbours; one below the
+    # grid keeps only its upper neighbour
+    h = O.histogram_cic_2d(np.array([3.5, -0.25]), np.array([4.0, 4.0]), np.array([1.0, 1.0]),
+                           4, 0.0, 4.0, 5, 0.0, 10.0)
+    assert h[3, 2] == 0.5 and h[0, 2] == 0.75 and np.isclose(h.sum(), 1.25)
+
+
+def test_cic_total_weight_and_loop_form():
+    rng = np.random.default_rng(1)
+    q1 = rng.uniform(0.2, 2.95, 500)      # i+1 stays on the grid
+    q2 = rng.uniform(0.3, 7.9, 500)
+    w = rng.normal(size=500)
+    h = O.histogram_cic_2d(q1, q2, w, 4, 0.0, 4.0, 5, 0.0, 10.0)
+    # interior particles keep all their weight
+    assert np.isclose(h.sum(), w.sum(), rtol=1e-12)
+    assert np.array_equal(h, O.histogram_cic_2d_loop(q1, q2, w, 4, 0.0, 4.0, 5, 0.0, 10.0))
+    i1, i2, _, _ = O.cic_cells(q1, q2, 4, 0.0, 4.0, 5, 0.0, 10.0)
+    assert np.array_equal(i1, np.floor(q1).astype(int)) and np.array_equal(i2, np.floor(q2 / 2).astype(int))
+    c = O.cell_counts(i1, i2, 4, 5)
+    assert c.sum() == 500
+
+
+# -- a11 / a12 interpolation --------------------------------------------------
+def test_interpolate3d_exact_on_trilinear():
+    t = np.linspace(0.0, 2.0, 5)
+    x = np.linspace(-1.0, 1.0, 7)
+    z = np.linspace(0.5, 3.5, 9)
+    T, X, Z = np.meshgrid(t, x, z, indexing="ij")
+    f = lambda a, b, c: 1.5 + 2 * a - 3 * b + 0.5 * c + a * b - 2 * b * c + 0.25 * a * b * c
+    data = f(T, X, Z)
+    rng = np.random.default_rng(2)
+    q = np.stack([rng.uniform(0, 2, 200), rng.uniform(-1, 1, 200), rng.uniform(0.5, 3.5, 200)])
+    r = O.interpolate3D(*q, data, 0.0, -1.0, 0.5, 0.5, 1 / 3, 0.375)
+    assert np.allclose(r, f(*q), rtol=0, atol=1e-12)
+
+
+def test_interpolate3d_truncation_clamp_and_zero():
+    data = np.arange(2 * 3 * 4, dtype=float).reshape(2, 3, 4)
+    mn = (0.0, 0.0, 0.0)
+    dl = (1.0, 1.0, 1.0)
+    # u in (-1, 0): trunc -> 0, negative fraction extrapolates
+    r = O.interpolate3D(np.array([-0.5]), np.array([1.0]), np.array([1.0]), data, *mn, *dl)
+    assert np.isclose(r[0], data[0, 1, 1] - 0.5 * (data[1, 1, 1] - data[0, 1, 1]))
+    # u in [size-1, size): clamps to the last plane
+    r = O.interpolate3D(np.array([1.7]), np.array([1.0]), np.array([1.0]), data, *mn, *dl)
+    assert np.isclose(r[0], data[1, 1, 1])
+    # u <= -1 or u >= size: zero
+    r = O.interpolate3D(np.array([-1.0, 2.0, 0.5]), np.array([1.0, 1.0, 3.0]),
+                        np.array([1.0, 1.0, 1.0]), data, *mn, *dl)
+    assert np.array_equal(r, np.zeros(3))
+    # NaN query: zero
+    assert O.interpolate3D(np.array([np.nan]), np.array([1.0]), np.array([1.0]), data, *mn, *dl)[0] == 0
+    q = np.random.default_rng(3).uniform(-1.5, 4.5, size=(3, 300))
+    assert np.array_equal(O.interpolate3D(*q, data, *mn, *dl), O.interpolate3D_loop(*q, data, *mn, *dl))
+
+
+def test_interpolate1d_linear_and_out_of_range():
+    d = 3.0 - 2.0 * np.linspace(0, 1, 11)
+    xs = np.array([-0.2, 0.0, 0.33, 0.999, 1.0, 1.09, 1.1, -0.05, -0.1])
+    r = O.interpolate1D(xs, d, 0.0, 0.1)
+    exp = np.array([0.0, 3.0, 3 - 0.66, 3 - 1.998, 1.0, 1.0, 0.0, 3.0 + 0.1, 0.0])
+    assert np.allclose(r, exp, atol=1e-12)
+
+
+# -- a12 lattice tables ------------------------------------------------------
+def test_reference_trajectory_closed_form():
+    R = 6.0
+    ang = 0.3 / R
+    els = [dict(type="dipole", L=0.3, angle=ang, nsep=1), dict(type="drift", L=0.5, nsep=1)]
+    s, rho, dist, nsep, coords, nv, tau = O.reference_trajectory(els, Nsample=401)
+    assert np.allclose(rho, [1 / R, 0.0]) and np.allclose(dist, [0.3, 0.8])
+    inb = s <= 0.3 - 1e-12
+    th = s[inb] / R
+    # arc of a leading dipole: x = R sin(th), y = R (1 - cos th)
+    assert np.allclose(coords[inb, 0], R * np.sin(th), atol=1e-12)
+    assert np.allclose(coords[inb, 1], R * (1 - np.cos(th)), atol=1e-12)
+    assert np.allclose(np.hypot(*tau.T), 1) and np.allclose(np.hypot(*nv.T), 1)
+    assert np.allclose((tau * nv).sum(1), 0, atol=1e-15)
+    # n = tau rotated by -90 degrees
+    assert np.allclose(nv[:, 0], tau[:, 1]) and np.allclose(nv[:, 1], -tau[:, 0])
+
+
+# -- a5 / a6 library composition (savgol 'interp', gradient) -----------------
+def test_get_df_uniform_threshold_quirks():
+    rng = np.random.default_rng(4)
+    n = 20000
+    z = rng.normal(0, 50e-6, n)
+    x = rng.normal(0, 30e-6, n)
+    px = 0.01 * x
+    r = O.get_DF(x, z, px, 0.0, xbins=32, zbins=32, filter_order=2, filter_window=5,
+                 velocity_threhold=1000)
+    assert r["nbins"] == (100, 100)          # frac <= 5 fallback (deposit.py:160-167)
+    dsum = np.trapezoid(np.trapezoid(r["density"], r["x_grids"], axis=0), r["z_grids"])
+    assert np.isclose(dsum, 1.0, rtol=1e-12)
+    # vx = px/x ratio -> 0.01 * x where density is significant
+    core = r["density"] > 0.5 * r["density"].max()
+    X = np.broadcast_to(r["x_grids"][:, None], r["density"].shape)
+    assert np.allclose(r["vx"][core], 0.01 * X[core], rtol=0.05, atol=2e-8)
+
+
+# -- a16 kick ------------------------------------------------------------------
+def test_apply_wakes_bilinear_closed_form():
+    xr = np.linspace(-1, 1, 5)
+    zr = np.linspace(-2, 2, 7)
+    X, Z = np.meshgrid(xr, zr, indexing="ij")
+    dE = 3 + X - 2 * Z + 0.5 * X * Z
+    xk = -1 + 2 * X + Z
+    rng = np.random.default_rng(5)
+    z = rng.uniform(-2.5, 2.5, 300)
+    x = 0.3 * z + rng.uniform(-1.2, 1.2, 300)
+    px0 = np.zeros(300)
+    pz0 = np.zeros(300)
+    px1, pz1 = O.apply_wakes(x, z, px0, pz0, dE, xk, xr, zr, 0.1, 1e9, 1)
+    p = np.polyfit(z, x, 1)
+    xt = x - np.polyval(p, z)
+    inside = (np.abs(xt) <= 1) & (np.abs(z) <= 2)
+    c = 0.1 * 1e6 / 1e9
+    assert np.allclose(pz1[inside], c * (3 + xt - 2 * z + 0.5 * xt * z)[inside], rtol=1e-12, atol=1e-18)
+    assert np.allclose(px1[inside], c * (-1 + 2 * xt + z)[inside], rtol=1e-12, atol=1e-18)
+    assert np.all(pz1[~inside] == 0) and np.all(px1[~inside] == 0)
+
+
+# -- a13 integrand invariants --------------------------------------------------
+def _toy_history(scale=1.0):
+    T = O.DFTracker(xbins=24, zbins=24, filter_order=2, filter_window=5, velocity_threhold=1000)
+    rng = np.random.default_rng(6)
+    for k, t in enumerate([0.0, 0.05, 0.1]):
+        z = rng.normal(0, 40e-6, 8000)
+        x = rng.normal(0, 20e-6, 8000) + 3.0 * z
+        px = 1e-3 * rng.normal(size=8000)
+        T.get_DF(x, z, px, t)
+        T.append_DF()
+        T.append_interpolant(float("inf") if k == 0 else 1.0, 1.0)
+    T.build_interpolant()
+    if scale != 1.0:
+        for f in ("density", "density_x", "density_z"):
+            T.data[f] = T.data[f] * scale
+    return T
+
+
+def test_integrand_zero_density_and_linearity():
+    L = O.LatticeTables([dict(type="dipole", L=0.5, angle=0.05, nsep=1)])
+    T = _toy_history()
+    s, x, t = 0.1 + 5e-5, 1e-5, 0.1
+    xp, sp = np.meshgrid(np.linspace(-1e-4, 1e-4, 9), np.linspace(0.06, 0.1002, 11), indexing="ij")
+    Iz, Ix = O.csr_integrand(s, x, t, sp, xp, L, T)
+    assert np.all(np.isfinite(Iz)) and np.any(Iz != 0)
+    T2 = _toy_history(scale=2.5)
+    Iz2, Ix2 = O.csr_integrand(s, x, t, sp, xp, L, T2)
+    # linear in the density fields (W1+W2+W3 cancels partly: compare to the max)
+    assert np.abs(Iz2 - 2.5 * Iz).max() <= 1e-12 * np.abs(Iz).max()
+    assert np.abs(Ix2 - 2.5 * Ix).max() <= 1e-12 * np.abs(Ix).max()
+    T0 = _toy_history(scale=0.0)
+    Iz0, Ix0 = O.csr_integrand(s, x, t, sp, xp, L, T0)
+    assert np.all(Iz0 == 0) and np.all(Ix0 == 0)
+
+
+def test_rgi_restatement_matches_scipy():
+    # the product re-grid and kick kernels restate RegularGridInterpolator's
+    # linear rule: interval search + (1-y0)(1-y1) corner weights
+    xg = np.linspace(-1, 1, 9)
+    zg = np.linspace(0, 2, 13)
+    V = np.random.default_rng(7).normal(size=(9, 13))
+    q = np.random.default_rng(8).uniform(-1.2, 2.2, size=(400, 2))
+    ref = RegularGridInterpolator((xg, zg), V, fill_value=0.0, bounds_error=False)(q)
+    i = np.clip(np.searchsorted(xg, q[:, 0], side="right") - 1, 0, 7)
+    j = np.clip(np.searchsorted(zg, q[:, 1], side="right") - 1, 0, 11)
+    y0 = (q[:, 0] - xg[i]) / (xg[i + 1] - xg[i])
+    y1 = (q[:, 1] - zg[j]) / (zg[j + 1] - zg[j])
+    v = (V[i, j] * (1 - y0) * (1 - y1) + V[i, j + 1] * (1 - y0) * y1
+         + V[i + 1, j] * y0 * (1 - y1) + V[i + 1, j + 1] * y0 * y1)
+    oob = (q[:, 0] < xg[0]) | (q[:, 0] > xg[-1]) | (q[:, 1] < zg[0]) | (q[:, 1] > zg[-1])
+    v[oob] = 0.0
+    assert np.array_equal(v, ref)

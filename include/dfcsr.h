@@ -1,0 +1,213 @@
+/* dfcsr.h -- C ABI of the MI355X (gfx950) 2D CSR wake step.
+ *
+ * Every entry point is stream-ordered and asynchronous on the HIP stream it
+ * is given (NULL = the null stream), takes caller-owned DEVICE pointers
+ * (contiguous, C order) and returns 0 on success or a negative DFCSR_E*
+ * code; dfcsr_last_error() then holds a message for the calling thread.
+ * No entry point allocates device memory: scratch comes in as a workspace
+ * pointer whose size the matching *_workspace_bytes() call reports.
+ * Calls that share a workspace must not overlap on different streams.
+ *
+ * Array conventions follow the reference (pyDFCSR, paths relative to
+ * pyDFCSR_2D/): 2D grids are [x_index][z_index]; the wake mesh is
+ * [xbins][zbins] flattened with x outer (CSR.py:384-387).
+ */
+#ifndef DFCSR_H
+#define DFCSR_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DFCSR_ABI_VERSION 1
+
+enum {
+  DFCSR_OK = 0,
+  DFCSR_EINVAL = -1,   /* bad argument (shape, size, NULL pointer) */
+  DFCSR_EHIP = -2,     /* a HIP runtime call failed */
+  DFCSR_ERANGE = -3,   /* a size exceeds a documented maximum */
+  DFCSR_ENOWS = -4     /* workspace too small */
+};
+
+enum { DFCSR_PREC_FP64 = 0, DFCSR_PREC_MIXED = 1 };
+
+/* Number of doubles written by dfcsr_beam_moments. */
+#define DFCSR_MOMENTS_COUNT 10
+/* Order of the five history fields in every slice record. */
+enum { DFCSR_F_DENSITY = 0, DFCSR_F_DENSITY_X = 1, DFCSR_F_DENSITY_Z = 2,
+       DFCSR_F_VX = 3, DFCSR_F_VX_X = 4 };
+/* History record strides: FP64 slices hold 6 doubles per (x, z) cell (five
+ * fields + one pad so a z-neighbour pair is 16-byte aligned), the FP32
+ * shadow holds 8 floats per cell (five fields + three pads). */
+#define DFCSR_HIST_REC64 6
+#define DFCSR_HIST_REC32 8
+#define DFCSR_MAX_ELEMENTS 64
+#define DFCSR_MAX_SG_WINDOW 21
+
+int dfcsr_abi_version(void);
+const char *dfcsr_last_error(void);
+
+/* ---- a1: beam moments -----------------------------------------------------
+ * Replaces the numpy statistics read by the step: Beam.update_status /
+ * sigma_x / sigma_z / mean_x / mean_z / slope / x_transform
+ * (beams.py:88-98, 137-215), the std/mean of x_transform in get_CSR_mesh
+ * (CSR.py:368-374) and the |z| < 0.1 sigma_z slice std of get_DF
+ * (deposit.py:147-159).
+ * out[0..9] = mean_x, mean_z, sigma_x, sigma_z, slope0, slope1,
+ *             mean(x_transform), std(x_transform), slice std(x), slice count.
+ * Population (ddof=0) statistics; slope is the least-squares line x(z)
+ * (np.polyfit(z, x, 1)).  Deterministic (fixed reduction tree). */
+int64_t dfcsr_moments_workspace_bytes(int64_t n);
+int dfcsr_beam_moments(const double *x, const double *z, int64_t n, void *ws, int64_t ws_bytes,
+                       double *out, void *stream);
+
+/* ---- a3: cloud-in-cell deposition ----------------------------------------
+ * Replaces histogram_cic_2d (deposit.py:42-87): hist[i1][i2] += w*S1*S2 over
+ * the four CIC corners with cell = (q - start) * (1/((end - start)/nbins)).
+ * w == NULL means unit weights.  Sums are accumulated as 64-bit fixed point
+ * (scale 2^k, k chosen from n * max|w|) so the result is bit-reproducible.
+ * Optional outputs (NULL to skip): cell_index[2*p + {0,1}] = the particle's
+ * lower cell (i1, i2) (floor, may lie off the grid); cell_count[i1][i2] =
+ * number of particles whose lower cell is (i1, i2) on the grid. */
+int64_t dfcsr_histogram_workspace_bytes(int nbins_1, int nbins_2);
+int dfcsr_histogram_cic_2d(const double *q1, const double *q2, const double *w, int64_t n,
+                           int nbins_1, double bins_start_1, double bins_end_1,
+                           int nbins_2, double bins_start_2, double bins_end_2,
+                           double *hist, int32_t *cell_index, int32_t *cell_count,
+                           void *ws, int64_t ws_bytes, void *stream);
+/* The two depositions of DF_tracker.get_DF in one pass over the particles
+ * (deposit.py:172-182): density (w = 1) and vx sum (w = px). */
+int dfcsr_deposit_density_vx(const double *x, const double *z, const double *px, int64_t n,
+                             int nbins_x, double x_start, double x_end,
+                             int nbins_z, double z_start, double z_end,
+                             double *density, double *vx, int32_t *cell_index,
+                             int32_t *cell_count, void *ws, int64_t ws_bytes, void *stream);
+
+/* ---- a4-a6: density-function post-processing -------------------------------
+ * Replaces deposit.py:183-235 on deposited grids (in place):
+ *   velocity threshold division, separable Savitzky-Golay smoothing
+ *   (scipy.signal.savgol_filter, mode='interp', axis 0 then 1), trapezoid
+ *   normalisation over linspace grids, np.gradient (non-uniform spacing form),
+ *   smoothing of the derivatives and the vx_x fill rule.
+ * density / vx: in = raw deposits, out = final fields; density_x,
+ * density_z, vx_x: out.  Grid axes are np.linspace(x_start, x_end, nbins_x)
+ * and np.linspace(z_start, z_end, nbins_z).  window odd, order < window,
+ * window <= min(nbins) and <= DFCSR_MAX_SG_WINDOW.
+ * info (device, may be NULL) receives {max raw density, dsum, max final
+ * density, vx_x fill value}. */
+int64_t dfcsr_df_workspace_bytes(int nbins_x, int nbins_z);
+int dfcsr_df_process(double *density, double *vx, double *density_x, double *density_z,
+                     double *vx_x, int nbins_x, double x_start, double x_end,
+                     int nbins_z, double z_start, double z_end,
+                     double velocity_threshold, int window, int order, double *info,
+                     void *ws, int64_t ws_bytes, void *stream);
+
+/* ---- a8-a10: history re-grid ------------------------------------------------
+ * Replaces DF_tracker.DF_interp / append_interpolant's re-gridding
+ * (deposit.py:296-309, 328-337, 379-390): bilinear RegularGridInterpolator
+ * (fill 0; vx_x fill = mean(vx_x) of that slice) of slices on their own
+ * deposition grids onto the common interp grid
+ * np.linspace(xi0, xi1, Xi) x np.linspace(zi0, zi1, Zi).
+ * src: raw slices, src_slot_stride doubles apart, each holding the five
+ * fields as [field][nbx][nbz] (field order DFCSR_F_*).
+ * jobs: HOST array of njobs descriptors. */
+typedef struct {
+  int32_t src_slot;   /* raw slice index in src */
+  int32_t dst_slot;   /* ring slot in the history */
+  int32_t nbx, nbz;   /* deposition grid of that slice */
+  double x0, x1, z0, z1;   /* its linspace end points */
+} DfcsrRegridJob;
+int64_t dfcsr_regrid_workspace_bytes(int njobs);
+int dfcsr_regrid(const double *src, int64_t src_slot_stride, const DfcsrRegridJob *jobs,
+                 int njobs, double *hist64, float *hist32, int Xi, int Zi, double xi0,
+                 double xi1, double zi0, double zi1, void *ws, int64_t ws_bytes, void *stream);
+
+/* ---- a11/a12: lookup primitives (exposed for parity tests) -----------------
+ * History descriptor: ring of cap slots, logical slice i lives in slot
+ * (head + i) % cap; axes as DF_tracker.build_interpolant (deposit.py:416-421):
+ * min_t = oldest time, dt = (t_last - t_first)/(nt - 1), min_x = xi0,
+ * dx = (xi1 - xi0)/(Xi - 1), same for z. */
+typedef struct {
+  const double *data;   /* [cap][Xi][Zi][DFCSR_HIST_REC64] */
+  const float *data32;  /* [cap][Xi][Zi][DFCSR_HIST_REC32] or NULL */
+  int32_t cap, head, nt, Xi, Zi;
+  double min_t, min_x, min_z, dt, dx, dz;
+} DfcsrHistory;
+
+/* Lattice tables (lattice.py:4-110, 136-150): table[k][0..5] =
+ * coords_x, coords_y, n_x, n_y, tau_x, tau_y at s = min_s + k*ds, k < ns;
+ * distance[], rho[] per element (device pointers). */
+typedef struct {
+  const double *table;
+  int32_t ns;
+  int32_t nelem;
+  double min_s, ds;
+  const double *distance;
+  const double *rho;
+} DfcsrLattice;
+
+/* interpolate3D (interp3D.py:18-66) on one history field at n points
+ * q[3*p + {0,1,2}] = (t, x, z). */
+int dfcsr_interpolate3d(const DfcsrHistory *hist, int field, const double *q, int64_t n,
+                        double *out, void *stream);
+/* interpolate1D (interp1D.py:13-36) on lattice table column col (0..5). */
+int dfcsr_interpolate1d(const DfcsrLattice *lat, int col, const double *s, int64_t n,
+                        double *out, void *stream);
+
+/* ---- a13-a15: retarded integral on the wake mesh ---------------------------- */
+typedef struct {
+  double t;                      /* beam position (CSR.py:456) */
+  double sigma_x, sigma_z;       /* beam._sigma_x, beam._sigma_z */
+  double slope0, slope1;         /* beam._slope (np.polyfit(z, x, 1)) */
+  double mean_x;                 /* beam._mean_x */
+  double formation_length, n_formation_length;
+  double csr_scaling;            /* 8.98755e3 * charge (CSR.py:80) */
+  int32_t Nx, Nz;                /* CSR_integration xbins, zbins */
+  int32_t mesh_xbins, mesh_zbins;
+  double xr0, xr1, zr0, zr1;     /* linspace ends of xrange, zrange (CSR.py:380-381) */
+  /* Optional DEVICE array [M][2] of observers (s - t, x); when non-NULL it
+   * replaces the mesh formula, e.g. for get_CSR_wake(s, x) at arbitrary
+   * points (CSR.py:454). */
+  const double *observers;
+} DfcsrWakeParams;
+
+/* get_CSR_integrand (CSR.py:605-782) on an explicit (xp, sp) grid of one
+ * observer (s, x, t = p->t): Iz, Ix are [nxp][nsp]. */
+int dfcsr_csr_integrand(const DfcsrHistory *hist, const DfcsrLattice *lat, double s, double x,
+                        double t, const double *xp, int nxp, const double *sp, int nsp,
+                        double *Iz, double *Ix, void *stream);
+
+/* get_CSR_mesh + calculate_2D_CSR + get_CSR_wake (CSR.py:361-602):
+ * xmesh, zmesh (M = mesh_xbins*mesh_zbins, may be NULL), dE_dct, x_kick
+ * ([mesh_xbins][mesh_zbins], MeV/m).  precision: DFCSR_PREC_FP64 or
+ * DFCSR_PREC_MIXED (FP32 integrand with compensated FP32 accumulation;
+ * FP64 retardation geometry and final sum; needs hist->data32). */
+int64_t dfcsr_wake_workspace_bytes(int mesh_points);
+int dfcsr_wake_mesh(const DfcsrHistory *hist, const DfcsrLattice *lat, const DfcsrWakeParams *p,
+                    int precision, double *xmesh, double *zmesh, double *dE_dct, double *x_kick,
+                    void *ws, int64_t ws_bytes, void *stream);
+
+/* Batched form for serving: k independent simulations (each its own history,
+ * lattice and parameters) in one launch.  All descriptor arrays are HOST
+ * arrays of length k; output pointers per simulation. */
+int dfcsr_wake_mesh_batched(int k, const DfcsrHistory *hist, const DfcsrLattice *lat,
+                            const DfcsrWakeParams *p, int precision, double *const *dE_dct,
+                            double *const *x_kick, void *ws, int64_t ws_bytes, void *stream);
+int64_t dfcsr_wake_batched_workspace_bytes(int k);
+
+/* ---- a16: mesh -> particle kick --------------------------------------------
+ * Replaces Beam.apply_wakes (beams.py:108-131): bilinear
+ * RegularGridInterpolator((xrange, zrange), step*W*1e6/E0, fill 0) at
+ * (x - (slope0*z + slope1), z); pz += dE part; px += x part if transverse_on. */
+int dfcsr_apply_wakes(double *px, double *pz, const double *x, const double *z, int64_t n,
+                      const double *dE_dct, const double *x_kick, int xbins, int zbins,
+                      double xr0, double xr1, double zr0, double zr1, double slope0,
+                      double slope1, double step_size, double init_energy, int transverse_on,
+                      void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,162 @@
+"""ctypes binding of libdfcsr.so (the gfx950 C ABI declared in include/dfcsr.h).
+
+The library is loaded from this package directory.  There is no fallback:
+if it is missing or does not load, every entry point raises DfcsrError.
+Device arrays are torch tensors on a HIP device; they cross the boundary as
+raw pointers (``data_ptr()``) together with torch's current HIP stream.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import torch
+
+LIB_NAME = "libdfcsr.so"
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
+
+c_dp = ctypes.c_void_p
+c_i32 = ctypes.c_int32
+c_i64 = ctypes.c_int64
+c_f64 = ctypes.c_double
+
+DFCSR_PREC_FP64 = 0
+DFCSR_PREC_MIXED = 1
+MOMENTS_COUNT = 10
+HIST_REC64 = 6
+HIST_REC32 = 8
+FIELDS = ("density", "density_x", "density_z", "vx", "vx_x")   # DFCSR_F_* order
+
+
+class DfcsrError(RuntimeError):
+    pass
+
+
+class RegridJob(ctypes.Structure):
+    _fields_ = [("src_slot", c_i32), ("dst_slot", c_i32), ("nbx", c_i32), ("nbz", c_i32),
+                ("x0", c_f64), ("x1", c_f64), ("z0", c_f64), ("z1", c_f64)]
+
+
+class History(ctypes.Structure):
+    _fields_ = [("data", c_dp), ("data32", c_dp), ("cap", c_i32), ("head", c_i32),
+                ("nt", c_i32), ("Xi", c_i32), ("Zi", c_i32), ("min_t", c_f64),
+                ("min_x", c_f64), ("min_z", c_f64), ("dt", c_f64), ("dx", c_f64),
+                ("dz", c_f64)]
+
+
+class Lattice(ctypes.Structure):
+    _fields_ = [("table", c_dp), ("ns", c_i32), ("nelem", c_i32), ("min_s", c_f64),
+                ("ds", c_f64), ("distance", c_dp), ("rho", c_dp)]
+
+
+class WakeParams(ctypes.Structure):
+    _fields_ = [("t", c_f64), ("sigma_x", c_f64), ("sigma_z", c_f64), ("slope0", c_f64),
+                ("slope1", c_f64), ("mean_x", c_f64), ("formation_length", c_f64),
+                ("n_formation_length", c_f64), ("csr_scaling", c_f64), ("Nx", c_i32),
+                ("Nz", c_i32), ("mesh_xbins", c_i32), ("mesh_zbins", c_i32), ("xr0", c_f64),
+                ("xr1", c_f64), ("zr0", c_f64), ("zr1", c_f64), ("observers", c_dp)]
+
+
+# (name, restype, argtypes) for every exported symbol of include/dfcsr.h
+_SIGNATURES = [
+    ("dfcsr_abi_version", c_i32, []),
+    ("dfcsr_last_error", ctypes.c_char_p, []),
+    ("dfcsr_moments_workspace_bytes", c_i64, [c_i64]),
+    ("dfcsr_beam_moments", c_i32, [c_dp, c_dp, c_i64, c_dp, c_i64, c_dp, c_dp]),
+    ("dfcsr_histogram_workspace_bytes", c_i64, [c_i32, c_i32]),
+    ("dfcsr_histogram_cic_2d", c_i32, [c_dp, c_dp, c_dp, c_i64, c_i32, c_f64, c_f64, c_i32,
+                                       c_f64, c_f64, c_dp, c_dp, c_dp, c_dp, c_i64, c_dp]),
+    ("dfcsr_deposit_density_vx", c_i32, [c_dp, c_dp, c_dp, c_i64, c_i32, c_f64, c_f64, c_i32,
+                                         c_f64, c_f64, c_dp, c_dp, c_dp, c_dp, c_dp, c_i64,
+                                         c_dp]),
+    ("dfcsr_df_workspace_bytes", c_i64, [c_i32, c_i32]),
+    ("dfcsr_df_process", c_i32, [c_dp, c_dp, c_dp, c_dp, c_dp, c_i32, c_f64, c_f64, c_i32,
+                                 c_f64, c_f64, c_f64, c_i32, c_i32, c_dp, c_dp, c_i64, c_dp]),
+    ("dfcsr_savgol_operator", c_i32, [c_i32, c_i32, c_dp, c_dp]),
+    ("dfcsr_regrid_workspace_bytes", c_i64, [c_i32]),
+    ("dfcsr_regrid", c_i32, [c_dp, c_i64, ctypes.POINTER(RegridJob), c_i32, c_dp, c_dp, c_i32,
+                             c_i32, c_f64, c_f64, c_f64, c_f64, c_dp, c_i64, c_dp]),
+    ("dfcsr_interpolate3d", c_i32, [ctypes.POINTER(History), c_i32, c_dp, c_i64, c_dp, c_dp]),
+    ("dfcsr_interpolate1d", c_i32, [ctypes.POINTER(Lattice), c_i32, c_dp, c_i64, c_dp, c_dp]),
+    ("dfcsr_csr_integrand", c_i32, [ctypes.POINTER(History), ctypes.POINTER(Lattice), c_f64,
+                                    c_f64, c_f64, c_dp, c_i32, c_dp, c_i32, c_dp, c_dp, c_dp]),
+    ("dfcsr_wake_workspace_bytes", c_i64, [c_i32]),
+    ("dfcsr_wake_mesh", c_i32, [ctypes.POINTER(History), ctypes.POINTER(Lattice),
+                                ctypes.POINTER(WakeParams), c_i32, c_dp, c_dp, c_dp, c_dp, c_dp,
+                                c_i64, c_dp]),
+    ("dfcsr_wake_batched_workspace_bytes", c_i64, [c_i32]),
+    ("dfcsr_wake_mesh_batched", c_i32, [c_i32, ctypes.POINTER(History), ctypes.POINTER(Lattice),
+                                        ctypes.POINTER(WakeParams), c_i32,
+                                        ctypes.POINTER(c_dp), ctypes.POINTER(c_dp), c_dp, c_i64,
+                                        c_dp]),
+    ("dfcsr_apply_wakes", c_i32, [c_dp, c_dp, c_dp, c_dp, c_i64, c_dp, c_dp, c_i32, c_i32, c_f64,
+                                  c_f64, c_f64, c_f64, c_f64, c_f64, c_f64, c_f64, c_i32, c_dp]),
+]
+
+EXPORTED = tuple(name for name, _, _ in _SIGNATURES)
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Loads and binds the library (once).  Raises DfcsrError if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise DfcsrError(f"{path} not found: build it first (python -c 'import __graft_entry__ as g; "
+                         "g.build()' or make -C pydfcsr_amd/csrc)")
+    try:
+        lib = ctypes.CDLL(path)
+    except OSError as e:
+        raise DfcsrError(f"cannot load {path}: {e}") from e
+    for name, res, args in _SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.dfcsr_abi_version() != 1:
+        raise DfcsrError(f"ABI version mismatch: {lib.dfcsr_abi_version()}")
+    _lib = lib
+    return lib
+
+
+def check(rc: int, what: str):
+    if rc != 0:
+        msg = load().dfcsr_last_error().decode(errors="replace")
+        raise DfcsrError(f"{what} failed ({rc}): {msg}")
+
+
+def ptr(t: torch.Tensor | None) -> int | None:
+    if t is None:
+        return None
+    return t.data_ptr()
+
+
+def stream_handle(device=None) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def require_device_tensor(t: torch.Tensor, dtype=torch.float64, name="tensor"):
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise DfcsrError(f"{name} must be a HIP device tensor")
+    if t.dtype != dtype:
+        raise DfcsrError(f"{name} must be {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise DfcsrError(f"{name} must be contiguous")
+    return t
+
+
+class Workspace:
+    """Caller-owned scratch buffers, grown on demand and reused across calls."""
+
+    def __init__(self, device):
+        self.device = torch.device(device)
+        self._bufs = {}
+
+    def get(self, name: str, nbytes: int) -> torch.Tensor:
+        nbytes = max(int(nbytes), 256)
+        b = self._bufs.get(name)
+        if b is None or b.numel() < nbytes:
+            b = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+            self._bufs[name] = b
+        return b

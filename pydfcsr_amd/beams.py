@@ -1,0 +1,263 @@
+"""Beam state on the device and its per-step statistics.
+
+Mirrors the reference's Beam (beams.py:12-229): particle coordinates in Bmad
+order (x, px, y, py, z, pz) relative to the constant reference momentum
+``init_energy`` (eV), the cached moments of update_status, the CSR kick of
+apply_wakes and tracking through lattice elements.
+
+Statistics (sigma, mean, slope = polyfit(z, x, 1), x_transform moments and
+the |z| < 0.1 sigma_z slice std) come from one deterministic device
+reduction; the kick is the gfx950 mesh->particle kernel.
+
+Tracking replaces bmadx's track_element, which this build cannot use: each
+step applies the linear transfer map of the element piece (r_gen6.py:4-122
+conventions: sector-bend body, hard-edge pole-face focusing at the piece
+that holds the entrance/exit face, quadrupoles; thin sextupole kick).  It is
+host-side plumbing outside the CSR step, and its numbers are not pinned
+against Bmad-X.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+from . import _lib, kernels
+
+MC2 = 510998.95   # electron rest energy, eV
+
+
+def transfer_matrix(L, angle=0.0, k1=0.0, E1=0.0, E2=0.0, hgap=0.0):
+    """6x6 linear map (x, px, y, py, z, pz) of a bend/quad/drift piece."""
+    R = np.eye(6)
+    if L == 0:
+        return R
+    h = angle / L
+    kx2 = k1 + h * h
+    ky2 = -k1
+
+    def plane(k2):
+        k = math.sqrt(abs(k2))
+        ph = k * L
+        if abs(ph) < 1e-12:
+            return np.array([[1.0, L], [0.0, 1.0]]), None, None, None
+        if k2 > 0:
+            co, si = math.cos(ph), math.sin(ph)
+            return np.array([[co, si / k], [-k * si, co]]), co, si, k
+        co, si = math.cosh(ph), math.sinh(ph)
+        return np.array([[co, si / k], [k * si, co]]), co, si, k
+
+    Rx, co, si, kx = plane(kx2)
+    Ry = plane(ky2)[0]
+    R[0:2, 0:2] = Rx
+    R[2:4, 2:4] = Ry
+    if co is not None:
+        D0 = h * (1 - co) / kx2
+        D1 = h * si / kx
+        R[0, 5] = D0
+        R[1, 5] = D1
+        R[4, 0] = -D1
+        R[4, 1] = -D0
+        R[4, 5] = -(h ** 2) * (kx * L - kx * Rx[0, 1]) / kx ** 3
+    if angle:
+        def face(e):
+            F = np.eye(6)
+            psi = h * hgap * (1 + math.sin(e) ** 2) / math.cos(e)
+            F[1, 0] = math.tan(e) * h
+            F[3, 2] = -math.tan(e - psi) * h
+            return F
+        R = face(E2) @ R @ face(E1)
+    return R
+
+
+def element_map(ele: dict, DL: float, entrance=False, exit=False):
+    """Map of a piece of length DL of `ele` (CSR.py:146-199 element pieces)."""
+    typ = ele["type"]
+    if typ == "dipole":
+        G = ele["G"] if "G" in ele else ele["angle"] / ele["L"]
+        E1 = ele.get("E1", 0.0) if entrance else 0.0
+        E2 = ele.get("E2", 0.0) if exit else 0.0
+        return transfer_matrix(DL, angle=G * DL, E1=E1, E2=E2, hgap=ele.get("HGAP", 0.0) or 0.0)
+    if typ == "quad":
+        return transfer_matrix(DL, k1=ele["K1"])
+    return transfer_matrix(DL)   # drift, sextupole body (kick applied separately)
+
+
+class Beam:
+    """Beam from an ``input_beam`` config (beams.py:16-86).
+
+    Styles: ``from_file`` (6 text columns x px y py z pz, plus charge [C] and
+    energy [eV]) and ``gaussian`` (seeded synthetic beam: n_particle, charge,
+    energy, sigma_x, sigma_z, chirp, sigma_delta, optional two_gaussian).
+    ``distgen`` and ``ParticleGroup`` need packages this build does not ship.
+    """
+
+    def __init__(self, input_beam, device="cuda", base_dir=None):
+        self.check_inputs(input_beam)
+        self.input_beam_config = input_beam
+        self.style = input_beam["style"]
+        self.device = torch.device(device)
+        if self.style == "from_file":
+            coords = np.loadtxt(input_beam["beamfile"])
+            assert coords.shape[1] == 6, \
+                f"Error: input beam must have 6 dimension, but get {coords.shape[1]} instead"
+            self._charge = input_beam["charge"]
+            self._init_energy = input_beam["energy"]
+            arr = coords.T.copy()
+        elif self.style == "gaussian":
+            arr = gaussian_beam(**{k: v for k, v in input_beam.items() if k != "style"})
+            self._charge = float(input_beam["charge"])
+            self._init_energy = float(input_beam["energy"])
+        else:
+            raise NotImplementedError(
+                f"input_beam style {self.style!r} needs distgen/openPMD, which this build "
+                "does not include; use 'from_file' or 'gaussian'")
+        self.coords = torch.from_numpy(np.ascontiguousarray(arr, dtype=np.float64)).to(self.device)
+        self._init_gamma = self._init_energy / MC2
+        self.position = 0
+        self.step = 0
+        self.ws = _lib.Workspace(self.device)
+        self.update_status()
+
+    @staticmethod
+    def from_arrays(x, px, z, pz, charge, energy, y=None, py=None, device="cuda"):
+        b = Beam.__new__(Beam)
+        b.style = "arrays"
+        b.device = torch.device(device)
+        n = len(x)
+        arr = np.zeros((6, n))
+        arr[0], arr[1], arr[4], arr[5] = x, px, z, pz
+        if y is not None:
+            arr[2] = y
+        if py is not None:
+            arr[3] = py
+        b.coords = torch.from_numpy(arr).to(b.device)
+        b._charge = float(charge)
+        b._init_energy = float(energy)
+        b._init_gamma = b._init_energy / MC2
+        b.position = 0
+        b.step = 0
+        b.ws = _lib.Workspace(b.device)
+        b.update_status()
+        return b
+
+    @staticmethod
+    def check_inputs(input_beam):
+        assert "style" in input_beam, "ERROR: input_beam must have keyword <style>"
+        req = {"from_file": ["style", "beamfile", "charge", "energy"],
+               "distgen": ["style", "distgen_input_file"],
+               "ParticleGroup": ["style", "ParticleGroup_h5"],
+               "gaussian": ["style", "n_particle", "charge", "energy"]}
+        if input_beam["style"] not in req:
+            raise Exception("input beam parsing Error: invalid input style")
+        if input_beam["style"] != "gaussian":
+            allowed = req[input_beam["style"]] + ["verbose"]
+            for p in input_beam:
+                assert p in allowed, f"Incorrect param given to Beam.__init__(**kwargs): {p}\n" \
+                                     f"Allowed params: {allowed}"
+        for r in req[input_beam["style"]]:
+            assert r in input_beam, \
+                f"Required input parameter {r} to Beam.__init__(**kwargs) was not found."
+
+    # -- statistics (beams.py:88-98, 137-215) ---------------------------------
+    def update_status(self):
+        m = kernels.beam_moments(self.x, self.z, self.ws)
+        self.moments = m
+        self._sigma_x = m["sigma_x"]
+        self._sigma_z = m["sigma_z"]
+        self._slope = m["slope"]
+        self._mean_x = m["mean_x"]
+        self._mean_z = m["mean_z"]
+
+    def track(self, element_map_6x6, step_size, update_step=True, sextupole_k2l=0.0):
+        M = torch.from_numpy(element_map_6x6).to(self.device)
+        self.coords = M @ self.coords
+        if sextupole_k2l:
+            x, y = self.coords[0], self.coords[2]
+            self.coords[1] -= 0.5 * sextupole_k2l * (x * x - y * y)
+            self.coords[3] += sextupole_k2l * x * y
+        self.position += step_size
+        if update_step:
+            self.step += 1
+        self.update_status()
+
+    def apply_wakes(self, dE_dct, x_kick, xrange, zrange, step_size, transverse_on):
+        """beams.py:108-131.  xrange/zrange: the mesh linspace axes, given as
+        arrays or as (start, stop, num) triples."""
+        xr = _axis(xrange)
+        zr = _axis(zrange)
+        dE = torch.as_tensor(dE_dct, dtype=torch.float64, device=self.device).contiguous()
+        xk = torch.as_tensor(x_kick, dtype=torch.float64, device=self.device).contiguous()
+        self.coords = self.coords.contiguous()
+        m = self.moments
+        kernels.apply_wakes(self.coords[1], self.coords[5], self.coords[0], self.coords[4], dE, xk,
+                            xr[0], xr[1], zr[0], zr[1], m["slope0"], m["slope1"], step_size,
+                            self.init_energy, transverse_on, xr[2], zr[2])
+        self.update_status()
+
+    # -- properties --------------------------------------------------------------
+    x = property(lambda self: self.coords[0])
+    px = property(lambda self: self.coords[1])
+    y = property(lambda self: self.coords[2])
+    py = property(lambda self: self.coords[3])
+    z = property(lambda self: self.coords[4])
+    pz = property(lambda self: self.coords[5])
+    charge = property(lambda self: self._charge)
+    init_energy = property(lambda self: self._init_energy)
+    init_gamma = property(lambda self: self._init_gamma)
+    sigma_x = property(lambda self: self._sigma_x)
+    sigma_z = property(lambda self: self._sigma_z)
+    mean_x = property(lambda self: self._mean_x)
+    mean_z = property(lambda self: self._mean_z)
+    slope = property(lambda self: self._slope)
+
+    @property
+    def x_transform(self):
+        return self.x - (self._slope[0] * self.z + self._slope[1])
+
+    @property
+    def energy(self):
+        return (self.pz + 1) * self.init_energy
+
+    @property
+    def mean_energy(self):
+        return float(self.energy.mean())
+
+    @property
+    def sigma_energy(self):
+        return float(self.energy.std(unbiased=False))
+
+
+def _axis(a):
+    if isinstance(a, (tuple, list)) and len(a) == 3:
+        return float(a[0]), float(a[1]), int(a[2])
+    a = np.asarray(a.cpu() if isinstance(a, torch.Tensor) else a, dtype=np.float64)
+    return float(a[0]), float(a[-1]), int(a.shape[0])
+
+
+def gaussian_beam(n_particle, sigma_x=50e-6, sigma_z=50e-6, chirp=0.0, sigma_delta=0.0,
+                  seed=0, two_gaussian=None, charge=None, energy=None, sigma_px=0.0, **_):
+    """Seeded synthetic beam, returns (6, n) coordinates.
+
+    two_gaussian: dict(sigmas=(s1, s2), separation=d, ratio=(q1, q2)) puts the
+    fraction q1 at -d/2 with sigma s1 and q2 at +d/2 with sigma s2 in z."""
+    rng = np.random.default_rng(seed)
+    n = int(n_particle)
+    arr = np.zeros((6, n))
+    if two_gaussian:
+        s1, s2 = two_gaussian["sigmas"]
+        d = two_gaussian["separation"]
+        q1, _q2 = two_gaussian["ratio"]
+        n1 = int(round(n * q1))
+        arr[4, :n1] = rng.normal(-d / 2, s1, n1)
+        arr[4, n1:] = rng.normal(d / 2, s2, n - n1)
+    else:
+        arr[4] = rng.normal(0.0, sigma_z, n)
+    arr[0] = rng.normal(0.0, sigma_x, n)
+    if sigma_px:
+        arr[1] = rng.normal(0.0, sigma_px, n)
+    arr[5] = chirp * arr[4]
+    if sigma_delta:
+        arr[5] += rng.normal(0.0, sigma_delta, n)
+    return arr

@@ -1,0 +1,196 @@
+// Device functions shared by the wake kernels and the parity entry points:
+// lattice lookups (interpolate1D, interp1D.py:13-36), the trilinear history
+// lookup (interpolate3D, interp3D.py:18-66) and one integrand point of
+// get_CSR_integrand (CSR.py:605-782).
+//
+// The floating-point contraction mode is the including file's: the parity
+// entry points compile this without fused multiply-add and with true
+// divisions (kRecip = false) so they reproduce the reference's rounding; the
+// wake kernels multiply by precomputed reciprocals (kRecip = true).
+#pragma once
+
+#include "common.h"
+
+namespace dfcsr {
+
+struct Lat6 {
+  double X0, Y0, nx, ny, tx, ty;
+};
+
+// interpolate1D of all six lattice columns at s.
+template <bool kRecip>
+__device__ inline Lat6 lattice_at(const DfcsrLattice &L, double s, double inv_ds) {
+  double u = kRecip ? (s - L.min_s) * inv_ds : (s - L.min_s) / L.ds;
+  TruncIdx t = trunc_index(u, L.ns);
+  Lat6 r;
+  if (!t.ok) {
+    r.X0 = r.Y0 = r.nx = r.ny = r.tx = r.ty = 0.0;
+    return r;
+  }
+  const double *a = L.table + (int64_t)t.i0 * 6;
+  const double *b = L.table + (int64_t)t.i1 * 6;
+  const double w = 1 - t.frac, f = t.frac;
+  r.X0 = a[0] * w + b[0] * f;
+  r.Y0 = a[1] * w + b[1] * f;
+  r.nx = a[2] * w + b[2] * f;
+  r.ny = a[3] * w + b[3] * f;
+  r.tx = a[4] * w + b[4] * f;
+  r.ty = a[5] * w + b[5] * f;
+  return r;
+}
+
+// rho(s') by the element masks of CSR.py:651-656 (later matches win; 0 past
+// the lattice end).
+__device__ inline double rho_at(const DfcsrLattice &L, double sp) {
+  double r = 0.0;
+  for (int k = 0; k < L.nelem; ++k) {
+    bool in = (k == 0) ? (sp < L.distance[0]) : (sp < L.distance[k] && sp >= L.distance[k - 1]);
+    if (in) r = L.rho[k];
+  }
+  return r;
+}
+
+// History axes with the per-call reciprocals.
+struct HistAxes {
+  double min_t, min_x, min_z, dt, dx, dz, it, ix, iz;
+};
+
+__host__ __device__ inline HistAxes make_axes(const DfcsrHistory &H) {
+  HistAxes a;
+  a.min_t = H.min_t;
+  a.min_x = H.min_x;
+  a.min_z = H.min_z;
+  a.dt = H.dt;
+  a.dx = H.dx;
+  a.dz = H.dz;
+  a.it = 1.0 / H.dt;
+  a.ix = 1.0 / H.dx;
+  a.iz = 1.0 / H.dz;
+  return a;
+}
+
+__device__ inline const double *hist_rec(const DfcsrHistory &H, int ti, int xi, int zi) {
+  int slot = H.head + ti;
+  if (slot >= H.cap) slot -= H.cap;
+  return H.data + (((int64_t)slot * H.Xi + xi) * H.Zi + zi) * DFCSR_HIST_REC64;
+}
+
+// Trilinear lookup of all five fields at (t, x, z) with interpolate3D's
+// truncation and validity rules.  Returns false (and zeros) when invalid.
+template <bool kRecip>
+__device__ inline bool hist_lookup5(const DfcsrHistory &H, const HistAxes &A, double tq,
+                                    double xq, double zq, double out[5]) {
+  double ut = kRecip ? (tq - A.min_t) * A.it : (tq - A.min_t) / A.dt;
+  double ux = kRecip ? (xq - A.min_x) * A.ix : (xq - A.min_x) / A.dx;
+  double uz = kRecip ? (zq - A.min_z) * A.iz : (zq - A.min_z) / A.dz;
+  TruncIdx T = trunc_index(ut, H.nt), X = trunc_index(ux, H.Xi), Z = trunc_index(uz, H.Zi);
+  if (!(T.ok && X.ok && Z.ok)) {
+    for (int f = 0; f < 5; ++f) out[f] = 0.0;
+    return false;
+  }
+  const double td = T.frac, xd = X.frac, zd = Z.frac;
+  const double *p000 = hist_rec(H, T.i0, X.i0, Z.i0), *p100 = hist_rec(H, T.i1, X.i0, Z.i0);
+  const double *p010 = hist_rec(H, T.i0, X.i1, Z.i0), *p110 = hist_rec(H, T.i1, X.i1, Z.i0);
+  const int dz1 = (Z.i1 - Z.i0) * DFCSR_HIST_REC64;
+#pragma unroll
+  for (int f = 0; f < 5; ++f) {
+    // interp3D.py:53-61: lerp along t, then x, then z
+    double c00 = p000[f] * (1 - td) + p100[f] * td;
+    double c01 = p000[f + dz1] * (1 - td) + p100[f + dz1] * td;
+    double c10 = p010[f] * (1 - td) + p110[f] * td;
+    double c11 = p010[f + dz1] * (1 - td) + p110[f + dz1] * td;
+    double c0 = c00 * (1 - xd) + c10 * xd;
+    double c1 = c01 * (1 - xd) + c11 * xd;
+    out[f] = c0 * (1 - zd) + c1 * zd;
+  }
+  return true;
+}
+
+// Observer-side quantities of one mesh point (CSR.py:608-642).
+struct Observer {
+  double s, x, t;
+  Lat6 L;
+  double vx;   // vx at (t, x, s - t)
+};
+
+template <bool kRecip>
+__device__ inline Observer make_observer(const DfcsrHistory &H, const HistAxes &A,
+                                         const DfcsrLattice &L, double s, double x, double t) {
+  Observer o;
+  o.s = s;
+  o.x = x;
+  o.t = t;
+  o.L = lattice_at<kRecip>(L, s, 1.0 / L.ds);
+  double v[5];
+  hist_lookup5<kRecip>(H, A, t, x, s - t, v);
+  o.vx = v[DFCSR_F_VX];
+  return o;
+}
+
+// Per-column (fixed s') quantities.
+struct Column {
+  double sp, rho;
+  Lat6 L;
+  double ax, ay;   // (X0_s - X0_sp) + x n_s  (r - r' before the -x' n' term)
+};
+
+template <bool kRecip>
+__device__ inline Column make_column(const DfcsrLattice &L, const Observer &o, double sp,
+                                     double inv_ds) {
+  Column c;
+  c.sp = sp;
+  c.L = lattice_at<kRecip>(L, sp, inv_ds);
+  c.rho = rho_at(L, sp);
+  c.ax = o.L.X0 - c.L.X0 + o.x * o.L.nx;
+  c.ay = o.L.Y0 - c.L.Y0 + o.x * o.L.ny;
+  return c;
+}
+
+// One integrand point (I_z, I_x) at source (x', s'), CSR.py:645-775 in the
+// reference's operation order.  Returns false (I = 0) when the retarded
+// lookup falls outside the history, where every gathered field is zero.
+template <bool kRecip>
+__device__ inline bool csr_point(const DfcsrHistory &H, const HistAxes &A, const Observer &o,
+                                 const Column &c, double xp, double &Iz, double &Ix) {
+  const double rx = c.ax - xp * c.L.nx;
+  const double ry = c.ay - xp * c.L.ny;
+  const double R = sqrt(rx * rx + ry * ry);
+  const double t_ret = o.t - R;
+  double f[5];
+  if (!hist_lookup5<kRecip>(H, A, t_ret, xp, c.sp - t_ret, f)) {
+    Iz = 0.0;
+    Ix = 0.0;
+    return false;
+  }
+  const double dens = f[DFCSR_F_DENSITY], dens_x = f[DFCSR_F_DENSITY_X],
+               dens_z = f[DFCSR_F_DENSITY_Z], vx_ret = f[DFCSR_F_VX], div_v = f[DFCSR_F_VX_X];
+  const double scale = 1 + xp * c.rho;
+  const double vel_x = o.L.tx + o.vx * o.L.nx;
+  const double vel_y = o.L.ty + o.vx * o.L.ny;
+  const double vr_x = c.L.tx + vx_ret * c.L.nx;
+  const double vr_y = c.L.ty + vx_ret * c.L.ny;
+  const double nab_x = dens_x * c.L.nx + dens_z / scale * c.L.tx;
+  const double nab_y = dens_x * c.L.ny + dens_z / scale * c.L.ty;
+  const double part1 = vel_x * vr_x + vel_y * vr_y;
+  const double num1 = scale * ((vel_x - part1 * vr_x) * nab_x + (vel_y - part1 * vr_y) * nab_y);
+  const double num2 = -scale * part1 * dens * div_v;
+  const double nmx = o.L.nx - c.L.nx, nmy = o.L.ny - c.L.ny;
+  const double p1 = rx * nmx + ry * nmy;
+  const double p2 = o.L.nx * c.L.tx + o.L.ny * c.L.ty;
+  const double pd = -(vr_x * nab_x + vr_y * nab_y) - dens * div_v;
+  if (kRecip) {
+    const double iR = 1.0 / R;
+    const double iR2 = iR * iR;
+    Iz = (num1 + num2) * iR;
+    Ix = scale * p1 * (iR2 * iR) * dens + scale * p1 * iR2 * pd + -scale * p2 * iR * pd;
+  } else {
+    Iz = num1 / R + (num2) / R;
+    const double W1 = scale * p1 / (R * R * R) * dens;
+    const double W2 = scale * p1 / (R * R) * pd;
+    const double W3 = -scale * p2 / R * pd;
+    Ix = W1 + W2 + W3;
+  }
+  return true;
+}
+
+}  // namespace dfcsr

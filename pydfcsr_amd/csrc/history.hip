@@ -1,0 +1,133 @@
+// a8-a10: re-gridding of density-function slices into the device history.
+//
+// Replaces DF_tracker.DF_interp (deposit.py:296-309) as used by
+// append_interpolant for the new slice (deposit.py:328-337) and for a full
+// re-grid of every logged slice (deposit.py:379-390).  Each target point
+// (xi, zi) of the common interp grid takes scipy RegularGridInterpolator's
+// linear value on the slice's own linspace grids: interval search, norm
+// distances y0, y1 and the corner sum
+//   v00(1-y0)(1-y1) + v01(1-y0)y1 + v10 y0(1-y1) + v11 y0 y1,
+// with fill 0 outside the grid (mean(vx_x) of the slice for vx_x).
+//
+// History layout: FP64 records [slot][Xi][Zi][6] (density, density_x,
+// density_z, vx, vx_x, pad) so the two z-neighbours a trilinear lookup needs
+// are 96 contiguous, 16-byte aligned bytes; the optional FP32 shadow uses
+// [slot][Xi][Zi][8].
+#include "common.h"
+
+namespace dfcsr {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kMaxJobs = 4096;
+
+// per-job mean of the raw vx_x field (np.mean(vx_x), deposit.py:332, 384)
+__global__ __launch_bounds__(kThreads) void vxx_mean_kernel(const double *__restrict__ src,
+                                                            int64_t slot_stride,
+                                                            const DfcsrRegridJob *__restrict__ jobs,
+                                                            double *__restrict__ means) {
+  __shared__ double red[kThreads / 64];
+  const DfcsrRegridJob J = jobs[blockIdx.x];
+  const int64_t N = (int64_t)J.nbx * J.nbz;
+  const double *f = src + (int64_t)J.src_slot * slot_stride + (int64_t)DFCSR_F_VX_X * N;
+  double s = 0.0;
+  for (int64_t e = threadIdx.x; e < N; e += kThreads) s += f[e];
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) means[blockIdx.x] = s / (double)N;
+}
+
+__global__ __launch_bounds__(kThreads) void regrid_kernel(
+    const double *__restrict__ src, int64_t slot_stride, const DfcsrRegridJob *__restrict__ jobs,
+    const double *__restrict__ means, double *__restrict__ h64, float *__restrict__ h32, int Xi,
+    int Zi, double xi0, double xi1, double zi0, double zi1) {
+#pragma clang fp contract(off)
+  const DfcsrRegridJob J = jobs[blockIdx.y];
+  const int64_t cells = (int64_t)Xi * Zi;
+  const int64_t Nsrc = (int64_t)J.nbx * J.nbz;
+  const double *base = src + (int64_t)J.src_slot * slot_stride;
+  const double fill_vxx = means[blockIdx.y];
+  for (int64_t c = (int64_t)blockIdx.x * kThreads + threadIdx.x; c < cells;
+       c += (int64_t)gridDim.x * kThreads) {
+    int ix = (int)(c / Zi), iz = (int)(c % Zi);
+    double xq = linspace_at(xi0, xi1, Xi, ix);
+    double zq = linspace_at(zi0, zi1, Zi, iz);
+    double v[5];
+    bool oob = !(xq >= J.x0 && xq <= J.x1 && zq >= J.z0 && zq <= J.z1);
+    if (oob || J.nbx < 2 || J.nbz < 2) {
+      for (int f = 0; f < 5; ++f) v[f] = (f == DFCSR_F_VX_X) ? fill_vxx : 0.0;
+    } else {
+      int i = rgi_interval(xq, J.x0, J.x1, J.nbx);
+      int j = rgi_interval(zq, J.z0, J.z1, J.nbz);
+      double gi = linspace_at(J.x0, J.x1, J.nbx, i), gi1 = linspace_at(J.x0, J.x1, J.nbx, i + 1);
+      double gj = linspace_at(J.z0, J.z1, J.nbz, j), gj1 = linspace_at(J.z0, J.z1, J.nbz, j + 1);
+      double y0 = (xq - gi) / (gi1 - gi);
+      double y1 = (zq - gj) / (gj1 - gj);
+      int64_t o = (int64_t)i * J.nbz + j;
+      for (int f = 0; f < 5; ++f) {
+        const double *F = base + (int64_t)f * Nsrc;
+        double r = F[o] * (1 - y0) * (1 - y1);
+        r = r + F[o + 1] * (1 - y0) * y1;
+        r = r + F[o + J.nbz] * y0 * (1 - y1);
+        r = r + F[o + J.nbz + 1] * y0 * y1;
+        v[f] = r;
+      }
+    }
+    double *d = h64 + ((int64_t)J.dst_slot * cells + c) * DFCSR_HIST_REC64;
+    double2 *d2 = reinterpret_cast<double2 *>(d);
+    d2[0] = make_double2(v[0], v[1]);
+    d2[1] = make_double2(v[2], v[3]);
+    d2[2] = make_double2(v[4], 0.0);
+    if (h32) {
+      float4 *s4 = reinterpret_cast<float4 *>(h32 + ((int64_t)J.dst_slot * cells + c) * DFCSR_HIST_REC32);
+      s4[0] = make_float4((float)v[0], (float)v[1], (float)v[2], (float)v[3]);
+      s4[1] = make_float4((float)v[4], 0.f, 0.f, 0.f);
+    }
+  }
+}
+
+}  // namespace
+}  // namespace dfcsr
+
+using namespace dfcsr;
+
+extern "C" int64_t dfcsr_regrid_workspace_bytes(int njobs) {
+  if (njobs < 1) njobs = 1;
+  return (int64_t)njobs * (int64_t)(sizeof(DfcsrRegridJob) + sizeof(double)) + 256;
+}
+
+extern "C" int dfcsr_regrid(const double *src, int64_t src_slot_stride, const DfcsrRegridJob *jobs,
+                            int njobs, double *hist64, float *hist32, int Xi, int Zi, double xi0,
+                            double xi1, double zi0, double zi1, void *ws, int64_t ws_bytes,
+                            void *stream) {
+  if (!src || !jobs || !hist64 || !ws) return set_error(DFCSR_EINVAL, "regrid: NULL pointer");
+  if (njobs < 1) return DFCSR_OK;
+  if (njobs > kMaxJobs) return set_error(DFCSR_ERANGE, "regrid: %d jobs > %d", njobs, kMaxJobs);
+  if (Xi < 2 || Zi < 2) return set_error(DFCSR_EINVAL, "regrid: interp grid %d x %d", Xi, Zi);
+  if (ws_bytes < dfcsr_regrid_workspace_bytes(njobs))
+    return set_error(DFCSR_ENOWS, "regrid: workspace too small");
+  for (int k = 0; k < njobs; ++k) {
+    const DfcsrRegridJob &J = jobs[k];
+    if (J.nbx < 1 || J.nbz < 1 || (int64_t)5 * J.nbx * J.nbz > src_slot_stride || J.src_slot < 0 ||
+        J.dst_slot < 0)
+      return set_error(DFCSR_EINVAL, "regrid: bad job %d (nbx %d nbz %d src %d dst %d)", k, J.nbx,
+                       J.nbz, J.src_slot, J.dst_slot);
+  }
+  hipStream_t s = as_stream(stream);
+  // Job descriptors travel in the workspace: written by a host->device copy
+  // ordered on the stream.
+  DfcsrRegridJob *djobs = static_cast<DfcsrRegridJob *>(ws);
+  double *means = reinterpret_cast<double *>(
+      static_cast<char *>(ws) + (((int64_t)njobs * sizeof(DfcsrRegridJob) + 63) / 64) * 64);
+  DFCSR_CHECK_HIP(hipMemcpyAsync(djobs, jobs, (size_t)njobs * sizeof(DfcsrRegridJob),
+                                 hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(vxx_mean_kernel, dim3(njobs), dim3(kThreads), 0, s, src, src_slot_stride,
+                     (const DfcsrRegridJob *)djobs, means);
+  int64_t cells = (int64_t)Xi * Zi;
+  int gx = (int)((cells + kThreads - 1) / kThreads);
+  if (gx > 2048) gx = 2048;
+  hipLaunchKernelGGL(regrid_kernel, dim3(gx, njobs), dim3(kThreads), 0, s, src, src_slot_stride,
+                     (const DfcsrRegridJob *)djobs, (const double *)means, hist64, hist32, Xi, Zi,
+                     xi0, xi1, zi0, zi1);
+  DFCSR_LAUNCH_CHECK("regrid");
+  return DFCSR_OK;
+}

@@ -1,0 +1,332 @@
+// a13-a15: the retarded CSR integral on the wake mesh.
+//
+// Replaces CSR2D.get_CSR_mesh / calculate_2D_CSR / get_CSR_wake /
+// get_CSR_integrand (CSR.py:361-782).  One workgroup integrates one mesh
+// point over all of its regions (3 without chirp, 4 with):
+//   work item  = (region, row chunk, s' column k); lanes of a wavefront take
+//                consecutive columns, so they share x' rows (the x-range test
+//                is wave-uniform) and their retarded lookups land on the same
+//                history rows (z' = s' - t_ret barely moves along s').
+//   per column : lattice lookups at s', rho(s'), the trapezoid weight in s'.
+//   per point  : |r - r'|, t_ret, the trilinear gather of five fields, the
+//                integrand; points whose gather is outside the history add 0
+//                and skip the arithmetic.
+//   reduction  : sum_j,k wx_j wz_k I_jk per lane, then a fixed-order block
+//                tree: deterministic, the same every launch.
+// Workgroups are mapped so each XCD integrates a contiguous range of mesh
+// points (neighbouring observers read the same history lines through that
+// XCD's L2).
+#include "csr_point.h"
+
+namespace dfcsr {
+
+int check_history(const DfcsrHistory *H, const char *who);
+int check_lattice(const DfcsrLattice *L, const char *who);
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kChunk = 32;   // x' rows per work item
+constexpr int kNumXcd = 8;
+
+struct LinAxis {
+  double a, b, step;
+  int n;
+  __device__ void init(double a_, double b_, int n_) {
+#pragma clang fp contract(off)
+    a = a_;
+    b = b_;
+    n = n_;
+    step = n_ > 1 ? (b_ - a_) / (double)(n_ - 1) : 0.0;
+  }
+  __device__ double at(int i) const {
+#pragma clang fp contract(off)
+    if (n <= 1) return a;
+    if (i == n - 1) return b;
+    double y = (double)i * step;
+    return y + a;
+  }
+  // trapezoid weight of node i (np.trapz over this axis)
+  __device__ double w(int i) const {
+#pragma clang fp contract(off)
+    if (n < 2) return 0.0;
+    double g = at(i);
+    double dl = i > 0 ? g - at(i - 1) : 0.0;
+    double dr = i < n - 1 ? at(i + 1) - g : 0.0;
+    return 0.5 * (dl + dr);
+  }
+};
+
+struct Regions {
+  int n;
+  LinAxis xa[4], sa[4];
+};
+
+__device__ inline double npmax0(double v) { return (v != v) ? v : fmax(0.0, v); }
+
+// get_CSR_wake region bounds (CSR.py:461-553).
+__device__ void make_regions(const DfcsrWakeParams &P, double s, double x, double t,
+                             Regions &rg) {
+#pragma clang fp contract(off)
+  const double sz = P.sigma_z, sx = P.sigma_x, tan_theta = P.slope0;
+  const double x0 = (s - t) * P.slope0;
+  const int Nx = P.Nx, Nz = P.Nz;
+  if (fabs(tan_theta) <= 1) {
+    double s2 = s - 500 * sz, s3 = s - 20 * sz, s4 = s + 5 * sz;
+    double s1 = npmax0(s2 - P.n_formation_length * P.formation_length);
+    rg.n = 3;
+    rg.xa[0].init(x0 - 20 * sx, x0 + 20 * sx, 2 * Nx);
+    rg.sa[0].init(s1, s2, Nz);
+    rg.xa[1].init(x0 - 10 * sx, x0 + 10 * sx, Nx);
+    rg.sa[1].init(s2, s3, Nz);
+    rg.xa[2].init(x0 - 10 * sx, x0 + 10 * sx, Nx);
+    rg.sa[2].init(s3, s4, Nz);
+    return;
+  }
+  double d, x1_l, x1_r, x2_l, x2_r;
+  if (tan_theta > 0) {
+    double tan_alpha = -2 * tan_theta / (1 - tan_theta * tan_theta);
+    d = (10 * sx + P.mean_x - x) / tan_alpha;
+    x1_l = x + 0.1 * sx;
+    x1_r = x + 10 * sx;
+    x2_l = x - 3 * sx;
+    x2_r = x1_l;
+  } else {
+    double tan_alpha = 2 * tan_theta / (1 - tan_theta * tan_theta);
+    d = -(P.mean_x - x - 10 * sx) / tan_alpha;
+    x1_l = x - 10 * sx;
+    x1_r = x - 1 * sx;
+    x2_l = x1_r;
+    x2_r = x + 3 * sx;
+  }
+  double s4 = s + 3 * sz;
+  double s3 = npmax0(s - d);
+  double s2 = s3 - 200 * sz;
+  double s1 = npmax0(s2 - P.n_formation_length * P.formation_length);
+  rg.n = 4;
+  rg.xa[0].init(x0 - 20 * sx, x0 + 20 * sx, 2 * Nx);   // xp4 over sp1
+  rg.sa[0].init(s1, s2, Nz);
+  rg.xa[1].init(x0 - 5 * sx, x0 + 5 * sx, Nx);         // xp3 over sp2
+  rg.sa[1].init(s2, s3, Nz);
+  rg.xa[2].init(x1_l, x1_r, Nx);                       // xp1 over sp3
+  rg.sa[2].init(s3, s4, Nz);
+  rg.xa[3].init(x2_l, x2_r, Nx);                       // xp2 over sp3
+  rg.sa[3].init(s3, s4, Nz);
+}
+
+// Mesh point i of get_CSR_mesh (CSR.py:380-389): zmesh = zrange[iz],
+// xmesh = xrange[ix] + polyval(slope, zmesh).
+__device__ inline void mesh_point(const DfcsrWakeParams &P, int i, double &xm, double &zm) {
+#pragma clang fp contract(off)
+  if (P.observers) {
+    zm = P.observers[2 * i];
+    xm = P.observers[2 * i + 1];
+    return;
+  }
+  int ix = i / P.mesh_zbins, iz = i % P.mesh_zbins;
+  zm = linspace_at(P.zr0, P.zr1, P.mesh_zbins, iz);
+  double xt = linspace_at(P.xr0, P.xr1, P.mesh_xbins, ix);
+  double pv = P.slope0 * zm;
+  pv = pv + P.slope1;
+  xm = xt + pv;
+}
+
+__device__ inline int xcd_remap(int b, int nb) {
+  int q = nb / kNumXcd, r = nb % kNumXcd;
+  int xcd = b % kNumXcd, pos = b / kNumXcd;
+  return xcd * q + (xcd < r ? xcd : r) + pos;
+}
+
+// Region table of one mesh point, shared by the workgroup through LDS
+// (dynamic indexing of a private array would put it in scratch memory).
+struct RegionLds {
+  Regions rg;
+  int off[5];
+};
+
+// Integrates mesh point i; every thread returns the block sums.
+__device__ void wake_point_fp64(const DfcsrHistory &H, const DfcsrLattice &L,
+                                const DfcsrWakeParams &P, int i, double *red, RegionLds &R,
+                                double &out_z, double &out_x, double &xm_out, double &zm_out) {
+  double xm, zm;
+  mesh_point(P, i, xm, zm);
+  xm_out = xm;
+  zm_out = zm;
+  const double t = P.t;
+  const double s = t + zm, x = xm;
+  const HistAxes A = make_axes(H);
+  const double inv_ds = 1.0 / L.ds;
+  const Observer o = make_observer<true>(H, A, L, s, x, t);
+  if (threadIdx.x == 0) {
+    make_regions(P, s, x, t, R.rg);
+    R.off[0] = 0;
+    for (int r = 0; r < R.rg.n; ++r)
+      R.off[r + 1] = R.off[r] + ((R.rg.xa[r].n + kChunk - 1) / kChunk) * R.rg.sa[r].n;
+  }
+  __syncthreads();
+  const int nreg = R.rg.n;
+  const int total = R.off[nreg];
+  double acc_z = 0.0, acc_x = 0.0;
+  for (int it = threadIdx.x; it < total; it += blockDim.x) {
+    int r = 0;
+    while (r + 1 < nreg && it >= R.off[r + 1]) ++r;
+    const int rem = it - R.off[r];
+    const LinAxis sa = R.rg.sa[r];
+    const LinAxis xa = R.rg.xa[r];
+    const int c = rem / sa.n, k = rem % sa.n;
+    const double sp = sa.at(k);
+    const double wz = sa.w(k);
+    const Column col = make_column<true>(L, o, sp, inv_ds);
+    const int j0 = c * kChunk;
+    const int j1 = min(j0 + kChunk, xa.n);
+    double cz = 0.0, cx = 0.0;
+    for (int j = j0; j < j1; ++j) {
+      const double xp = xa.at(j);
+      const double ux = (xp - A.min_x) * A.ix;
+      if (!(ux > -1.0 && ux < (double)H.Xi)) continue;   // wave-uniform: rows are shared
+      double Iz, Ix;
+      if (!csr_point<true>(H, A, o, col, xp, Iz, Ix)) continue;
+      const double wx = xa.w(j);
+      cz += wx * Iz;
+      cx += wx * Ix;
+    }
+    acc_z += wz * cz;
+    acc_x += wz * cx;
+  }
+  out_z = block_sum(acc_z, red);
+  out_x = block_sum(acc_x, red);
+}
+
+__global__ __launch_bounds__(kThreads) void wake_fp64_kernel(DfcsrHistory H, DfcsrLattice L,
+                                                             DfcsrWakeParams P,
+                                                             double *__restrict__ xmesh,
+                                                             double *__restrict__ zmesh,
+                                                             double *__restrict__ dE,
+                                                             double *__restrict__ xk) {
+  __shared__ double red[kThreads / 64];
+  __shared__ RegionLds R;
+  const int i = xcd_remap(blockIdx.x, gridDim.x);
+  double sz, sx, xm, zm;
+  wake_point_fp64(H, L, P, i, red, R, sz, sx, xm, zm);
+  if (threadIdx.x == 0) {
+    dE[i] = -P.csr_scaling * sz;
+    xk[i] = P.csr_scaling * sx;
+    if (xmesh) xmesh[i] = xm;
+    if (zmesh) zmesh[i] = zm;
+  }
+}
+
+struct WakeTask {
+  DfcsrHistory H;
+  DfcsrLattice L;
+  DfcsrWakeParams P;
+  double *dE, *xk;
+  int block0;   // first block of this task in the flattened launch
+  int M;
+};
+
+__global__ __launch_bounds__(kThreads) void wake_fp64_batched_kernel(
+    const WakeTask *__restrict__ tasks, int ntask) {
+  __shared__ double red[kThreads / 64];
+  __shared__ RegionLds R;
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  int lo = 0, hi = ntask - 1;
+  while (lo < hi) {   // last task with block0 <= b
+    int mid = (lo + hi + 1) >> 1;
+    if (tasks[mid].block0 <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const WakeTask &T = tasks[lo];
+  const int i = b - T.block0;
+  double sz, sx, xm, zm;
+  wake_point_fp64(T.H, T.L, T.P, i, red, R, sz, sx, xm, zm);
+  if (threadIdx.x == 0) {
+    T.dE[i] = -T.P.csr_scaling * sz;
+    T.xk[i] = T.P.csr_scaling * sx;
+  }
+}
+
+int check_params(const DfcsrWakeParams *p, const char *who) {
+  if (!p) return set_error(DFCSR_EINVAL, "%s: NULL params", who);
+  if (p->Nx < 1 || p->Nz < 1 || p->mesh_xbins < 1 || p->mesh_zbins < 1)
+    return set_error(DFCSR_EINVAL, "%s: bad bins (Nx %d Nz %d mesh %d x %d)", who, p->Nx, p->Nz,
+                     p->mesh_xbins, p->mesh_zbins);
+  if ((int64_t)p->mesh_xbins * p->mesh_zbins > (int64_t)1 << 24)
+    return set_error(DFCSR_ERANGE, "%s: mesh too large", who);
+  return DFCSR_OK;
+}
+
+}  // namespace
+}  // namespace dfcsr
+
+using namespace dfcsr;
+
+extern "C" int64_t dfcsr_wake_workspace_bytes(int mesh_points) {
+  (void)mesh_points;
+  return 256;
+}
+
+extern "C" int dfcsr_wake_mesh(const DfcsrHistory *hist, const DfcsrLattice *lat,
+                               const DfcsrWakeParams *p, int precision, double *xmesh,
+                               double *zmesh, double *dE_dct, double *x_kick, void *ws,
+                               int64_t ws_bytes, void *stream) {
+  int rc;
+  if ((rc = check_history(hist, "wake_mesh"))) return rc;
+  if ((rc = check_lattice(lat, "wake_mesh"))) return rc;
+  if ((rc = check_params(p, "wake_mesh"))) return rc;
+  if (!dE_dct || !x_kick) return set_error(DFCSR_EINVAL, "wake_mesh: NULL output");
+  (void)ws;
+  (void)ws_bytes;
+  const int M = p->mesh_xbins * p->mesh_zbins;
+  if (precision == DFCSR_PREC_FP64) {
+    hipLaunchKernelGGL(wake_fp64_kernel, dim3(M), dim3(kThreads), 0, as_stream(stream), *hist,
+                       *lat, *p, xmesh, zmesh, dE_dct, x_kick);
+  } else {
+    return set_error(DFCSR_EINVAL, "wake_mesh: unknown precision %d", precision);
+  }
+  DFCSR_LAUNCH_CHECK("wake_mesh");
+  return DFCSR_OK;
+}
+
+extern "C" int64_t dfcsr_wake_batched_workspace_bytes(int k) {
+  if (k < 1) k = 1;
+  return (int64_t)k * (int64_t)sizeof(WakeTask) + 256;
+}
+
+extern "C" int dfcsr_wake_mesh_batched(int k, const DfcsrHistory *hist, const DfcsrLattice *lat,
+                                       const DfcsrWakeParams *p, int precision,
+                                       double *const *dE_dct, double *const *x_kick, void *ws,
+                                       int64_t ws_bytes, void *stream) {
+  if (k < 1) return DFCSR_OK;
+  if (!hist || !lat || !p || !dE_dct || !x_kick || !ws)
+    return set_error(DFCSR_EINVAL, "wake_mesh_batched: NULL pointer");
+  if (ws_bytes < dfcsr_wake_batched_workspace_bytes(k))
+    return set_error(DFCSR_ENOWS, "wake_mesh_batched: workspace too small");
+  if (precision != DFCSR_PREC_FP64)
+    return set_error(DFCSR_EINVAL, "wake_mesh_batched: unknown precision %d", precision);
+  WakeTask tasks_h[256];
+  if (k > 256) return set_error(DFCSR_ERANGE, "wake_mesh_batched: %d tasks > 256", k);
+  int blocks = 0;
+  for (int j = 0; j < k; ++j) {
+    int rc;
+    if ((rc = check_history(&hist[j], "wake_mesh_batched"))) return rc;
+    if ((rc = check_lattice(&lat[j], "wake_mesh_batched"))) return rc;
+    if ((rc = check_params(&p[j], "wake_mesh_batched"))) return rc;
+    if (!dE_dct[j] || !x_kick[j]) return set_error(DFCSR_EINVAL, "wake_mesh_batched: NULL output");
+    tasks_h[j].H = hist[j];
+    tasks_h[j].L = lat[j];
+    tasks_h[j].P = p[j];
+    tasks_h[j].dE = dE_dct[j];
+    tasks_h[j].xk = x_kick[j];
+    tasks_h[j].block0 = blocks;
+    tasks_h[j].M = p[j].mesh_xbins * p[j].mesh_zbins;
+    blocks += tasks_h[j].M;
+  }
+  hipStream_t s = as_stream(stream);
+  WakeTask *dt = static_cast<WakeTask *>(ws);
+  DFCSR_CHECK_HIP(hipMemcpyAsync(dt, tasks_h, (size_t)k * sizeof(WakeTask), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(wake_fp64_batched_kernel, dim3(blocks), dim3(kThreads), 0, s,
+                     (const WakeTask *)dt, k);
+  DFCSR_LAUNCH_CHECK("wake_mesh_batched");
+  return DFCSR_OK;
+}

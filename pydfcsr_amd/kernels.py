@@ -1,0 +1,202 @@
+"""Typed wrappers over the C ABI, one per reference operator on the hot path.
+
+Each takes/returns torch float64 tensors on the HIP device and launches on
+torch's current stream.  Reference functions they replace (paths relative to
+pyDFCSR_2D/):
+
+  beam_moments        beams.py:88-98,137-215; CSR.py:368-374; deposit.py:147-159
+  histogram_cic_2d    deposit.py:42-87
+  deposit_density_vx  deposit.py:172-182
+  df_process          deposit.py:183-235
+  regrid              deposit.py:296-309, 328-337, 379-390
+  interpolate3D       interp3D.py:18-66
+  interpolate1D       interp1D.py:13-36
+  csr_integrand       CSR.py:605-782
+  wake_mesh           CSR.py:361-602
+  apply_wakes         beams.py:108-131
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import DfcsrError, check, ptr, require_device_tensor, stream_handle
+
+_MOMENT_KEYS = ("mean_x", "mean_z", "sigma_x", "sigma_z", "slope0", "slope1", "mean_xt",
+                "sigma_xt", "slice_sigma_x", "n_slice")
+
+
+def _f64(t, name):
+    return require_device_tensor(t, torch.float64, name)
+
+
+def beam_moments(x, z, ws: _lib.Workspace, out: torch.Tensor | None = None, host=True):
+    lib = _lib.load()
+    _f64(x, "x")
+    _f64(z, "z")
+    n = x.numel()
+    if z.numel() != n:
+        raise DfcsrError("beam_moments: x and z differ in length")
+    if out is None:
+        out = torch.empty(_lib.MOMENTS_COUNT, dtype=torch.float64, device=x.device)
+    b = ws.get("moments", lib.dfcsr_moments_workspace_bytes(n))
+    check(lib.dfcsr_beam_moments(ptr(x), ptr(z), n, ptr(b), b.numel(), ptr(out),
+                                 stream_handle(x.device)), "beam_moments")
+    if not host:
+        return out
+    v = out.cpu().numpy()
+    d = dict(zip(_MOMENT_KEYS, (float(a) for a in v)))
+    d["slope"] = np.array([d["slope0"], d["slope1"]])
+    return d
+
+
+def histogram_cic_2d(q1, q2, w, nbins_1, bins_start_1, bins_end_1, nbins_2, bins_start_2,
+                     bins_end_2, ws: _lib.Workspace, want_cells=False):
+    """Device histogram_cic_2d; w=None means unit weights.  With want_cells,
+    also returns per-particle lower-cell indices (n, 2) and per-cell counts."""
+    lib = _lib.load()
+    _f64(q1, "q1")
+    _f64(q2, "q2")
+    if w is not None:
+        _f64(w, "w")
+    n = q1.numel()
+    dev = q1.device
+    hist = torch.empty((nbins_1, nbins_2), dtype=torch.float64, device=dev)
+    cells = torch.empty((n, 2), dtype=torch.int32, device=dev) if want_cells else None
+    counts = torch.empty((nbins_1, nbins_2), dtype=torch.int32, device=dev) if want_cells else None
+    b = ws.get("hist", lib.dfcsr_histogram_workspace_bytes(nbins_1, nbins_2))
+    check(lib.dfcsr_histogram_cic_2d(ptr(q1), ptr(q2), ptr(w), n, nbins_1, bins_start_1,
+                                     bins_end_1, nbins_2, bins_start_2, bins_end_2, ptr(hist),
+                                     ptr(cells), ptr(counts), ptr(b), b.numel(),
+                                     stream_handle(dev)), "histogram_cic_2d")
+    if want_cells:
+        return hist, cells, counts
+    return hist
+
+
+def deposit_density_vx(x, z, px, nbx, x0, x1, nbz, z0, z1, ws: _lib.Workspace, density=None,
+                       vx=None, cells=None, counts=None):
+    lib = _lib.load()
+    for t, nm in ((x, "x"), (z, "z"), (px, "px")):
+        _f64(t, nm)
+    n = x.numel()
+    dev = x.device
+    if density is None:
+        density = torch.empty((nbx, nbz), dtype=torch.float64, device=dev)
+    if vx is None:
+        vx = torch.empty((nbx, nbz), dtype=torch.float64, device=dev)
+    b = ws.get("hist", lib.dfcsr_histogram_workspace_bytes(nbx, nbz))
+    check(lib.dfcsr_deposit_density_vx(ptr(x), ptr(z), ptr(px), n, nbx, x0, x1, nbz, z0, z1,
+                                       ptr(density), ptr(vx), ptr(cells), ptr(counts), ptr(b),
+                                       b.numel(), stream_handle(dev)), "deposit_density_vx")
+    return density, vx
+
+
+def df_process(density, vx, x0, x1, z0, z1, velocity_threshold, window, order,
+               ws: _lib.Workspace, out=None, info=None):
+    """In place on density/vx; returns (density, vx, density_x, density_z, vx_x)."""
+    lib = _lib.load()
+    nbx, nbz = density.shape
+    dev = density.device
+    if out is None:
+        out = [torch.empty((nbx, nbz), dtype=torch.float64, device=dev) for _ in range(3)]
+    b = ws.get("df", lib.dfcsr_df_workspace_bytes(nbx, nbz))
+    check(lib.dfcsr_df_process(ptr(density), ptr(vx), ptr(out[0]), ptr(out[1]), ptr(out[2]), nbx,
+                               x0, x1, nbz, z0, z1, float(velocity_threshold), int(window),
+                               int(order), ptr(info), ptr(b), b.numel(), stream_handle(dev)),
+          "df_process")
+    return density, vx, out[0], out[1], out[2]
+
+
+def savgol_operator(window, order):
+    """Host-side Savitzky-Golay operator: (interior stencil, left-edge rows)."""
+    lib = _lib.load()
+    c = np.zeros(window)
+    e = np.zeros((max(window // 2, 1), window))
+    check(lib.dfcsr_savgol_operator(window, order, c.ctypes.data, e.ctypes.data), "savgol_operator")
+    return c, e[: window // 2]
+
+
+def regrid(src: torch.Tensor, slot_stride: int, jobs, hist64, hist32, Xi, Zi, xi0, xi1, zi0, zi1,
+           ws: _lib.Workspace):
+    """jobs: list of (src_slot, dst_slot, nbx, nbz, x0, x1, z0, z1)."""
+    lib = _lib.load()
+    n = len(jobs)
+    if n == 0:
+        return
+    arr = (_lib.RegridJob * n)(*[_lib.RegridJob(*j) for j in jobs])
+    b = ws.get("regrid", lib.dfcsr_regrid_workspace_bytes(n))
+    check(lib.dfcsr_regrid(ptr(src), int(slot_stride), arr, n, ptr(hist64), ptr(hist32), int(Xi),
+                           int(Zi), float(xi0), float(xi1), float(zi0), float(zi1), ptr(b),
+                           b.numel(), stream_handle(src.device)), "regrid")
+
+
+def interpolate3D(hist: _lib.History, field: int, q: torch.Tensor, device):
+    lib = _lib.load()
+    out = torch.empty(q.shape[0], dtype=torch.float64, device=device)
+    check(lib.dfcsr_interpolate3d(ctypes.byref(hist), int(field), ptr(q), q.shape[0], ptr(out),
+                                  stream_handle(device)), "interpolate3d")
+    return out
+
+
+def interpolate1D(lat: _lib.Lattice, col: int, s: torch.Tensor):
+    lib = _lib.load()
+    out = torch.empty(s.numel(), dtype=torch.float64, device=s.device)
+    check(lib.dfcsr_interpolate1d(ctypes.byref(lat), int(col), ptr(s), s.numel(), ptr(out),
+                                  stream_handle(s.device)), "interpolate1d")
+    return out
+
+
+def csr_integrand(hist: _lib.History, lat: _lib.Lattice, s, x, t, xp: torch.Tensor,
+                  sp: torch.Tensor):
+    """Integrand grids (Iz, Ix), shape (len(xp), len(sp)), for one observer."""
+    lib = _lib.load()
+    dev = xp.device
+    Iz = torch.empty((xp.numel(), sp.numel()), dtype=torch.float64, device=dev)
+    Ix = torch.empty_like(Iz)
+    check(lib.dfcsr_csr_integrand(ctypes.byref(hist), ctypes.byref(lat), float(s), float(x),
+                                  float(t), ptr(xp), xp.numel(), ptr(sp), sp.numel(), ptr(Iz),
+                                  ptr(Ix), stream_handle(dev)), "csr_integrand")
+    return Iz, Ix
+
+
+def wake_mesh(hist: _lib.History, lat: _lib.Lattice, p: _lib.WakeParams, precision, device,
+              ws: _lib.Workspace, out=None, want_mesh=True):
+    lib = _lib.load()
+    M = p.mesh_xbins * p.mesh_zbins
+    if out is None:
+        out = {k: torch.empty(M, dtype=torch.float64, device=device)
+               for k in ("xmesh", "zmesh", "dE_dct", "x_kick")}
+    b = ws.get("wake", lib.dfcsr_wake_workspace_bytes(M))
+    check(lib.dfcsr_wake_mesh(ctypes.byref(hist), ctypes.byref(lat), ctypes.byref(p),
+                              int(precision), ptr(out["xmesh"]) if want_mesh else None,
+                              ptr(out["zmesh"]) if want_mesh else None, ptr(out["dE_dct"]),
+                              ptr(out["x_kick"]), ptr(b), b.numel(), stream_handle(device)),
+          "wake_mesh")
+    return out
+
+
+def wake_mesh_batched(hists, lats, params, precision, outs, device, ws: _lib.Workspace):
+    lib = _lib.load()
+    k = len(hists)
+    H = (_lib.History * k)(*hists)
+    L = (_lib.Lattice * k)(*lats)
+    P = (_lib.WakeParams * k)(*params)
+    dE = (_lib.c_dp * k)(*[o["dE_dct"].data_ptr() for o in outs])
+    xk = (_lib.c_dp * k)(*[o["x_kick"].data_ptr() for o in outs])
+    b = ws.get("wake_batched", lib.dfcsr_wake_batched_workspace_bytes(k))
+    check(lib.dfcsr_wake_mesh_batched(k, H, L, P, int(precision), dE, xk, ptr(b), b.numel(),
+                                      stream_handle(device)), "wake_mesh_batched")
+
+
+def apply_wakes(px, pz, x, z, dE_dct, x_kick, xr0, xr1, zr0, zr1, slope0, slope1, step_size,
+                init_energy, transverse_on, xbins, zbins):
+    lib = _lib.load()
+    check(lib.dfcsr_apply_wakes(ptr(px), ptr(pz), ptr(x), ptr(z), x.numel(), ptr(dE_dct),
+                                ptr(x_kick), int(xbins), int(zbins), float(xr0), float(xr1),
+                                float(zr0), float(zr1), float(slope0), float(slope1),
+                                float(step_size), float(init_energy), int(bool(transverse_on)),
+                                stream_handle(x.device)), "apply_wakes")

@@ -1,0 +1,70 @@
+"""Helpers shared by the GPU tests, smoke() and bench.py: move oracle state
+(history stacks, lattice tables) onto the device in the library's layouts."""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from pydfcsr_amd import _lib
+
+FIELDS = _lib.FIELDS
+
+
+class DeviceHistory:
+    """A device ring holding an oracle DFTracker's stacked history.
+
+    wrap=k places logical slice 0 at slot cap-k so the ring wraps around."""
+
+    def __init__(self, tracker, device="cuda", wrap=0, fp32=True):
+        data = np.stack([tracker.data[f] for f in FIELDS], axis=-1)   # (nt, Xi, Zi, 5)
+        nt, Xi, Zi, _ = data.shape
+        cap = nt + (wrap + 1 if wrap else 0)
+        head = (cap - wrap) % cap if wrap else 0
+        rec = np.zeros((cap, Xi, Zi, _lib.HIST_REC64))
+        for i in range(nt):
+            rec[(head + i) % cap, :, :, :5] = data[i]
+        self.h64 = torch.from_numpy(rec).to(device)
+        self.h32 = None
+        if fp32:
+            r32 = np.zeros((cap, Xi, Zi, _lib.HIST_REC32), dtype=np.float32)
+            r32[..., :5] = rec[..., :5].astype(np.float32)
+            self.h32 = torch.from_numpy(r32).to(device)
+        a = tracker.axes()
+        self.desc = _lib.History(self.h64.data_ptr(),
+                                 self.h32.data_ptr() if self.h32 is not None else None,
+                                 cap, head, nt, Xi, Zi, float(a["min_t"]), float(a["min_x"]),
+                                 float(a["min_z"]), float(a["dt"]), float(a["dx"]),
+                                 float(a["dz"]))
+
+
+class DeviceLattice:
+    def __init__(self, lat, device="cuda"):
+        tab = np.stack([lat.coords[:, 0], lat.coords[:, 1], lat.n_vec[:, 0], lat.n_vec[:, 1],
+                        lat.tau_vec[:, 0], lat.tau_vec[:, 1]], axis=1)
+        self.tab = torch.from_numpy(np.ascontiguousarray(tab)).to(device)
+        self.dist = torch.from_numpy(np.array(lat.distance, dtype=np.float64)).to(device)
+        self.rho = torch.from_numpy(np.array(lat.rho, dtype=np.float64)).to(device)
+        self.desc = _lib.Lattice(self.tab.data_ptr(), tab.shape[0], len(lat.distance),
+                                 float(lat.min_x), float(lat.delta_x), self.dist.data_ptr(),
+                                 self.rho.data_ptr())
+
+
+def wake_params(position, beam, formation_length, n_fl, Nx, Nz, csr_scaling, mesh_x_axis,
+                mesh_z_axis, observers=None):
+    """beam: dict with sigma_x, sigma_z, slope, mean_x; mesh axes (start, stop, n)."""
+    return _lib.WakeParams(float(position), float(beam["sigma_x"]), float(beam["sigma_z"]),
+                           float(beam["slope"][0]), float(beam["slope"][1]),
+                           float(beam["mean_x"]), float(formation_length), float(n_fl),
+                           float(csr_scaling), int(Nx), int(Nz), int(mesh_x_axis[2]),
+                           int(mesh_z_axis[2]), float(mesh_x_axis[0]), float(mesh_x_axis[1]),
+                           float(mesh_z_axis[0]), float(mesh_z_axis[1]),
+                           observers.data_ptr() if observers is not None else None)
+
+
+def rel_max(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    den = np.nanmax(np.abs(b))
+    if den == 0:
+        return float(np.nanmax(np.abs(a - b)))
+    return float(np.nanmax(np.abs(a - b)) / den)

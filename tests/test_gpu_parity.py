@@ -1,0 +1,255 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle and the
+reference's golden vectors, at sizes the oracle finishes in seconds.
+
+Tolerances (float64 throughout):
+  * per-particle cell indices and per-cell counts: bit-exact;
+  * interpolate3D / interpolate1D / the integrand grids: bit-exact (same IEEE
+    operation sequence, no fused multiply-add);
+  * deposited grids: |d| <= 1e-12 * max (64-bit fixed-point sums against the
+    reference's serial float64 sums);
+  * density-function fields, history and kicks: <= 1e-9 * max;
+  * wakes (dE_dct, x_kick): <= 1e-10 * max|W| (summation order differs).
+"""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import cpu_ref as O
+from tests.gpu_util import DeviceHistory, DeviceLattice, rel_max, wake_params
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda"
+LAT_ELEMENTS = [dict(type="drift", L=0.1, nsep=1), dict(type="dipole", L=0.3, angle=0.05, nsep=1),
+                dict(type="drift", L=0.4, nsep=2), dict(type="dipole", L=0.3, angle=-0.05, nsep=1)]
+
+
+def _g(golden_dir, name):
+    return np.load(os.path.join(golden_dir, name))
+
+
+def _t(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(DEV)
+
+
+@pytest.fixture(scope="module")
+def K():
+    from pydfcsr_amd import _lib, kernels
+    _lib.load()
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    return kernels
+
+
+@pytest.fixture(scope="module")
+def ws():
+    from pydfcsr_amd import _lib
+    return _lib.Workspace(DEV)
+
+
+@pytest.fixture(scope="module")
+def golden_history(golden_dir):
+    d = _g(golden_dir, "ref_history.npz")
+    T = O.DFTracker(xbins=40, zbins=48, filter_order=2, filter_window=7,
+                    velocity_threhold=1000.0, upper_limit=40)
+    for k in range(len(d["times"])):
+        x, z, px = d["inputs"][k]
+        T.get_DF(x, z, px, float(d["times"][k]))
+        T.append_DF()
+        T.append_interpolant(float(d["fls"][k]), float(d["n_fl"]))
+        if k >= 1:
+            T.build_interpolant()
+    return d, T
+
+
+# -- a1 ------------------------------------------------------------------------------
+def test_beam_moments(K, ws):
+    rng = np.random.default_rng(11)
+    z = rng.normal(3e-5, 60e-6, 200003)
+    x = 0.7 * z + rng.normal(-1e-5, 20e-6, z.size)
+    m = K.beam_moments(_t(x), _t(z), ws)
+    ref = O.beam_moments(x, z)
+    for k in ("mean_x", "mean_z", "sigma_x", "sigma_z", "mean_xt", "sigma_xt", "slice_sigma_x"):
+        assert abs(m[k] - ref[k]) <= 1e-12 * max(abs(ref[k]), ref["sigma_x"]), k
+    assert np.allclose(m["slope"], ref["slope"], rtol=1e-11, atol=1e-17)
+    assert m["n_slice"] == ref["n_slice"]
+    m2 = K.beam_moments(_t(x), _t(z), ws)
+    assert m2 == m   # deterministic
+
+
+# -- a3 ------------------------------------------------------------------------------
+def test_histogram_golden_and_cells(K, ws, golden_dir):
+    d = _g(golden_dir, "ref_histogram_cic_2d.npz")
+    a = d["args"]
+    args = (int(a[0]), float(a[1]), float(a[2]), int(a[3]), float(a[4]), float(a[5]))
+    q1, q2 = _t(d["q1"]), _t(d["q2"])
+    h1, cells, counts = K.histogram_cic_2d(q1, q2, None, *args, ws, want_cells=True)
+    h2 = K.histogram_cic_2d(q1, q2, _t(d["w"]), *args, ws)
+    assert rel_max(h1.cpu(), d["h_ones"]) <= 1e-12
+    assert rel_max(h2.cpu(), d["h_w"]) <= 1e-12
+    i1, i2, _, _ = O.cic_cells(d["q1"], d["q2"], *args)
+    c = cells.cpu().numpy()
+    assert np.array_equal(c[:, 0], i1) and np.array_equal(c[:, 1], i2)
+    assert np.array_equal(counts.cpu().numpy(), O.cell_counts(i1, i2, args[0], args[3]))
+
+
+def test_histogram_edges_and_determinism(K, ws):
+    # particles on nodes, on the far edge, below the grid, zero weights
+    q1 = np.array([0.0, 1.0, 4.0, -0.25, 3.5, 2.0, 3.999999999])
+    q2 = np.array([0.0, 4.0, 4.0, 4.0, 9.0, 10.0, 9.999999999])
+    w = np.array([1.0, 2.0, 3.0, 1.0, 0.0, 5.0, 1.0])
+    h = K.histogram_cic_2d(_t(q1), _t(q2), _t(w), 4, 0.0, 4.0, 5, 0.0, 10.0, ws).cpu().numpy()
+    ref = O.histogram_cic_2d_loop(q1, q2, w, 4, 0.0, 4.0, 5, 0.0, 10.0)
+    assert np.allclose(h, ref, rtol=0, atol=1e-15 * np.abs(ref).max())
+    # empty input
+    e = K.histogram_cic_2d(_t(np.zeros(0)), _t(np.zeros(0)), None, 3, 0.0, 1.0, 3, 0.0, 1.0, ws)
+    assert torch.count_nonzero(e) == 0
+    # 2^20 particles: bit-identical on repeat
+    rng = np.random.default_rng(3)
+    x = _t(rng.normal(size=1 << 20))
+    z = _t(rng.normal(size=1 << 20))
+    px = _t(rng.normal(size=1 << 20) * 1e-3)
+    d1, v1 = K.deposit_density_vx(x, z, px, 64, -5.0, 5.0, 64, -5.0, 5.0, ws)
+    d1, v1 = d1.clone(), v1.clone()
+    d2, v2 = K.deposit_density_vx(x, z, px, 64, -5.0, 5.0, 64, -5.0, 5.0, ws)
+    assert torch.equal(d1, d2) and torch.equal(v1, v2)
+    xr, zr, pr = x.cpu().numpy(), z.cpu().numpy(), px.cpu().numpy()
+    assert rel_max(d1.cpu(), O.histogram_cic_2d(xr, zr, np.ones(xr.size), 64, -5.0, 5.0, 64, -5.0, 5.0)) <= 1e-12
+    assert rel_max(v1.cpu(), O.histogram_cic_2d(xr, zr, pr, 64, -5.0, 5.0, 64, -5.0, 5.0)) <= 1e-12
+
+
+# -- a11 / a12 ----------------------------------------------------------------------
+def test_interpolate_golden_bitexact(K, golden_dir):
+    from pydfcsr_amd import _lib
+    d = _g(golden_dir, "ref_interpolate.npz")
+    data = d["data3"]
+    nt, nx, nz = data.shape
+    rec = np.zeros((nt, nx, nz, _lib.HIST_REC64))
+    rec[..., 0] = data
+    h = _t(rec)
+    H = _lib.History(h.data_ptr(), None, nt, 0, nt, nx, nz, *[float(v) for v in d["mins"]],
+                     *[float(v) for v in d["dels"]])
+    r = K.interpolate3D(H, 0, _t(d["q3"]), DEV).cpu().numpy()
+    assert np.array_equal(r, d["r3"])
+    tab = np.zeros((d["data1"].size, 6))
+    tab[:, 3] = d["data1"]
+    tt = _t(tab)
+    dist = _t(np.array([1.0]))
+    L = _lib.Lattice(tt.data_ptr(), tab.shape[0], 1, float(d["min1"]), float(d["del1"]),
+                     dist.data_ptr(), dist.data_ptr())
+    r1 = K.interpolate1D(L, 3, _t(d["q1"])).cpu().numpy()
+    assert np.array_equal(r1, d["r1"])
+
+
+# -- a4-a6 ---------------------------------------------------------------------------
+@pytest.mark.parametrize("tag,bins", [("corr", (40, 48)), ("fallback", (100, 100))])
+def test_get_df_golden(golden_dir, tag, bins):
+    from pydfcsr_amd.deposit import DF_tracker
+    d = _g(golden_dir, f"ref_get_df_{tag}.npz")
+    tr = DF_tracker(dict(xbins=40, zbins=48, filter_order=2, filter_window=7,
+                         velocity_threhold=1000.0), device=DEV)
+    m = tr.get_DF(_t(d["x"]), _t(d["z"]), _t(d["px"]), 0.25)
+    assert (m.nbx, m.nbz) == bins
+    f = tr.current_fields()
+    assert np.allclose(f["x_grids"], d["x_grids"], rtol=1e-13, atol=0)
+    for k in ("density", "vx", "density_x", "density_z", "vx_x"):
+        assert rel_max(f[k].cpu(), d[k]) <= 1e-9, k
+
+
+# -- a8-a10 --------------------------------------------------------------------------
+def test_history_golden(golden_dir):
+    from pydfcsr_amd.deposit import DF_tracker
+    d = _g(golden_dir, "ref_history.npz")
+    tr = DF_tracker(dict(xbins=40, zbins=48, filter_order=2, filter_window=7,
+                         velocity_threhold=1000.0, upper_limit=40), device=DEV, capacity=3)
+    flags = []
+    for k in range(len(d["times"])):
+        x, z, px = d["inputs"][k]
+        tr.get_DF(_t(x), _t(z), _t(px), float(d["times"][k]))
+        tr.append_DF()
+        flags.append(int(tr.append_interpolant(float(d["fls"][k]), float(d["n_fl"]))))
+        if k >= 1:
+            tr.build_interpolant()
+    assert flags == list(d["regrid_flags"])
+    assert np.array_equal(np.array([m.t for m in tr.log]), d["time_interp"])
+    ax = np.array([tr.min_x, tr.min_y, tr.min_z, tr.delta_x, tr.delta_y, tr.delta_z])
+    assert np.allclose(ax, d["axes"], rtol=1e-13, atol=0)
+    hf = tr.history_fields()
+    for f in ("density", "density_x", "density_z", "vx", "vx_x"):
+        assert rel_max(hf[f].cpu(), d[f]) <= 1e-9, f
+
+
+# -- a13 -----------------------------------------------------------------------------
+def test_integrand_golden_bitexact(K, golden_dir, golden_history):
+    _, T = golden_history
+    w = _g(golden_dir, "ref_wake.npz")
+    L = O.LatticeTables(LAT_ELEMENTS)
+    DH = DeviceHistory(T, DEV, wrap=2)
+    DL = DeviceLattice(L, DEV)
+    i = 6
+    s, x = 0.3 + w["zmesh"][i], w["xmesh"][i]
+    xps = [w["dbg_xp_w"], w["dbg_xp_n"], w["dbg_xp_n"]]
+    sps = [w["dbg_sp1"], w["dbg_sp2"], w["dbg_sp3"]]
+    for r in range(3):
+        Iz, Ix = K.csr_integrand(DH.desc, DL.desc, s, x, 0.3, _t(xps[r]), _t(sps[r]))
+        assert np.array_equal(Iz.cpu().numpy(), w[f"dbg_Iz{r + 1}"])
+        assert np.array_equal(Ix.cpu().numpy(), w[f"dbg_Ix{r + 1}"])
+
+
+# -- a13-a15 -------------------------------------------------------------------------
+def _golden_wake_inputs(golden_dir, golden_history, wrap=0):
+    _, T = golden_history
+    w = _g(golden_dir, "ref_wake.npz")
+    L = O.LatticeTables(LAT_ELEMENTS)
+    beam = dict(sigma_x=float(w["sigma_x"]), sigma_z=float(w["sigma_z"]), slope=w["slope"],
+                mean_x=float(w["mean_x"]))
+    xr, zr = w["xrange"], w["zrange"]
+    P = wake_params(0.3, beam, 0.05, 1.0, 12, 16, float(w["csr_scaling"]),
+                    (xr[0], xr[-1], xr.size), (zr[0], zr[-1], zr.size))
+    return w, beam, DeviceHistory(T, DEV, wrap=wrap), DeviceLattice(L, DEV), P
+
+
+def test_wake_golden(K, ws, golden_dir, golden_history):
+    w, beam, DH, DL, P = _golden_wake_inputs(golden_dir, golden_history)
+    out = K.wake_mesh(DH.desc, DL.desc, P, 0, DEV, ws)
+    assert np.allclose(out["xmesh"].cpu().numpy(), w["xmesh"], rtol=0, atol=1e-18)
+    assert np.array_equal(out["zmesh"].cpu().numpy(), w["zmesh"])
+    assert rel_max(out["dE_dct"].cpu().view(3, 4), w["dE_dct"]) <= 1e-10
+    assert rel_max(out["x_kick"].cpu().view(3, 4), w["x_kick"]) <= 1e-10
+    # deterministic, and independent of where the ring wraps
+    out2 = K.wake_mesh(DH.desc, DL.desc, P, 0, DEV, ws)
+    assert torch.equal(out["dE_dct"], out2["dE_dct"]) and torch.equal(out["x_kick"], out2["x_kick"])
+    _, _, DHw, _, _ = _golden_wake_inputs(golden_dir, golden_history, wrap=3)
+    out3 = K.wake_mesh(DHw.desc, DL.desc, P, 0, DEV, ws)
+    assert torch.equal(out["dE_dct"], out3["dE_dct"]) and torch.equal(out["x_kick"], out3["x_kick"])
+
+
+@pytest.mark.parametrize("slope0,key", [(2.5, "chirp_pos"), (-2.5, "chirp_neg")])
+def test_wake_chirp_branch_golden(K, ws, golden_dir, golden_history, slope0, key):
+    w, beam, DH, DL, _ = _golden_wake_inputs(golden_dir, golden_history)
+    b = dict(beam, slope=np.array([slope0, w["slope"][1]]))
+    rows = w["chirp_rows"]
+    obs = _t(np.stack([w["zmesh"][rows], w["xmesh"][rows]], axis=1))
+    P = wake_params(0.3, b, 0.05, 1.0, 12, 16, float(w["csr_scaling"]), (0.0, 0.0, 1),
+                    (0.0, 0.0, len(rows)), observers=obs)
+    out = K.wake_mesh(DH.desc, DL.desc, P, 0, DEV, ws)
+    res = np.stack([out["dE_dct"].cpu().numpy(), out["x_kick"].cpu().numpy()], axis=1)
+    assert rel_max(res[:, 0], w[key][:, 0]) <= 1e-10
+    assert rel_max(res[:, 1], w[key][:, 1]) <= 1e-10
+
+
+# -- a16 -----------------------------------------------------------------------------
+def test_apply_wakes_golden(K, ws, golden_dir):
+    a = _g(golden_dir, "ref_apply_wakes.npz")
+    x, z = _t(a["x"]), _t(a["z"])
+    px, pz = _t(a["px0"]), _t(a["pz0"])
+    m = K.beam_moments(x, z, ws)
+    xr, zr = a["xrange"], a["zrange"]
+    K.apply_wakes(px, pz, x, z, _t(a["dE_dct"]), _t(a["x_kick"]), xr[0], xr[-1], zr[0], zr[-1],
+                  m["slope0"], m["slope1"], float(a["step_size"]), float(a["energy"]), 1,
+                  xr.size, zr.size)
+    dpz_ref = a["pz1"] - a["pz0"]
+    dpx_ref = a["px1"] - a["px0"]
+    assert rel_max(pz.cpu().numpy() - a["pz0"], dpz_ref) <= 1e-9
+    assert rel_max(px.cpu().numpy() - a["px0"], dpx_ref) <= 1e-9

@@ -1,0 +1,191 @@
+#!/usr/bin/env python
+"""CSR step benchmark on one MI355X.
+
+Workload (default ``chicane``): plain float64 CSR on a 4-bend chicane bunch
+compressor.  Beam: 1e6 macroparticles, 1 nC, 1 GeV, Gaussian sigma_z = 100 um,
+sigma_x = 30 um, linear chirp -10 /m, uncorrelated energy spread 1e-4, seed 0.
+Chicane: 4 x 0.3 m dipoles at +, -, -, + 0.05 rad (pole faces as the
+reference's chicane example), drifts 2 m, 1 m, 2 m, 400 steps.  Deposition
+128 x 128 with Savitzky-Golay 7/2 (the reference's frac <= 5 rule may select
+its 100 x 100 / window 5 grid), wake mesh 64 x 64, integration 200 x 200,
+n_formation_length 4, full history kept on the device.
+
+A step is one lattice step of the simulation: tracking plus one CSR step
+(deposition -> density functions -> history -> retarded integral -> kicks).
+W untimed warm-up steps fill the retarded-time window, then exactly K steps
+are timed.  ``value`` is the integrand throughput of those K steps (points
+evaluated by the retarded integral per second, one point = one (x', s')
+source for one mesh point); ms_per_step is their wall time per step.
+
+The JSON line also carries the roofline of the dominant kernel (the wake
+integral, timed with HIP events on the stream it runs on) and a CPU baseline:
+the oracle's C/OpenMP restatement of the same integral on the same history,
+on a bounded sample of mesh points, on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from collections import OrderedDict
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "CSR step ms + integrand Gpts/s, mixed vs FP64 @1e-4 rel err; serve p50/p99"
+FP64_PEAK_TFLOPS = 78.6     # MI355X vector/matrix FP64 peak
+FLOPS_PER_POINT = 214       # FP64 ops per integrand point
+
+
+def chicane_config(n_particle=1_000_000, nbins=128, window=7, mesh=64, Nxz=200):
+    lat = OrderedDict(step_size=6.2 / 400)
+    lat["B1"] = dict(type="dipole", L=0.3, angle=0.05, E1=0.0, E2=0.05, nsep=1)
+    lat["D1"] = dict(type="drift", L=2.0, nsep=1)
+    lat["B2"] = dict(type="dipole", L=0.3, angle=-0.05, E1=-0.05, E2=0.0, nsep=1)
+    lat["D2"] = dict(type="drift", L=1.0, nsep=1)
+    lat["B3"] = dict(type="dipole", L=0.3, angle=-0.05, E1=0.0, E2=-0.05, nsep=1)
+    lat["D3"] = dict(type="drift", L=2.0, nsep=1)
+    lat["B4"] = dict(type="dipole", L=0.3, angle=0.05, E1=0.05, E2=0.0, nsep=1)
+    return {
+        "input_beam": dict(style="gaussian", n_particle=n_particle, charge=1e-9, energy=1e9,
+                           sigma_x=30e-6, sigma_z=100e-6, chirp=-10.0, sigma_delta=1e-4, seed=0),
+        "input_lattice": dict(lattice_input_file="<chicane>", _config=lat),
+        "particle_deposition": dict(xbins=nbins, zbins=nbins, xlim=5, zlim=5, filter_order=2,
+                                    filter_window=window, velocity_threhold=1000),
+        "CSR_integration": dict(n_formation_length=4, zbins=Nxz, xbins=Nxz),
+        "CSR_computation": dict(xbins=mesh, zbins=mesh, xlim=5, zlim=5, apply_CSR=1,
+                                transverse_on=1, write_wakes=False),
+    }
+
+
+def cpu_baseline(sim, budget_s=15.0, max_points=None):
+    """The oracle's C/OpenMP integral on the simulation's current history."""
+    import numpy as np
+    from oracle import csr_ref_c as C
+    tr = sim.DF_tracker
+    rec = tr.hist.cpu().numpy()
+    keep = C._Keep()
+    H = C.hist_from_records(rec, tr.head, tr.nt, (float(tr.min_x), float(tr.min_y),
+                                                  float(tr.min_z), float(tr.delta_x),
+                                                  float(tr.delta_y), float(tr.delta_z)), keep)
+    L = C.lat_from_tables(sim.lattice, keep)
+    b = sim.beam
+    beam = dict(sigma_x=b._sigma_x, sigma_z=b._sigma_z, slope=b._slope, mean_x=b._mean_x)
+    P = sim.wake_params()
+    obs = np.stack([sim.CSR_zmesh.cpu().numpy(), sim.CSR_xmesh.cpu().numpy()], axis=1)
+    ip = sim.integration_params
+    per_point = sim.integrand_points() // sim._mesh_M()
+    threads = C.threads()
+    gpu_dE = sim.dE_dct.reshape(-1).cpu().numpy()
+    # probe, then size the sample to the time budget
+    n0 = min(threads, obs.shape[0])
+    t0 = time.perf_counter()
+    C.wake(H, L, P.t, beam, P.formation_length, P.n_formation_length, ip.xbins, ip.zbins,
+           P.csr_scaling, obs[:n0], threads)
+    dt0 = time.perf_counter() - t0
+    k = int(min(obs.shape[0], max(n0, n0 * budget_s / max(dt0, 1e-6))))
+    if max_points:
+        k = min(k, max_points)
+    k = max(threads * (k // threads), n0)
+    t0 = time.perf_counter()
+    dE, _ = C.wake(H, L, P.t, beam, P.formation_length, P.n_formation_length, ip.xbins, ip.zbins,
+                   P.csr_scaling, obs[:k], threads)
+    dt = time.perf_counter() - t0
+    err = float(np.abs(dE - gpu_dE[:k]).max() / max(np.abs(gpu_dE).max(), 1e-300))
+    return {"value": k * per_point / dt / 1e9, "unit": "Gpts/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{k} of {obs.shape[0]} mesh points of the last timed step "
+                      f"({per_point} integrand points each, {dt:.1f} s)",
+            "max_rel_diff_vs_gpu_dE": err}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--workload", default="chicane", choices=["chicane"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU mesh "
+              "split is a design note, not a measured path.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from pydfcsr_amd.csr import CSR2D
+
+    dev = torch.device("cuda:0")
+    cfg = chicane_config()
+    t_init = time.perf_counter()
+    sim = CSR2D(config=cfg, device=dev, precision=args.precision)
+    it = sim.iter_steps()
+    for i in range(args.warmup):
+        next(it)
+        if (i + 1) % 50 == 0:
+            print(f"warmup step {i + 1}/{args.warmup}: s = {sim.beam.position:.4f} m, "
+                  f"history {sim.DF_tracker.nt} slices on {sim.DF_tracker.Xi}x"
+                  f"{sim.DF_tracker.Zi}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    sim.timing = True
+    sim.timing_events.clear()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        next(it)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    sim.timing = False
+    ev = sim.timing_events
+    csr_ms = [a.elapsed_time(d) for a, _, _, d, _, _ in ev]
+    wake_ms = [b.elapsed_time(c) for _, b, c, _, _, _ in ev]
+    pts = [p for *_, p, _ in ev]
+    gpts = sum(pts) / (sum(wake_ms) * 1e-3) / 1e9
+    wake_avg_ms = float(np.mean(wake_ms))
+    achieved = FLOPS_PER_POINT * float(np.mean(pts)) / (wake_avg_ms * 1e-3) / 1e12
+    tr = sim.DF_tracker
+    res = {
+        "metric": METRIC,
+        "value": round(gpts, 4),
+        "unit": "Gpts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round((t1 - t0) * 1e3 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "data": "synthetic (seeded Gaussian beam, chicane lattice)",
+        "config": {"workload": "chicane bunch compressor: 1e6 particles, 1 nC, 1 GeV, "
+                               "4 bends x 0.3 m at 0.05 rad, drifts 2/1/2 m, 400 steps; "
+                               "deposition 128x128 SG 7/2; wake mesh 64x64; integration "
+                               "200x200; n_fl 4",
+                   "precision": args.precision,
+                   "steps_timed": f"{ev[0][5] if ev else 0} mesh points x "
+                                  f"{(pts[0] // ev[0][5]) if ev else 0} points per step",
+                   "s_start_m": None},
+        "csr_step_ms": round(float(np.mean(csr_ms)), 3),
+        "wake_ms": round(wake_avg_ms, 3),
+        "history": {"slices": tr.nt, "Xi": tr.Xi, "Zi": tr.Zi,
+                    "bytes": int(tr.footprint_bytes()), "regrids": tr.n_regrid},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+                     "traffic": None,
+                     "kernel": "wake_fp64_kernel", "flops_per_point": FLOPS_PER_POINT},
+        "init_s": round(t0 - t_init, 2),
+    }
+    res["config"]["s_start_m"] = round(float(sim.beam.position), 4)
+    if not args.no_cpu_baseline:
+        print("cpu baseline: oracle C/OpenMP integral on the same history", file=sys.stderr,
+              flush=True)
+        res["cpu_baseline"] = cpu_baseline(sim, budget_s=args.cpu_budget)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

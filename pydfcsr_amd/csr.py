@@ -44,6 +44,8 @@ class CSR2D:
         self.parallel = False
         self.formation_length = None
         self.ws = _lib.Workspace(self.device)
+        self.timing = False        # record HIP events around each CSR step
+        self.timing_events = []    # (start, wake_start, wake_end, end, points, M)
         if input_file is not None or config is not None:
             self.parse_input(input_file, config)
             self.input_file = input_file
@@ -122,6 +124,14 @@ class CSR2D:
     def run(self, stop_time=None, debug=False, max_steps=None, on_step=None):
         """The reference's element/step loop.  ``max_steps`` stops after that
         many steps; ``on_step(self, step_count)`` is called after each step."""
+        for step_count in self.iter_steps(stop_time=stop_time, debug=debug):
+            if on_step is not None:
+                on_step(self, step_count)
+            if max_steps is not None and step_count >= max_steps:
+                return
+
+    def iter_steps(self, stop_time=None, debug=False):
+        """Generator form of run(): yields the step count after each step."""
         step_count = 1
         DL = self.lattice.step_size
         ele_count = 0
@@ -137,6 +147,7 @@ class CSR2D:
             typ = cfg["type"]
             steps = self.lattice.steps_per_element[ele_count]
             if (not skip_ele) and ele_count > 0:
+                # the part of the step still in the previous element (CSR.py:228-236)
                 DL_1 = self.lattice.distance[ele_count - 1] - self.beam.position
                 if DL_1 > 1.0e-6:
                     self._track(ele_prev, DL_1, exit=True, update_step=False)
@@ -167,25 +178,29 @@ class CSR2D:
                     skip_ele = False
                 else:
                     self._track(ele, DL)
+                ev = self._event() if self.timing else None
                 if debug or self.CSR_params.compute_CSR:
                     self.update_history()
                 if self.CSR_params.compute_CSR and step % self.lattice.nsep[ele_count] == 0:
                     self.get_CSR_mesh()
+                    ev_w0 = self._event() if self.timing else None
                     self.calculate_2D_CSR()
+                    ev_w1 = self._event() if self.timing else None
+                    pts = self.integrand_points()
                     if self.CSR_params.apply_CSR:
                         self.beam.apply_wakes(self.dE_dct, self.x_kick, self.mesh_x_axis,
                                               self.mesh_z_axis, DL * self.lattice.nsep[ele_count],
                                               self.CSR_params.transverse_on)
+                    if self.timing:
+                        self.timing_events.append((ev, ev_w0, ev_w1, self._event(), pts,
+                                                   self._mesh_M()))
                 self.update_statistics(step=step_count)
                 if self.verbose:
                     print(f"Finish step {step_count}, s = {self.beam.position},  in "
-                          f"{time.time() - t0} seconds")
-                if on_step is not None:
-                    on_step(self, step_count)
+                          f"{time.time() - t0} seconds", flush=True)
+                yield step_count
                 step_count += 1
                 if stop_time and self.beam.position > stop_time:
-                    return
-                if max_steps is not None and step_count > max_steps:
                     return
             ele_prev = ele
             ele_count += 1
@@ -247,6 +262,18 @@ class CSR2D:
         self.x_kick = self._wake_out["x_kick"].view(p.xbins, p.zbins)
         self.CSR_xmesh = self._wake_out["xmesh"]
         self.CSR_zmesh = self._wake_out["zmesh"]
+
+    def _event(self):
+        e = torch.cuda.Event(enable_timing=True)
+        e.record()
+        return e
+
+    def integrand_points(self):
+        """Integrand evaluations of the last wake: M * P with P = 4 Nx Nz
+        (three regions) or 5 Nx Nz (four, chirp branch), CSR.py:541-597."""
+        ip = self.integration_params
+        per = (5 if abs(self.beam._slope[0]) > 1 else 4) * ip.xbins * ip.zbins
+        return self._mesh_M() * per
 
     def _mesh_M(self):
         return int(self.CSR_params.xbins) * int(self.CSR_params.zbins)

@@ -1,13 +1,16 @@
-// a3: cloud-in-cell deposition with 64-bit fixed-point atomics.
+// a3: cloud-in-cell deposition with two-word fixed-point atomics.
 //
 // Replaces histogram_cic_2d (deposit.py:42-87) and the pair of calls in
 // DF_tracker.get_DF (deposit.py:172-182).  Cell indices use the reference's
 // FP64 expression cell = (q - start) * (1 / ((end - start) / nbins)),
 // i = floor(cell), S = 1 - (cell - i), evaluated without fused multiply-add
 // so they are bit-identical to the reference.  Each corner weight w*S1*S2 is
-// rounded once to a multiple of 2^-k and summed as a 64-bit integer: integer
-// addition is associative, so the grid is the same for every launch order.
-// k is chosen per call from n * max|w| so no partial sum can overflow.
+// scaled by 2^k and split into a signed high word (multiples of 2^32) and an
+// unsigned low word (< 2^32, rounded); both are summed with 64-bit integer
+// atomics.  Integer addition is associative, so the grid is the same for
+// every launch order, and with k chosen from n * max|w| (n*max|w|*2^k < 2^94)
+// no word can overflow while the quantum 2^-k is ~1e-22 of the largest
+// weight: the sums are exact to far below float64 rounding.
 #include "common.h"
 
 namespace dfcsr {
@@ -18,7 +21,8 @@ constexpr int kMaxBlocks = 2048;
 
 // ws layout: [0] scale for grid A, [1] scale for grid B (doubles),
 //            [8 .. 8+kMaxBlocks) block max|w| partials,
-//            then 2 * nb1*nb2 int64 accumulators (8-byte aligned).
+//            then per grid (A, B): nb1*nb2 int64 high words and nb1*nb2
+//            uint64 low words.
 constexpr int kHdr = 8 + kMaxBlocks;
 
 int nblocks_for(int64_t n) {
@@ -39,14 +43,15 @@ __global__ __launch_bounds__(kThreads) void absmax_partial(const double *__restr
   if (threadIdx.x == 0) part[blockIdx.x] = m;
 }
 
-// Picks 2^k with n*max|w|*2^k < 2^61 (k capped at 60): scale[0] for unit
-// weights (grid A), scale[1] for w (grid B, only if has_w).
+// Picks 2^k with n*max|w|*2^k < 2^94: every scaled weight s has |s| < 2^94/n,
+// so n high words (|s|/2^32 each) stay below 2^62 and n low words (< 2^32
+// each, n < 2^31) below 2^63.  scale[0]: unit weights, scale[1]: w.
 __device__ double pick_scale(double bound) {
   if (!(bound > 0.0) || !isfinite(bound)) return ldexp(1.0, 60);
   int e;
   frexp(bound, &e);   // bound < 2^e
-  int k = 61 - e;
-  if (k > 60) k = 60;
+  int k = 94 - e;
+  if (k > 900) k = 900;
   return ldexp(1.0, k);
 }
 
@@ -71,9 +76,15 @@ __global__ __launch_bounds__(kThreads) void zero_i64(int64_t *__restrict__ a, in
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += stride) c[i] = 0;
 }
 
-__device__ inline void fx_add(int64_t *acc, double v, double scale) {
-  long long q = (long long)rint(v * scale);
-  if (q != 0) atomicAdd(reinterpret_cast<unsigned long long *>(acc), (unsigned long long)q);
+// acc[0..ncell) high words, acc[ncell..2 ncell) low words
+__device__ inline void fx_add(int64_t *acc, int64_t cell, int64_t ncell, double v, double scale) {
+  const double s = v * scale;                   // exact: scale is a power of two
+  const double h = floor(s * 0x1p-32);
+  const double l = s - h * 0x1p32;              // in [0, 2^32]
+  const long long hi = (long long)h;
+  const unsigned long long lo = (unsigned long long)rint(l);
+  if (hi != 0) atomicAdd(reinterpret_cast<unsigned long long *>(acc + cell), (unsigned long long)hi);
+  if (lo != 0) atomicAdd(reinterpret_cast<unsigned long long *>(acc + ncell + cell), lo);
 }
 
 // One pass over the particles.  Grid A: weights wA (unit weights when wA is
@@ -112,31 +123,39 @@ __global__ __launch_bounds__(kThreads) void cic_kernel(
     // Corner weights in the reference's association: (w*S1)*S2.
     double t00 = wa * S1 * S2, t01 = wa * S1 * (1.0 - S2);
     double t10 = wa * (1.0 - S1) * S2, t11 = wa * (1.0 - S1) * (1.0 - S2);
-    int64_t base = (int64_t)i1 * nb2 + i2;
-    if (a0 && b0) fx_add(accA + base, t00, scA);
-    if (a0 && b1) fx_add(accA + base + 1, t01, scA);
-    if (a1 && b0) fx_add(accA + base + nb2, t10, scA);
-    if (a1 && b1) fx_add(accA + base + nb2 + 1, t11, scA);
+    const int64_t nc = (int64_t)nb1 * nb2;
+    const int64_t base = (int64_t)i1 * nb2 + i2;
+    if (a0 && b0) fx_add(accA, base, nc, t00, scA);
+    if (a0 && b1) fx_add(accA, base + 1, nc, t01, scA);
+    if (a1 && b0) fx_add(accA, base + nb2, nc, t10, scA);
+    if (a1 && b1) fx_add(accA, base + nb2 + 1, nc, t11, scA);
     if (wB) {
       double wb = wB[p];
       double u00 = wb * S1 * S2, u01 = wb * S1 * (1.0 - S2);
       double u10 = wb * (1.0 - S1) * S2, u11 = wb * (1.0 - S1) * (1.0 - S2);
-      if (a0 && b0) fx_add(accB + base, u00, scB);
-      if (a0 && b1) fx_add(accB + base + 1, u01, scB);
-      if (a1 && b0) fx_add(accB + base + nb2, u10, scB);
-      if (a1 && b1) fx_add(accB + base + nb2 + 1, u11, scB);
+      if (a0 && b0) fx_add(accB, base, nc, u00, scB);
+      if (a0 && b1) fx_add(accB, base + 1, nc, u01, scB);
+      if (a1 && b0) fx_add(accB, base + nb2, nc, u10, scB);
+      if (a1 && b1) fx_add(accB, base + nb2 + 1, nc, u11, scB);
     }
   }
 }
 
+// total = hi * 2^32 + lo, normalised so the low part is < 2^32, then one
+// rounding to float64 and an exact power-of-two rescale.
 __global__ __launch_bounds__(kThreads) void fx_to_double(const int64_t *__restrict__ acc,
                                                          const double *__restrict__ scale,
                                                          int which, int64_t n,
                                                          double *__restrict__ out) {
   double inv = 1.0 / scale[which];
+  const uint64_t *lo_w = reinterpret_cast<const uint64_t *>(acc + n);
   int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    out[i] = (double)acc[i] * inv;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    uint64_t lo = lo_w[i];
+    int64_t hi = acc[i] + (int64_t)(lo >> 32);
+    double l = (double)(lo & 0xffffffffull);
+    out[i] = fma((double)hi, 0x1p32, l) * inv;
+  }
 }
 
 int grid_for(int64_t n) {
@@ -151,8 +170,12 @@ int deposit_impl(const char *who, const double *q1, const double *q2, const doub
                  double e2, double *outA, double *outB, int32_t *cell_index,
                  int32_t *cell_count, void *ws, int64_t ws_bytes, void *stream) {
 #pragma clang fp contract(off)
-  if (!q1 || !q2 || !outA || !ws) return set_error(DFCSR_EINVAL, "%s: NULL pointer", who);
+  // particle arrays may be NULL when there are no particles
+  if ((n > 0 && (!q1 || !q2)) || !outA || !ws)
+    return set_error(DFCSR_EINVAL, "%s: NULL pointer", who);
   if (n < 0 || nb1 < 1 || nb2 < 1) return set_error(DFCSR_EINVAL, "%s: bad sizes", who);
+  if (n >= ((int64_t)1 << 31))
+    return set_error(DFCSR_ERANGE, "%s: %lld particles >= 2^31", who, (long long)n);
   if ((int64_t)nb1 * nb2 > (int64_t)1 << 28)
     return set_error(DFCSR_ERANGE, "%s: grid %d x %d too large", who, nb1, nb2);
   if (ws_bytes < dfcsr_histogram_workspace_bytes(nb1, nb2))
@@ -161,7 +184,7 @@ int deposit_impl(const char *who, const double *q1, const double *q2, const doub
   double *hdr = static_cast<double *>(ws);
   int64_t ncell = (int64_t)nb1 * nb2;
   int64_t *accA = reinterpret_cast<int64_t *>(hdr + kHdr);
-  int64_t *accB = accA + ncell;
+  int64_t *accB = accA + 2 * ncell;
   // bins: inv_spacing = 1 / ((end - start) / nbins), deposit.py:55-58
   double inv1 = 1.0 / ((e1 - s1) / (double)nb1);
   double inv2 = 1.0 / ((e2 - s2) / (double)nb2);
@@ -170,7 +193,7 @@ int deposit_impl(const char *who, const double *q1, const double *q2, const doub
   if (wmax) hipLaunchKernelGGL(absmax_partial, dim3(nb), dim3(kThreads), 0, s, wmax, n, hdr + 8);
   hipLaunchKernelGGL(scale_final, dim3(1), dim3(kThreads), 0, s, (const double *)(hdr + 8), nb,
                      n, wmax ? 1 : 0, hdr);
-  hipLaunchKernelGGL(zero_i64, dim3(grid_for(2 * ncell)), dim3(kThreads), 0, s, accA, 2 * ncell,
+  hipLaunchKernelGGL(zero_i64, dim3(grid_for(4 * ncell)), dim3(kThreads), 0, s, accA, 4 * ncell,
                      cell_count, cell_count ? ncell : 0);
   // A lone weighted deposit (wA) uses the weighted scale[1] for grid A.
   int sA = wA ? 1 : 0;
@@ -179,7 +202,7 @@ int deposit_impl(const char *who, const double *q1, const double *q2, const doub
                      cell_count);
   hipLaunchKernelGGL(fx_to_double, dim3(grid_for(ncell)), dim3(kThreads), 0, s,
                      (const int64_t *)accA, (const double *)hdr, sA, ncell, outA);
-  if (wB && outB)
+  if (outB)   // zero-filled when there are no particles
     hipLaunchKernelGGL(fx_to_double, dim3(grid_for(ncell)), dim3(kThreads), 0, s,
                        (const int64_t *)accB, (const double *)hdr, 1, ncell, outB);
   DFCSR_LAUNCH_CHECK(who);
@@ -193,7 +216,7 @@ using namespace dfcsr;
 
 extern "C" int64_t dfcsr_histogram_workspace_bytes(int nbins_1, int nbins_2) {
   if (nbins_1 < 1 || nbins_2 < 1) return 0;
-  return (int64_t)kHdr * 8 + 2 * (int64_t)nbins_1 * nbins_2 * 8;
+  return (int64_t)kHdr * 8 + 4 * (int64_t)nbins_1 * nbins_2 * 8;
 }
 
 extern "C" int dfcsr_histogram_cic_2d(const double *q1, const double *q2, const double *w,
@@ -213,7 +236,7 @@ extern "C" int dfcsr_deposit_density_vx(const double *x, const double *z, const 
                                         double *density, double *vx, int32_t *cell_index,
                                         int32_t *cell_count, void *ws, int64_t ws_bytes,
                                         void *stream) {
-  if (!px || !vx) return set_error(DFCSR_EINVAL, "deposit_density_vx: NULL pointer");
+  if ((n > 0 && !px) || !vx) return set_error(DFCSR_EINVAL, "deposit_density_vx: NULL pointer");
   return deposit_impl("deposit_density_vx", x, z, nullptr, px, n, nbins_x, x_start, x_end,
                       nbins_z, z_start, z_end, density, vx, cell_index, cell_count, ws,
                       ws_bytes, stream);

@@ -114,3 +114,40 @@ def test_apply_wakes_golden(golden_dir):
     px1, pz1 = O.apply_wakes(a["x"], a["z"], a["px0"], a["pz0"], a["dE_dct"], a["x_kick"],
                              a["xrange"], a["zrange"], float(a["step_size"]), float(a["energy"]), 1)
     assert np.array_equal(px1, a["px1"]) and np.array_equal(pz1, a["pz1"])
+
+
+def test_c_oracle_matches_golden_wake(golden_dir):
+    """The oracle's C restatement (CPU baseline) against the reference's wakes."""
+    from oracle import csr_ref_c as C
+    h = _g(golden_dir, "ref_history.npz")
+    T, _ = _history(h)
+    w = _g(golden_dir, "ref_wake.npz")
+    L = O.LatticeTables(LAT_ELEMENTS)
+    keep = C._Keep()
+    H = C.hist_from_tracker(T, keep)
+    Lr = C.lat_from_tables(L, keep)
+    beam = dict(sigma_x=float(w["sigma_x"]), sigma_z=float(w["sigma_z"]), slope=w["slope"],
+                mean_x=float(w["mean_x"]))
+    obs = np.stack([w["zmesh"], w["xmesh"]], axis=1)
+    dE, xk = C.wake(H, Lr, 0.3, beam, 0.05, 1.0, 12, 16, float(w["csr_scaling"]), obs, 2)
+    for got, ref in ((dE, w["dE_dct"].ravel()), (xk, w["x_kick"].ravel())):
+        assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    for sl, key in ((2.5, "chirp_pos"), (-2.5, "chirp_neg")):
+        b = dict(beam, slope=np.array([sl, w["slope"][1]]))
+        rows = w["chirp_rows"]
+        dE, xk = C.wake(H, Lr, 0.3, b, 0.05, 1.0, 12, 16, float(w["csr_scaling"]), obs[rows], 2)
+        res = np.stack([dE, xk], axis=1)
+        assert np.abs(res - w[key]).max() <= 1e-12 * np.abs(w[key]).max()
+    # the ring-of-records layout reads the same history
+    data = np.stack([T.data[f] for f in O.FIELDS], axis=-1)
+    nt = data.shape[0]
+    rec = np.zeros((nt + 3,) + data.shape[1:3] + (6,))
+    head = 2
+    for i in range(nt):
+        rec[(head + i) % (nt + 3), :, :, :5] = data[i]
+    a = T.axes()
+    H2 = C.hist_from_records(rec, head, nt, (a["min_t"], a["min_x"], a["min_z"], a["dt"],
+                                             a["dx"], a["dz"]), keep)
+    dE2, xk2 = C.wake(H2, Lr, 0.3, beam, 0.05, 1.0, 12, 16, float(w["csr_scaling"]), obs, 1)
+    dE1, xk1 = C.wake(H, Lr, 0.3, beam, 0.05, 1.0, 12, 16, float(w["csr_scaling"]), obs, 1)
+    assert np.array_equal(dE1, dE2) and np.array_equal(xk1, xk2)

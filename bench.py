@@ -60,8 +60,9 @@ def chicane_config(n_particle=1_000_000, nbins=128, window=7, mesh=64, Nxz=200):
     }
 
 
-def cpu_baseline(sim, budget_s=15.0, max_points=None):
-    """The oracle's C/OpenMP integral on the simulation's current history."""
+def cpu_baseline(sim, gpu_out, budget_s=15.0, max_points=None):
+    """The oracle's C/OpenMP integral on the simulation's current history and
+    wake parameters; gpu_out is the FP64 device wake for the same inputs."""
     import numpy as np
     from oracle import csr_ref_c as C
     tr = sim.DF_tracker
@@ -74,11 +75,11 @@ def cpu_baseline(sim, budget_s=15.0, max_points=None):
     b = sim.beam
     beam = dict(sigma_x=b._sigma_x, sigma_z=b._sigma_z, slope=b._slope, mean_x=b._mean_x)
     P = sim.wake_params()
-    obs = np.stack([sim.CSR_zmesh.cpu().numpy(), sim.CSR_xmesh.cpu().numpy()], axis=1)
+    obs = np.stack([gpu_out["zmesh"], gpu_out["xmesh"]], axis=1)
     ip = sim.integration_params
     per_point = sim.integrand_points() // sim._mesh_M()
     threads = C.threads()
-    gpu_dE = sim.dE_dct.reshape(-1).cpu().numpy()
+    gpu_dE = gpu_out["dE_dct"]
     # probe, then size the sample to the time budget
     n0 = min(threads, obs.shape[0])
     t0 = time.perf_counter()
@@ -99,6 +100,41 @@ def cpu_baseline(sim, budget_s=15.0, max_points=None):
             "sample": f"{k} of {obs.shape[0]} mesh points of the last timed step "
                       f"({per_point} integrand points each, {dt:.1f} s)",
             "max_rel_diff_vs_gpu_dE": err}
+
+
+def compare_precisions(sim, reps=3):
+    """FP64 and mixed wake on the simulation's current state (same history,
+    same parameters): best-of-reps kernel time and the mixed form's error."""
+    import numpy as np
+    import torch
+    from pydfcsr_amd import _lib, kernels
+    H = sim.DF_tracker.descriptor()
+    L = sim.lattice.device_tables(sim.device)
+    P = sim.wake_params()
+    pts = sim.integrand_points()
+    res = {}
+    outs = {}
+    for name, prec in (("fp64", _lib.DFCSR_PREC_FP64), ("mixed", _lib.DFCSR_PREC_MIXED)):
+        ts = []
+        for _ in range(reps + 1):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record()
+            o = kernels.wake_mesh(H, L, P, prec, sim.device, sim.ws)
+            b.record()
+            torch.cuda.synchronize()
+            ts.append(a.elapsed_time(b))
+        outs[name] = {k: v.cpu().numpy() for k, v in o.items()}
+        t = min(ts[1:])
+        res[name] = {"wake_ms": round(t, 3), "gpts": round(pts / (t * 1e-3) / 1e9, 3)}
+    ref = outs["fp64"]
+    for k in ("dE_dct", "x_kick"):
+        d = outs["mixed"][k] - ref[k]
+        m = np.abs(ref[k]).max()
+        res["mixed"][f"max_rel_err_{k}"] = float(np.abs(d).max() / m)
+        res["mixed"][f"rms_rel_err_{k}"] = float(np.sqrt(np.mean(d * d)) / m)
+    res["mixed_speedup"] = round(res["fp64"]["wake_ms"] / res["mixed"]["wake_ms"], 3)
+    return res, outs
 
 
 def main():
@@ -123,7 +159,7 @@ def main():
     dev = torch.device("cuda:0")
     cfg = chicane_config()
     t_init = time.perf_counter()
-    sim = CSR2D(config=cfg, device=dev, precision=args.precision)
+    sim = CSR2D(config=cfg, device=dev, precision=args.precision, keep_fp32=True)
     it = sim.iter_steps()
     for i in range(args.warmup):
         next(it)
@@ -180,10 +216,13 @@ def main():
         "init_s": round(t0 - t_init, 2),
     }
     res["config"]["s_start_m"] = round(float(sim.beam.position), 4)
+    print("precision comparison: fp64 and mixed wake on the final state", file=sys.stderr,
+          flush=True)
+    res["precision_compare"], outs = compare_precisions(sim)
     if not args.no_cpu_baseline:
         print("cpu baseline: oracle C/OpenMP integral on the same history", file=sys.stderr,
               flush=True)
-        res["cpu_baseline"] = cpu_baseline(sim, budget_s=args.cpu_budget)
+        res["cpu_baseline"] = cpu_baseline(sim, outs["fp64"], budget_s=args.cpu_budget)
     print(json.dumps(res), flush=True)
 
 

@@ -98,11 +98,13 @@ EXPORTED = tuple(name for name, _, _ in _SIGNATURES)
 _lib = None
 
 
-def load(path: str = LIB_PATH):
-    """Loads and binds the library (once).  Raises DfcsrError if absent."""
+def load(path: str | None = None):
+    """Loads and binds the library (once).  Raises DfcsrError if absent.
+    DFCSR_LIB overrides the path (e.g. an alternative build)."""
     global _lib
-    if _lib is not None:
+    if _lib is not None and path is None:
         return _lib
+    path = path or os.environ.get("DFCSR_LIB", LIB_PATH)
     if not os.path.exists(path):
         raise DfcsrError(f"{path} not found: build it first (python -c 'import __graft_entry__ as g; "
                          "g.build()' or make -C pydfcsr_amd/csrc)")
@@ -116,7 +118,8 @@ def load(path: str = LIB_PATH):
         fn.argtypes = args
     if lib.dfcsr_abi_version() != 1:
         raise DfcsrError(f"ABI version mismatch: {lib.dfcsr_abi_version()}")
-    _lib = lib
+    if _lib is None:
+        _lib = lib
     return lib
 
 

@@ -31,7 +31,7 @@ CSR_SCALING_PER_COULOMB = 8.98755e3   # MeV/m per (C/m^2), CSR.py:80
 
 class CSR2D:
     def __init__(self, input_file=None, parallel=False, device=None, precision="fp64",
-                 config=None, verbose=False):
+                 config=None, verbose=False, keep_fp32=None):
         """input_file: YAML path (or a parsed dict via ``config``).  parallel is
         accepted for interface parity: the reference's MPI split over mesh
         points is the kernel's launch grid here."""
@@ -39,6 +39,7 @@ class CSR2D:
             device = "cuda"
         self.device = torch.device(device)
         self.precision = precision
+        self.keep_fp32 = (precision == "mixed") if keep_fp32 is None else bool(keep_fp32)
         self.verbose = verbose
         self.timestamp = time.strftime("%Y-%m-%dT%H_%M_%S")
         self.parallel = False
@@ -63,7 +64,7 @@ class CSR2D:
         self.beam = Beam(config["input_beam"], device=self.device, base_dir=base)
         self.lattice = Lattice(config["input_lattice"], device=self.device, base_dir=base)
         self.DF_tracker = DF_tracker(config.get("particle_deposition") or {}, device=self.device,
-                                     keep_fp32=(self.precision == "mixed"))
+                                     keep_fp32=self.keep_fp32)
         self.integration_params = Integration_params(config.get("CSR_integration") or {})
         self.CSR_params = CSR_params(config.get("CSR_computation") or {})
 

@@ -143,6 +143,106 @@ __global__ __launch_bounds__(kThreads) void cic_kernel(
 
 // total = hi * 2^32 + lo, normalised so the low part is < 2^32, then one
 // rounding to float64 and an exact power-of-two rescale.
+// LDS-privatised form of cic_kernel for grids whose row tiles fit in LDS.
+// Workgroup (chunk c, tile y) takes particles [c*per, (c+1)*per) and the
+// grid rows [y*rows, y*rows + rows): it accumulates the same two-word
+// fixed-point words in LDS with 64-bit LDS atomics and adds its non-zero
+// cells to the global words once.  Integer sums: the grid is identical to
+// the direct form's, for every launch order.
+constexpr int kTileLdsBytes = 48 * 1024;
+
+__device__ inline void lds_fx_add(unsigned long long *hi, unsigned long long *lo, int cell,
+                                  double v, double scale) {
+  const double s = v * scale;
+  const double h = floor(s * 0x1p-32);
+  const double l = s - h * 0x1p32;
+  const long long hv = (long long)h;
+  const unsigned long long lv = (unsigned long long)rint(l);
+  if (hv != 0) atomicAdd(hi + cell, (unsigned long long)hv);
+  if (lv != 0) atomicAdd(lo + cell, lv);
+}
+
+__global__ __launch_bounds__(kThreads) void cic_tile_kernel(
+    const double *__restrict__ q1, const double *__restrict__ q2,
+    const double *__restrict__ wA, const double *__restrict__ wB, int64_t n, int64_t per,
+    int nb1, double start1, double inv1, int nb2, double start2, double inv2, int rows,
+    const double *__restrict__ scale, int sA, int64_t *__restrict__ accA,
+    int64_t *__restrict__ accB, int32_t *__restrict__ cell_index,
+    int32_t *__restrict__ cell_count) {
+#pragma clang fp contract(off)
+  extern __shared__ __align__(16) unsigned long long lds[];
+  const int r0 = blockIdx.y * rows;
+  const int r1 = min(r0 + rows, nb1);
+  const int tcells = (r1 - r0) * nb2;
+  const int ngrid = wB ? 2 : 1;
+  unsigned long long *hA = lds, *lA = lds + tcells;
+  unsigned long long *hB = lds + 2 * tcells, *lB = lds + 3 * tcells;
+  int *cnt = reinterpret_cast<int *>(lds + 2 * ngrid * tcells);
+  for (int e = threadIdx.x; e < 2 * ngrid * tcells; e += kThreads) lds[e] = 0ull;
+  if (cell_count)
+    for (int e = threadIdx.x; e < tcells; e += kThreads) cnt[e] = 0;
+  __syncthreads();
+  const double scA = scale[sA], scB = scale[1];
+  const int64_t p0 = (int64_t)blockIdx.x * per;
+  const int64_t p1 = min(p0 + per, n);
+  for (int64_t p = p0 + threadIdx.x; p < p1; p += kThreads) {
+    double c1 = (q1[p] - start1) * inv1;
+    double f1 = floor(c1);
+    // rows touched: f1 and f1 + 1; skip particles that miss this tile
+    const bool owner_low = (f1 >= (double)r0 && f1 < (double)r1) ||
+                           (blockIdx.y == 0 && !(f1 >= 0.0)) ||
+                           (r1 == nb1 && f1 >= (double)nb1);
+    const bool touches = f1 >= (double)(r0 - 1) && f1 < (double)r1;
+    if (!touches && !(owner_low && cell_index)) continue;
+    double c2 = (q2[p] - start2) * inv2;
+    double f2 = floor(c2);
+    double S1 = 1.0 - (c1 - f1);
+    double S2 = 1.0 - (c2 - f2);
+    int i1 = (f1 > -4.0 && f1 < (double)nb1 + 4.0) ? (int)f1 : -8;
+    int i2 = (f2 > -4.0 && f2 < (double)nb2 + 4.0) ? (int)f2 : -8;
+    if (cell_index && owner_low) {
+      double g1 = fmin(fmax(f1, -2147483648.0), 2147483647.0);
+      double g2 = fmin(fmax(f2, -2147483648.0), 2147483647.0);
+      cell_index[2 * p] = isnan(f1) ? INT32_MIN : (int32_t)g1;
+      cell_index[2 * p + 1] = isnan(f2) ? INT32_MIN : (int32_t)g2;
+    }
+    if (!touches) continue;
+    const bool a0 = (i1 >= r0) && (i1 < r1), a1 = (i1 + 1 >= r0) && (i1 + 1 < r1);
+    const bool b0 = (i2 >= 0) && (i2 < nb2), b1 = (i2 + 1 >= 0) && (i2 + 1 < nb2);
+    const int lrow = i1 - r0;   // may be -1 when only row i1+1 is in the tile
+    if (cell_count && a0 && b0) atomicAdd(cnt + lrow * nb2 + i2, 1);
+    const double wa = wA ? wA[p] : 1.0;
+    const double t00 = wa * S1 * S2, t01 = wa * S1 * (1.0 - S2);
+    const double t10 = wa * (1.0 - S1) * S2, t11 = wa * (1.0 - S1) * (1.0 - S2);
+    const int base = lrow * nb2 + i2;
+    if (a0 && b0) lds_fx_add(hA, lA, base, t00, scA);
+    if (a0 && b1) lds_fx_add(hA, lA, base + 1, t01, scA);
+    if (a1 && b0) lds_fx_add(hA, lA, base + nb2, t10, scA);
+    if (a1 && b1) lds_fx_add(hA, lA, base + nb2 + 1, t11, scA);
+    if (wB) {
+      const double wb = wB[p];
+      const double u00 = wb * S1 * S2, u01 = wb * S1 * (1.0 - S2);
+      const double u10 = wb * (1.0 - S1) * S2, u11 = wb * (1.0 - S1) * (1.0 - S2);
+      if (a0 && b0) lds_fx_add(hB, lB, base, u00, scB);
+      if (a0 && b1) lds_fx_add(hB, lB, base + 1, u01, scB);
+      if (a1 && b0) lds_fx_add(hB, lB, base + nb2, u10, scB);
+      if (a1 && b1) lds_fx_add(hB, lB, base + nb2 + 1, u11, scB);
+    }
+  }
+  __syncthreads();
+  const int64_t nc = (int64_t)nb1 * nb2;
+  const int64_t g0 = (int64_t)r0 * nb2;
+  for (int e = threadIdx.x; e < tcells; e += kThreads) {
+    if (hA[e]) atomicAdd(reinterpret_cast<unsigned long long *>(accA + g0 + e), hA[e]);
+    if (lA[e]) atomicAdd(reinterpret_cast<unsigned long long *>(accA + nc + g0 + e), lA[e]);
+    if (wB) {
+      if (hB[e]) atomicAdd(reinterpret_cast<unsigned long long *>(accB + g0 + e), hB[e]);
+      if (lB[e]) atomicAdd(reinterpret_cast<unsigned long long *>(accB + nc + g0 + e), lB[e]);
+    }
+    if (cell_count && cnt[e]) atomicAdd(cell_count + g0 + e, cnt[e]);
+  }
+}
+
 __global__ __launch_bounds__(kThreads) void fx_to_double(const int64_t *__restrict__ acc,
                                                          const double *__restrict__ scale,
                                                          int which, int64_t n,
@@ -197,9 +297,27 @@ int deposit_impl(const char *who, const double *q1, const double *q2, const doub
                      cell_count, cell_count ? ncell : 0);
   // A lone weighted deposit (wA) uses the weighted scale[1] for grid A.
   int sA = wA ? 1 : 0;
-  hipLaunchKernelGGL(cic_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s, q1, q2, wA, wB, n, nb1,
-                     s1, inv1, nb2, s2, inv2, (const double *)hdr, sA, accA, accB, cell_index,
-                     cell_count);
+  // LDS tiles: 2 words per cell per grid (+ an int count) must fit kTileLdsBytes
+  const int ngrid = wB ? 2 : 1;
+  const int per_cell = 16 * ngrid + (cell_count ? 4 : 0);
+  const int rows = (int)(kTileLdsBytes / per_cell) / nb2;
+  const int ntile = rows > 0 ? (nb1 + rows - 1) / rows : 0;
+  if (rows >= 4 && ntile <= 16 && n > 0) {
+    // enough particles per workgroup that the flush stays small
+    int64_t chunks = (n + 16383) / 16384;
+    if (chunks < 1) chunks = 1;
+    if (chunks * ntile < 512) chunks = (512 + ntile - 1) / ntile;
+    int64_t per = (n + chunks - 1) / chunks;
+    chunks = (n + per - 1) / per;
+    const size_t lds = (size_t)rows * nb2 * per_cell + 16;
+    hipLaunchKernelGGL(cic_tile_kernel, dim3((unsigned)chunks, ntile), dim3(kThreads), lds, s, q1,
+                       q2, wA, wB, n, per, nb1, s1, inv1, nb2, s2, inv2, rows,
+                       (const double *)hdr, sA, accA, accB, cell_index, cell_count);
+  } else {
+    hipLaunchKernelGGL(cic_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s, q1, q2, wA, wB, n,
+                       nb1, s1, inv1, nb2, s2, inv2, (const double *)hdr, sA, accA, accB,
+                       cell_index, cell_count);
+  }
   hipLaunchKernelGGL(fx_to_double, dim3(grid_for(ncell)), dim3(kThreads), 0, s,
                      (const int64_t *)accA, (const double *)hdr, sA, ncell, outA);
   if (outB)   // zero-filled when there are no particles

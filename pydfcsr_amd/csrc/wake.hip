@@ -26,7 +26,17 @@ int check_lattice(const DfcsrLattice *L, const char *who);
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kChunk = 32;   // x' rows per work item
+#ifndef DFCSR_WAKE_CHUNK
+#define DFCSR_WAKE_CHUNK 64
+#endif
+#ifndef DFCSR_WAKE_MINBLOCKS
+#define DFCSR_WAKE_MINBLOCKS 2
+#endif
+#ifndef DFCSR_WAKE_ORDER
+#define DFCSR_WAKE_ORDER 0
+#endif
+
+constexpr int kChunk = DFCSR_WAKE_CHUNK;   // x' rows per work item
 constexpr int kNumXcd = 8;
 
 struct LinAxis {
@@ -197,17 +207,217 @@ __device__ void wake_point_fp64(const DfcsrHistory &H, const DfcsrLattice &L,
   out_x = block_sum(acc_x, red);
 }
 
-__global__ __launch_bounds__(kThreads) void wake_fp64_kernel(DfcsrHistory H, DfcsrLattice L,
+// ---- mixed precision --------------------------------------------------------
+//
+// Same integral, regrouped so that FP32 only sees well-conditioned terms.
+// With v = tau + vx n (observer), v' = tau' + vx' n' and grad = dx n' + g tau'
+// (g = dz/scale), the reference's numerators expand exactly into per-column
+// coefficients (A = v.tau', B = v.n', e = tau'.n', a = tau'.tau', b = n'.n'):
+//   v.grad - (v.v') v'.grad = dx (D0 + vx'(D1 + vx' D2)) + g (G0 + vx'(G1 + vx' G2))
+//     D0 = B - A e, D1 = -(A b + B e), D2 = -B b,
+//     G0 = A (1 - a), G1 = -(A e + B a), G2 = -B e,
+//   v.v' = A + vx' B,   v'.grad = dx (e + vx' b) + g (a + vx' e).
+// Those coefficients, |r - r'|, t_ret, z' = s' - t_ret, the history indices
+// and q = (r-r').(n-n')/R - n.tau' (whose two terms cancel to leading order)
+// are float64; the gathered fields, the coefficients' products and the
+// integrand are float32.  1/R comes from an FP32 reciprocal square root with
+// one FP64 Newton step.  Each 32-row chunk is summed in FP32 with Neumaier's
+// compensated summation (Neumaier 1974, the Kahan-Babuska improvement); chunk
+// sums are weighted and accumulated in float64, and the block reduction is
+// float64.
+struct ColumnMixed {
+  double sp, ax, ay, npx, npy, P0, P1, p2;
+  float rho, D0, D1, D2, G0, G1, G2, e, a, b, A, B;
+};
+
+__device__ inline ColumnMixed make_column_mixed(const DfcsrLattice &L, const Observer &o,
+                                                double sp, double inv_ds) {
+  const Column c = make_column<true>(L, o, sp, inv_ds);
+  ColumnMixed m;
+  m.sp = sp;
+  m.ax = c.ax;
+  m.ay = c.ay;
+  m.npx = c.L.nx;
+  m.npy = c.L.ny;
+  const double vx_ = o.L.tx + o.vx * o.L.nx, vy_ = o.L.ty + o.vx * o.L.ny;
+  const double A = vx_ * c.L.tx + vy_ * c.L.ty, B = vx_ * c.L.nx + vy_ * c.L.ny;
+  const double e = c.L.tx * c.L.nx + c.L.ty * c.L.ny;
+  const double a = c.L.tx * c.L.tx + c.L.ty * c.L.ty;
+  const double b = c.L.nx * c.L.nx + c.L.ny * c.L.ny;
+  const double nmx = o.L.nx - c.L.nx, nmy = o.L.ny - c.L.ny;
+  m.P0 = c.ax * nmx + c.ay * nmy;
+  m.P1 = c.L.nx * nmx + c.L.ny * nmy;
+  m.p2 = o.L.nx * c.L.tx + o.L.ny * c.L.ty;
+  m.rho = (float)c.rho;
+  m.D0 = (float)(B - A * e);
+  m.D1 = (float)(-(A * b + B * e));
+  m.D2 = (float)(-B * b);
+  m.G0 = (float)(A * (1.0 - a));
+  m.G1 = (float)(-(A * e + B * a));
+  m.G2 = (float)(-B * e);
+  m.e = (float)e;
+  m.a = (float)a;
+  m.b = (float)b;
+  m.A = (float)A;
+  m.B = (float)B;
+  return m;
+}
+
+__device__ inline const float *hist_rec32(const DfcsrHistory &H, int ti, int xi, int zi) {
+  int slot = H.head + ti;
+  if (slot >= H.cap) slot -= H.cap;
+  return H.data32 + (((int64_t)slot * H.Xi + xi) * H.Zi + zi) * DFCSR_HIST_REC32;
+}
+
+struct Neumaier {
+  float s = 0.f, c = 0.f;
+  __device__ void add(float x) {
+    float t = s + x;
+    c += (fabsf(s) >= fabsf(x)) ? ((s - t) + x) : ((x - t) + s);
+    s = t;
+  }
+  __device__ float value() const { return s + c; }
+};
+
+// One source point in mixed precision; false when outside the history.
+__device__ inline bool csr_point_mixed(const DfcsrHistory &H, const HistAxes &A, double t,
+                                       const ColumnMixed &c, double xp, int xi0, int xi1,
+                                       float xd, float &Iz, float &Ix) {
+  const double rx = c.ax - xp * c.npx;
+  const double ry = c.ay - xp * c.npy;
+  const double R2 = rx * rx + ry * ry;
+  double y = (double)rsqrtf((float)R2);
+  y = y * (1.5 - 0.5 * R2 * y * y);               // one Newton step: ~1e-14 relative
+  const double R = R2 * y;
+  const double t_ret = t - R;
+  const double ut = (t_ret - A.min_t) * A.it;
+  const double uz = ((c.sp - t_ret) - A.min_z) * A.iz;
+  const TruncIdx T = trunc_index(ut, H.nt), Z = trunc_index(uz, H.Zi);
+  if (!(T.ok && Z.ok)) return false;
+  const float td = (float)T.frac, zd = (float)Z.frac;
+  const float *p00 = hist_rec32(H, T.i0, xi0, Z.i0), *p10 = hist_rec32(H, T.i1, xi0, Z.i0);
+  const float *p01 = hist_rec32(H, T.i0, xi1, Z.i0), *p11 = hist_rec32(H, T.i1, xi1, Z.i0);
+  const int dz1 = (Z.i1 - Z.i0) * DFCSR_HIST_REC32;
+  float f[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    const float c00 = fmaf(td, p10[k] - p00[k], p00[k]);
+    const float c01 = fmaf(td, p10[k + dz1] - p00[k + dz1], p00[k + dz1]);
+    const float c10 = fmaf(td, p11[k] - p01[k], p01[k]);
+    const float c11 = fmaf(td, p11[k + dz1] - p01[k + dz1], p01[k + dz1]);
+    const float c0 = fmaf(xd, c10 - c00, c00);
+    const float c1 = fmaf(xd, c11 - c01, c01);
+    f[k] = fmaf(zd, c1 - c0, c0);
+  }
+  const float dens = f[DFCSR_F_DENSITY], dx = f[DFCSR_F_DENSITY_X], dz = f[DFCSR_F_DENSITY_Z];
+  const float vr = f[DFCSR_F_VX], dv = f[DFCSR_F_VX_X];
+  const float scale = fmaf((float)xp, c.rho, 1.0f);
+  const float g = dz / scale;
+  const float Cd = fmaf(vr, fmaf(vr, c.D2, c.D1), c.D0);
+  const float Cg = fmaf(vr, fmaf(vr, c.G2, c.G1), c.G0);
+  const float part1 = fmaf(vr, c.B, c.A);
+  const float vgrad = fmaf(dx, fmaf(vr, c.b, c.e), g * fmaf(vr, c.e, c.a));
+  const float ddv = dens * dv;
+  const float pd = -vgrad - ddv;
+  const double p1 = c.P0 - xp * c.P1;
+  const double p1R = p1 * y;
+  const float q = (float)(p1R - c.p2);
+  const float ir = (float)y;
+  const float sir = scale * ir;
+  Iz = sir * (fmaf(dx, Cd, g * Cg) - part1 * ddv);
+  Ix = sir * fmaf((float)p1R * ir, dens, pd * q);
+  return true;
+}
+
+__device__ void wake_point_mixed(const DfcsrHistory &H, const DfcsrLattice &L,
+                                 const DfcsrWakeParams &P, int i, double *red, RegionLds &R,
+                                 double &out_z, double &out_x, double &xm_out, double &zm_out) {
+  double xm, zm;
+  mesh_point(P, i, xm, zm);
+  xm_out = xm;
+  zm_out = zm;
+  const double t = P.t;
+  const double s = t + zm, x = xm;
+  const HistAxes A = make_axes(H);
+  const double inv_ds = 1.0 / L.ds;
+  const Observer o = make_observer<true>(H, A, L, s, x, t);
+  if (threadIdx.x == 0) {
+    make_regions(P, s, x, t, R.rg);
+    R.off[0] = 0;
+    for (int r = 0; r < R.rg.n; ++r)
+      R.off[r + 1] = R.off[r] + ((R.rg.xa[r].n + kChunk - 1) / kChunk) * R.rg.sa[r].n;
+  }
+  __syncthreads();
+  const int nreg = R.rg.n;
+  const int total = R.off[nreg];
+  double acc_z = 0.0, acc_x = 0.0;
+  for (int it = threadIdx.x; it < total; it += blockDim.x) {
+    int r = 0;
+    while (r + 1 < nreg && it >= R.off[r + 1]) ++r;
+    const int rem = it - R.off[r];
+    const LinAxis sa = R.rg.sa[r];
+    const LinAxis xa = R.rg.xa[r];
+    const int c = rem / sa.n, k = rem % sa.n;
+    const double sp = sa.at(k);
+    const double wz = sa.w(k);
+    const ColumnMixed col = make_column_mixed(L, o, sp, inv_ds);
+    const int j0 = c * kChunk;
+    const int j1 = min(j0 + kChunk, xa.n);
+    Neumaier cz, cx;
+    for (int j = j0; j < j1; ++j) {
+      const double xp = xa.at(j);
+      const TruncIdx X = trunc_index((xp - A.min_x) * A.ix, H.Xi);
+      if (!X.ok) continue;   // wave-uniform: rows are shared
+      float Iz, Ix;
+      if (!csr_point_mixed(H, A, t, col, xp, X.i0, X.i1, (float)X.frac, Iz, Ix)) continue;
+      const float wx = (float)xa.w(j);
+      cz.add(wx * Iz);
+      cx.add(wx * Ix);
+    }
+    acc_z += wz * (double)cz.value();
+    acc_x += wz * (double)cx.value();
+  }
+  out_z = block_sum(acc_z, red);
+  out_x = block_sum(acc_x, red);
+}
+
+// Mesh point of block b.  Blocks of one XCD integrate a contiguous range of
+// mesh points (row-major: neighbouring z at one x, whose retarded lookups
+// overlap), so they share that XCD's L2.  DFCSR_WAKE_ORDER=1 instead runs
+// over x first (measured slower on the chicane workload).
+__device__ inline int mesh_of_block(const DfcsrWakeParams &P, int b, int nb) {
+  const int l = xcd_remap(b, nb);
+  if (DFCSR_WAKE_ORDER == 0 || P.observers) return l;
+  return (l % P.mesh_xbins) * P.mesh_zbins + l / P.mesh_xbins;
+}
+
+__global__ __launch_bounds__(kThreads, DFCSR_WAKE_MINBLOCKS) void wake_fp64_kernel(DfcsrHistory H, DfcsrLattice L,
                                                              DfcsrWakeParams P,
                                                              double *__restrict__ xmesh,
                                                              double *__restrict__ zmesh,
                                                              double *__restrict__ dE,
                                                              double *__restrict__ xk) {
   __shared__ double red[kThreads / 64];
+  const int i = mesh_of_block(P, blockIdx.x, gridDim.x);
   __shared__ RegionLds R;
-  const int i = xcd_remap(blockIdx.x, gridDim.x);
   double sz, sx, xm, zm;
   wake_point_fp64(H, L, P, i, red, R, sz, sx, xm, zm);
+  if (threadIdx.x == 0) {
+    dE[i] = -P.csr_scaling * sz;
+    xk[i] = P.csr_scaling * sx;
+    if (xmesh) xmesh[i] = xm;
+    if (zmesh) zmesh[i] = zm;
+  }
+}
+
+__global__ __launch_bounds__(kThreads, DFCSR_WAKE_MINBLOCKS) void wake_mixed_kernel(
+    DfcsrHistory H, DfcsrLattice L, DfcsrWakeParams P, double *__restrict__ xmesh,
+    double *__restrict__ zmesh, double *__restrict__ dE, double *__restrict__ xk) {
+  __shared__ double red[kThreads / 64];
+  const int i = mesh_of_block(P, blockIdx.x, gridDim.x);
+  __shared__ RegionLds R;
+  double sz, sx, xm, zm;
+  wake_point_mixed(H, L, P, i, red, R, sz, sx, xm, zm);
   if (threadIdx.x == 0) {
     dE[i] = -P.csr_scaling * sz;
     xk[i] = P.csr_scaling * sx;
@@ -225,10 +435,10 @@ struct WakeTask {
   int M;
 };
 
-__global__ __launch_bounds__(kThreads) void wake_fp64_batched_kernel(
+template <bool kMixed>
+__global__ __launch_bounds__(kThreads, DFCSR_WAKE_MINBLOCKS) void wake_batched_kernel(
     const WakeTask *__restrict__ tasks, int ntask) {
   __shared__ double red[kThreads / 64];
-  __shared__ RegionLds R;
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   int lo = 0, hi = ntask - 1;
   while (lo < hi) {   // last task with block0 <= b
@@ -238,8 +448,10 @@ __global__ __launch_bounds__(kThreads) void wake_fp64_batched_kernel(
   }
   const WakeTask &T = tasks[lo];
   const int i = b - T.block0;
+  __shared__ RegionLds R;
   double sz, sx, xm, zm;
-  wake_point_fp64(T.H, T.L, T.P, i, red, R, sz, sx, xm, zm);
+  if (kMixed) wake_point_mixed(T.H, T.L, T.P, i, red, R, sz, sx, xm, zm);
+  else wake_point_fp64(T.H, T.L, T.P, i, red, R, sz, sx, xm, zm);
   if (threadIdx.x == 0) {
     T.dE[i] = -T.P.csr_scaling * sz;
     T.xk[i] = T.P.csr_scaling * sx;
@@ -253,6 +465,7 @@ int check_params(const DfcsrWakeParams *p, const char *who) {
                      p->mesh_xbins, p->mesh_zbins);
   if ((int64_t)p->mesh_xbins * p->mesh_zbins > (int64_t)1 << 24)
     return set_error(DFCSR_ERANGE, "%s: mesh too large", who);
+
   return DFCSR_OK;
 }
 
@@ -281,6 +494,11 @@ extern "C" int dfcsr_wake_mesh(const DfcsrHistory *hist, const DfcsrLattice *lat
   if (precision == DFCSR_PREC_FP64) {
     hipLaunchKernelGGL(wake_fp64_kernel, dim3(M), dim3(kThreads), 0, as_stream(stream), *hist,
                        *lat, *p, xmesh, zmesh, dE_dct, x_kick);
+  } else if (precision == DFCSR_PREC_MIXED) {
+    if (!hist->data32)
+      return set_error(DFCSR_EINVAL, "wake_mesh: mixed precision needs the FP32 history");
+    hipLaunchKernelGGL(wake_mixed_kernel, dim3(M), dim3(kThreads), 0, as_stream(stream), *hist,
+                       *lat, *p, xmesh, zmesh, dE_dct, x_kick);
   } else {
     return set_error(DFCSR_EINVAL, "wake_mesh: unknown precision %d", precision);
   }
@@ -302,7 +520,7 @@ extern "C" int dfcsr_wake_mesh_batched(int k, const DfcsrHistory *hist, const Df
     return set_error(DFCSR_EINVAL, "wake_mesh_batched: NULL pointer");
   if (ws_bytes < dfcsr_wake_batched_workspace_bytes(k))
     return set_error(DFCSR_ENOWS, "wake_mesh_batched: workspace too small");
-  if (precision != DFCSR_PREC_FP64)
+  if (precision != DFCSR_PREC_FP64 && precision != DFCSR_PREC_MIXED)
     return set_error(DFCSR_EINVAL, "wake_mesh_batched: unknown precision %d", precision);
   WakeTask tasks_h[256];
   if (k > 256) return set_error(DFCSR_ERANGE, "wake_mesh_batched: %d tasks > 256", k);
@@ -313,6 +531,8 @@ extern "C" int dfcsr_wake_mesh_batched(int k, const DfcsrHistory *hist, const Df
     if ((rc = check_lattice(&lat[j], "wake_mesh_batched"))) return rc;
     if ((rc = check_params(&p[j], "wake_mesh_batched"))) return rc;
     if (!dE_dct[j] || !x_kick[j]) return set_error(DFCSR_EINVAL, "wake_mesh_batched: NULL output");
+    if (precision == DFCSR_PREC_MIXED && !hist[j].data32)
+      return set_error(DFCSR_EINVAL, "wake_mesh_batched: task %d lacks the FP32 history", j);
     tasks_h[j].H = hist[j];
     tasks_h[j].L = lat[j];
     tasks_h[j].P = p[j];
@@ -325,8 +545,12 @@ extern "C" int dfcsr_wake_mesh_batched(int k, const DfcsrHistory *hist, const Df
   hipStream_t s = as_stream(stream);
   WakeTask *dt = static_cast<WakeTask *>(ws);
   DFCSR_CHECK_HIP(hipMemcpyAsync(dt, tasks_h, (size_t)k * sizeof(WakeTask), hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(wake_fp64_batched_kernel, dim3(blocks), dim3(kThreads), 0, s,
-                     (const WakeTask *)dt, k);
+  if (precision == DFCSR_PREC_MIXED)
+    hipLaunchKernelGGL(wake_batched_kernel<true>, dim3(blocks), dim3(kThreads), 0, s,
+                       (const WakeTask *)dt, k);
+  else
+    hipLaunchKernelGGL(wake_batched_kernel<false>, dim3(blocks), dim3(kThreads), 0, s,
+                       (const WakeTask *)dt, k);
   DFCSR_LAUNCH_CHECK("wake_mesh_batched");
   return DFCSR_OK;
 }

@@ -253,3 +253,24 @@ def test_apply_wakes_golden(K, ws, golden_dir):
     dpx_ref = a["px1"] - a["px0"]
     assert rel_max(pz.cpu().numpy() - a["pz0"], dpz_ref) <= 1e-9
     assert rel_max(px.cpu().numpy() - a["px0"], dpx_ref) <= 1e-9
+
+
+def test_wake_mixed_precision_golden(K, ws, golden_dir, golden_history):
+    """Mixed form (FP32 integrand, FP64 retardation and sums) against the
+    reference's float64 wakes: max |err| <= 1e-4 * max|W| for both wakes."""
+    w, beam, DH, DL, P = _golden_wake_inputs(golden_dir, golden_history)
+    out = K.wake_mesh(DH.desc, DL.desc, P, 1, DEV, ws)
+    e_z = rel_max(out["dE_dct"].cpu().view(3, 4), w["dE_dct"])
+    e_x = rel_max(out["x_kick"].cpu().view(3, 4), w["x_kick"])
+    assert e_z <= 1e-4 and e_x <= 1e-4, (e_z, e_x)
+    out2 = K.wake_mesh(DH.desc, DL.desc, P, 1, DEV, ws)
+    assert torch.equal(out["dE_dct"], out2["dE_dct"])   # deterministic
+    for slope0, key in ((2.5, "chirp_pos"), (-2.5, "chirp_neg")):
+        b = dict(beam, slope=np.array([slope0, w["slope"][1]]))
+        rows = w["chirp_rows"]
+        obs = _t(np.stack([w["zmesh"][rows], w["xmesh"][rows]], axis=1))
+        Pc = wake_params(0.3, b, 0.05, 1.0, 12, 16, float(w["csr_scaling"]), (0.0, 0.0, 1),
+                         (0.0, 0.0, len(rows)), observers=obs)
+        o = K.wake_mesh(DH.desc, DL.desc, Pc, 1, DEV, ws)
+        assert rel_max(o["dE_dct"].cpu().numpy(), w[key][:, 0]) <= 1e-4
+        assert rel_max(o["x_kick"].cpu().numpy(), w[key][:, 1]) <= 1e-4

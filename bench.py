@@ -102,6 +102,18 @@ def cpu_baseline(sim, gpu_out, budget_s=15.0, max_points=None):
             "max_rel_diff_vs_gpu_dE": err}
 
 
+def profiled_traffic():
+    """HBM bytes per wake launch from the committed rocprofv3 counter runs
+    (FETCH_SIZE and WRITE_SIZE, each divided by its copy-kernel factor)."""
+    path = os.path.join(ROOT, "profiles", "wake_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return int(d["fetch_kib_raw"] * 1024 / d["fetch_factor"] +
+               d["write_kib_raw"] * 1024 / d["write_factor"])
+
+
 def compare_precisions(sim, reps=3):
     """FP64 and mixed wake on the simulation's current state (same history,
     same parameters): best-of-reps kernel time and the mixed form's error."""
@@ -146,6 +158,8 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only copy BYTES device-to-device 3 times (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU mesh "
@@ -155,6 +169,15 @@ def main():
     import numpy as np
     import torch
     from pydfcsr_amd.csr import CSR2D
+
+    if args.copy_calibration:
+        src = torch.ones(args.copy_calibration // 8, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": src.numel() * 8}))
+        return
 
     dev = torch.device("cuda:0")
     cfg = chicane_config()
@@ -211,7 +234,7 @@ def main():
                     "bytes": int(tr.footprint_bytes()), "regrids": tr.n_regrid},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-                     "traffic": None,
+                     "traffic": profiled_traffic(),
                      "kernel": "wake_fp64_kernel", "flops_per_point": FLOPS_PER_POINT},
         "init_s": round(t0 - t_init, 2),
     }

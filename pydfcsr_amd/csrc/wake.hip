@@ -388,7 +388,15 @@ __device__ void wake_point_mixed(const DfcsrHistory &H, const DfcsrLattice &L,
 __device__ inline int mesh_of_block(const DfcsrWakeParams &P, int b, int nb) {
   const int l = xcd_remap(b, nb);
   if (DFCSR_WAKE_ORDER == 0 || P.observers) return l;
-  return (l % P.mesh_xbins) * P.mesh_zbins + l / P.mesh_xbins;
+  const int ix = l % P.mesh_xbins, zt = l / P.mesh_xbins;
+  int iz = zt;
+  if (DFCSR_WAKE_ORDER == 2 && P.mesh_zbins % kNumXcd == 0) {
+    // XCD k owns z-lines k, k+8, ...: the same z at all x runs together on
+    // one XCD while every XCD gets observers from the whole bunch
+    const int q = P.mesh_zbins / kNumXcd;
+    iz = (zt % q) * kNumXcd + zt / q;
+  }
+  return ix * P.mesh_zbins + iz;
 }
 
 __global__ __launch_bounds__(kThreads, DFCSR_WAKE_MINBLOCKS) void wake_fp64_kernel(DfcsrHistory H, DfcsrLattice L,

@@ -149,12 +149,57 @@ def compare_precisions(sim, reps=3):
     return res, outs
 
 
+def serve_main(args, dev):
+    """Serving mix: seeded requests (particles 1e5-1e7 log-uniform, deposition
+    64/128/256, wake mesh 32/64/128, single dipole rho 5-20 m or chicane at
+    0.02-0.08 rad, sigma 20-100 um, chirp -20..0 /m), Poisson arrivals, mixed
+    precision, admission by HBM footprint; runs for --serve-seconds."""
+    from pydfcsr_amd.serving import Server, mix_requests
+    lo, hi = (int(v) for v in args.serve_steps.split(","))
+    reqs = mix_requests(n=args.serve_requests, rate=args.serve_rate, seed=0,
+                        steps_range=(lo, hi))
+    srv = Server(device=dev, precision="mixed", use_graph=True)
+    for r in reqs:
+        srv.submit(r)
+    t_stop = args.serve_seconds
+
+    def progress():
+        return (f"serving: t={time.perf_counter() - t0:.1f}s ticks={srv.ticks} steps={srv.steps} "
+                f"active={len(srv.active)} waiting={len(srv.waiting)} finished={len(srv.finished)}")
+
+    t0 = time.perf_counter()
+    last = t0
+    import numpy as np
+    start = srv.clock()
+    while (srv.waiting or srv.active) and time.perf_counter() - t0 < t_stop:
+        now = srv.clock() - start
+        if not srv.active and srv.waiting and srv.waiting[0].arrival > now:
+            time.sleep(min(0.05, srv.waiting[0].arrival - now))
+            continue
+        srv.tick(now)
+        if time.perf_counter() - last > 10:
+            print(progress(), file=sys.stderr, flush=True)
+            last = time.perf_counter()
+    rep = srv.report(time.perf_counter() - t0)
+    res = {"metric": METRIC, "value": rep["steps_per_s"], "unit": "sim-steps/s", "n_gpus": 1,
+           "higher_is_better": True, "dtype": "f32+f64", "data": "synthetic request mix",
+           "config": {"workload": f"serving mix: {args.serve_requests} requests, Poisson "
+                                  f"{args.serve_rate}/s, steps {lo}-{hi}, mixed precision, "
+                                  f"{args.serve_seconds:.0f} s wall budget"},
+           "serving": rep}
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--workload", default="chicane", choices=["chicane"])
+    ap.add_argument("--workload", default="chicane", choices=["chicane", "serving"])
+    ap.add_argument("--serve-requests", type=int, default=64)
+    ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrivals per second")
+    ap.add_argument("--serve-seconds", type=float, default=90.0)
+    ap.add_argument("--serve-steps", default="100,800", help="history length range (steps)")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -180,6 +225,9 @@ def main():
         return
 
     dev = torch.device("cuda:0")
+    if args.workload == "serving":
+        serve_main(args, dev)
+        return
     cfg = chicane_config()
     t_init = time.perf_counter()
     sim = CSR2D(config=cfg, device=dev, precision=args.precision, keep_fp32=True)

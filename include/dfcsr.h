@@ -196,6 +196,17 @@ int dfcsr_wake_mesh_batched(int k, const DfcsrHistory *hist, const DfcsrLattice 
                             const DfcsrWakeParams *p, int precision, double *const *dE_dct,
                             double *const *x_kick, void *ws, int64_t ws_bytes, void *stream);
 int64_t dfcsr_wake_batched_workspace_bytes(int k);
+/* The batched form in two halves, so the launch can be captured in a HIP
+ * graph and replayed with new task contents: dfcsr_wake_tasks_pack writes
+ * the k task descriptors into host memory `tasks` (size
+ * dfcsr_wake_batched_workspace_bytes(k)) and the workgroup count into
+ * *blocks; the caller copies those bytes to device memory and
+ * dfcsr_wake_batched_launch runs them (no host-device copy inside). */
+int dfcsr_wake_tasks_pack(int k, const DfcsrHistory *hist, const DfcsrLattice *lat,
+                          const DfcsrWakeParams *p, int precision, double *const *dE_dct,
+                          double *const *x_kick, void *tasks, int64_t tasks_bytes, int *blocks);
+int dfcsr_wake_batched_launch(const void *dev_tasks, int k, int blocks, int precision,
+                              void *stream);
 
 /* ---- a16: mesh -> particle kick --------------------------------------------
  * Replaces Beam.apply_wakes (beams.py:108-131): bilinear

@@ -89,6 +89,11 @@ _SIGNATURES = [
                                         ctypes.POINTER(WakeParams), c_i32,
                                         ctypes.POINTER(c_dp), ctypes.POINTER(c_dp), c_dp, c_i64,
                                         c_dp]),
+    ("dfcsr_wake_tasks_pack", c_i32, [c_i32, ctypes.POINTER(History), ctypes.POINTER(Lattice),
+                                      ctypes.POINTER(WakeParams), c_i32, ctypes.POINTER(c_dp),
+                                      ctypes.POINTER(c_dp), c_dp, c_i64,
+                                      ctypes.POINTER(ctypes.c_int)]),
+    ("dfcsr_wake_batched_launch", c_i32, [c_dp, c_i32, c_i32, c_i32, c_dp]),
     ("dfcsr_apply_wakes", c_i32, [c_dp, c_dp, c_dp, c_dp, c_i64, c_dp, c_dp, c_i32, c_i32, c_f64,
                                   c_f64, c_f64, c_f64, c_f64, c_f64, c_f64, c_f64, c_i32, c_dp]),
 ]

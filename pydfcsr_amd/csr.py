@@ -31,7 +31,7 @@ CSR_SCALING_PER_COULOMB = 8.98755e3   # MeV/m per (C/m^2), CSR.py:80
 
 class CSR2D:
     def __init__(self, input_file=None, parallel=False, device=None, precision="fp64",
-                 config=None, verbose=False, keep_fp32=None):
+                 config=None, verbose=False, keep_fp32=None, history_capacity=16):
         """input_file: YAML path (or a parsed dict via ``config``).  parallel is
         accepted for interface parity: the reference's MPI split over mesh
         points is the kernel's launch grid here."""
@@ -40,6 +40,7 @@ class CSR2D:
         self.device = torch.device(device)
         self.precision = precision
         self.keep_fp32 = (precision == "mixed") if keep_fp32 is None else bool(keep_fp32)
+        self.history_capacity = int(history_capacity)
         self.verbose = verbose
         self.timestamp = time.strftime("%Y-%m-%dT%H_%M_%S")
         self.parallel = False
@@ -64,7 +65,8 @@ class CSR2D:
         self.beam = Beam(config["input_beam"], device=self.device, base_dir=base)
         self.lattice = Lattice(config["input_lattice"], device=self.device, base_dir=base)
         self.DF_tracker = DF_tracker(config.get("particle_deposition") or {}, device=self.device,
-                                     keep_fp32=self.keep_fp32)
+                                     keep_fp32=self.keep_fp32,
+                                     capacity=self.history_capacity)
         self.integration_params = Integration_params(config.get("CSR_integration") or {})
         self.CSR_params = CSR_params(config.get("CSR_computation") or {})
 
@@ -133,6 +135,15 @@ class CSR2D:
 
     def iter_steps(self, stop_time=None, debug=False):
         """Generator form of run(): yields the step count after each step."""
+        for kind, step in self.iter_phases(stop_time=stop_time, debug=debug, external_wake=False):
+            yield step
+
+    def iter_phases(self, stop_time=None, debug=False, external_wake=False):
+        """Step generator yielding ("done", step) after each step.  With
+        external_wake, a step that needs a wake first yields ("wake", step)
+        with the mesh prepared (prepare_wake()); the caller computes the wake
+        into self.wake_out (e.g. batched with other simulations) and resumes,
+        and the step then applies the kicks."""
         step_count = 1
         DL = self.lattice.step_size
         ele_count = 0
@@ -185,7 +196,11 @@ class CSR2D:
                 if self.CSR_params.compute_CSR and step % self.lattice.nsep[ele_count] == 0:
                     self.get_CSR_mesh()
                     ev_w0 = self._event() if self.timing else None
-                    self.calculate_2D_CSR()
+                    if external_wake:
+                        yield ("wake", step_count)
+                        self.finish_wake()
+                    else:
+                        self.calculate_2D_CSR()
                     ev_w1 = self._event() if self.timing else None
                     pts = self.integrand_points()
                     if self.CSR_params.apply_CSR:
@@ -199,7 +214,7 @@ class CSR2D:
                 if self.verbose:
                     print(f"Finish step {step_count}, s = {self.beam.position},  in "
                           f"{time.time() - t0} seconds", flush=True)
-                yield step_count
+                yield ("done", step_count)
                 step_count += 1
                 if stop_time and self.beam.position > stop_time:
                     return
@@ -263,6 +278,20 @@ class CSR2D:
         self.x_kick = self._wake_out["x_kick"].view(p.xbins, p.zbins)
         self.CSR_xmesh = self._wake_out["xmesh"]
         self.CSR_zmesh = self._wake_out["zmesh"]
+
+    def prepare_wake(self):
+        """(history, lattice, params, outputs) of the pending wake."""
+        M = self._mesh_M()
+        if getattr(self, "wake_out", None) is None or self.wake_out["dE_dct"].numel() != M:
+            self.wake_out = {k: torch.empty(M, dtype=torch.float64, device=self.device)
+                             for k in ("dE_dct", "x_kick")}
+        return (self.DF_tracker.descriptor(), self.lattice.device_tables(self.device),
+                self.wake_params(), self.wake_out)
+
+    def finish_wake(self):
+        p = self.CSR_params
+        self.dE_dct = self.wake_out["dE_dct"].view(p.xbins, p.zbins)
+        self.x_kick = self.wake_out["x_kick"].view(p.xbins, p.zbins)
 
     def _event(self):
         e = torch.cuda.Event(enable_timing=True)

@@ -519,46 +519,72 @@ extern "C" int64_t dfcsr_wake_batched_workspace_bytes(int k) {
   return (int64_t)k * (int64_t)sizeof(WakeTask) + 256;
 }
 
+extern "C" int dfcsr_wake_tasks_pack(int k, const DfcsrHistory *hist, const DfcsrLattice *lat,
+                                     const DfcsrWakeParams *p, int precision,
+                                     double *const *dE_dct, double *const *x_kick, void *tasks,
+                                     int64_t tasks_bytes, int *blocks) {
+  if (k < 1 || k > 4096) return set_error(DFCSR_ERANGE, "wake_tasks_pack: %d tasks", k);
+  if (!hist || !lat || !p || !dE_dct || !x_kick || !tasks || !blocks)
+    return set_error(DFCSR_EINVAL, "wake_tasks_pack: NULL pointer");
+  if (tasks_bytes < (int64_t)k * (int64_t)sizeof(WakeTask))
+    return set_error(DFCSR_ENOWS, "wake_tasks_pack: buffer too small");
+  if (precision != DFCSR_PREC_FP64 && precision != DFCSR_PREC_MIXED)
+    return set_error(DFCSR_EINVAL, "wake_tasks_pack: unknown precision %d", precision);
+  WakeTask *th = static_cast<WakeTask *>(tasks);
+  int64_t nb = 0;
+  for (int j = 0; j < k; ++j) {
+    int rc;
+    if ((rc = check_history(&hist[j], "wake_tasks_pack"))) return rc;
+    if ((rc = check_lattice(&lat[j], "wake_tasks_pack"))) return rc;
+    if ((rc = check_params(&p[j], "wake_tasks_pack"))) return rc;
+    if (!dE_dct[j] || !x_kick[j]) return set_error(DFCSR_EINVAL, "wake_tasks_pack: NULL output");
+    if (precision == DFCSR_PREC_MIXED && !hist[j].data32)
+      return set_error(DFCSR_EINVAL, "wake_tasks_pack: task %d lacks the FP32 history", j);
+    th[j].H = hist[j];
+    th[j].L = lat[j];
+    th[j].P = p[j];
+    th[j].dE = dE_dct[j];
+    th[j].xk = x_kick[j];
+    th[j].block0 = (int)nb;
+    th[j].M = p[j].mesh_xbins * p[j].mesh_zbins;
+    nb += th[j].M;
+    if (nb > ((int64_t)1 << 30)) return set_error(DFCSR_ERANGE, "wake_tasks_pack: too many blocks");
+  }
+  *blocks = (int)nb;
+  return DFCSR_OK;
+}
+
+extern "C" int dfcsr_wake_batched_launch(const void *dev_tasks, int k, int blocks, int precision,
+                                         void *stream) {
+  if (!dev_tasks || k < 1 || blocks < 1)
+    return set_error(DFCSR_EINVAL, "wake_batched_launch: bad arguments");
+  hipStream_t s = as_stream(stream);
+  const WakeTask *dt = static_cast<const WakeTask *>(dev_tasks);
+  if (precision == DFCSR_PREC_MIXED)
+    hipLaunchKernelGGL(wake_batched_kernel<true>, dim3(blocks), dim3(kThreads), 0, s, dt, k);
+  else if (precision == DFCSR_PREC_FP64)
+    hipLaunchKernelGGL(wake_batched_kernel<false>, dim3(blocks), dim3(kThreads), 0, s, dt, k);
+  else
+    return set_error(DFCSR_EINVAL, "wake_batched_launch: unknown precision %d", precision);
+  DFCSR_LAUNCH_CHECK("wake_batched_launch");
+  return DFCSR_OK;
+}
+
 extern "C" int dfcsr_wake_mesh_batched(int k, const DfcsrHistory *hist, const DfcsrLattice *lat,
                                        const DfcsrWakeParams *p, int precision,
                                        double *const *dE_dct, double *const *x_kick, void *ws,
                                        int64_t ws_bytes, void *stream) {
   if (k < 1) return DFCSR_OK;
-  if (!hist || !lat || !p || !dE_dct || !x_kick || !ws)
-    return set_error(DFCSR_EINVAL, "wake_mesh_batched: NULL pointer");
+  if (!ws) return set_error(DFCSR_EINVAL, "wake_mesh_batched: NULL workspace");
   if (ws_bytes < dfcsr_wake_batched_workspace_bytes(k))
     return set_error(DFCSR_ENOWS, "wake_mesh_batched: workspace too small");
-  if (precision != DFCSR_PREC_FP64 && precision != DFCSR_PREC_MIXED)
-    return set_error(DFCSR_EINVAL, "wake_mesh_batched: unknown precision %d", precision);
-  WakeTask tasks_h[256];
   if (k > 256) return set_error(DFCSR_ERANGE, "wake_mesh_batched: %d tasks > 256", k);
+  WakeTask tasks_h[256];
   int blocks = 0;
-  for (int j = 0; j < k; ++j) {
-    int rc;
-    if ((rc = check_history(&hist[j], "wake_mesh_batched"))) return rc;
-    if ((rc = check_lattice(&lat[j], "wake_mesh_batched"))) return rc;
-    if ((rc = check_params(&p[j], "wake_mesh_batched"))) return rc;
-    if (!dE_dct[j] || !x_kick[j]) return set_error(DFCSR_EINVAL, "wake_mesh_batched: NULL output");
-    if (precision == DFCSR_PREC_MIXED && !hist[j].data32)
-      return set_error(DFCSR_EINVAL, "wake_mesh_batched: task %d lacks the FP32 history", j);
-    tasks_h[j].H = hist[j];
-    tasks_h[j].L = lat[j];
-    tasks_h[j].P = p[j];
-    tasks_h[j].dE = dE_dct[j];
-    tasks_h[j].xk = x_kick[j];
-    tasks_h[j].block0 = blocks;
-    tasks_h[j].M = p[j].mesh_xbins * p[j].mesh_zbins;
-    blocks += tasks_h[j].M;
-  }
+  int rc = dfcsr_wake_tasks_pack(k, hist, lat, p, precision, dE_dct, x_kick, tasks_h,
+                                 (int64_t)sizeof(tasks_h), &blocks);
+  if (rc) return rc;
   hipStream_t s = as_stream(stream);
-  WakeTask *dt = static_cast<WakeTask *>(ws);
-  DFCSR_CHECK_HIP(hipMemcpyAsync(dt, tasks_h, (size_t)k * sizeof(WakeTask), hipMemcpyHostToDevice, s));
-  if (precision == DFCSR_PREC_MIXED)
-    hipLaunchKernelGGL(wake_batched_kernel<true>, dim3(blocks), dim3(kThreads), 0, s,
-                       (const WakeTask *)dt, k);
-  else
-    hipLaunchKernelGGL(wake_batched_kernel<false>, dim3(blocks), dim3(kThreads), 0, s,
-                       (const WakeTask *)dt, k);
-  DFCSR_LAUNCH_CHECK("wake_mesh_batched");
-  return DFCSR_OK;
+  DFCSR_CHECK_HIP(hipMemcpyAsync(ws, tasks_h, (size_t)k * sizeof(WakeTask), hipMemcpyHostToDevice, s));
+  return dfcsr_wake_batched_launch(ws, k, blocks, precision, stream);
 }

@@ -1,0 +1,355 @@
+"""Serving: continuous batching of many CSR simulations on one GPU.
+
+Requests (a simulation config + arrival time + number of lattice steps) wait
+in a FIFO queue.  At every tick the server
+
+  1. admits arrived requests while a slot is free and the HBM footprint of the
+     request (particles + history ring + scratch, an upper bound fixed at
+     admission) fits the budget; each admitted simulation keeps its growing
+     history on the device for its whole life;
+  2. advances every active simulation through the host side of one step
+     (tracking, deposition, density functions, history update, mesh) up to
+     its wake;
+  3. runs the wakes of all of them in ONE batched launch
+     (dfcsr_wake_batched_launch: one workgroup per mesh point across all
+     simulations), replayed from a HIP graph captured once per active-set
+     shape;
+  4. resumes each simulation to apply its kicks, retires finished ones and
+     frees their memory, so waiting requests get in at the next tick.
+
+Each simulation's wake is computed by the same per-workgroup code whether it
+runs alone or in a mix, so its results do not depend on its neighbours.
+Step latency is the wall time of the tick that completed the step; the
+report gives p50/p99 of it with throughput (simulation steps/s and integrand
+points/s).
+"""
+from __future__ import annotations
+
+import collections
+import ctypes
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+
+
+@dataclass
+class Request:
+    rid: int
+    config: dict
+    arrival: float
+    steps: int
+    precision: str = "mixed"
+    footprint: int = 0
+
+
+@dataclass
+class _Slot:
+    req: Request
+    sim: object
+    gen: object
+    admitted_at: float
+    steps_done: int = 0
+    wake_results: list = field(default_factory=list)
+
+
+def request_footprint(config: dict, steps: int, precision: str = "mixed") -> int:
+    """Upper bound of the device bytes one simulation holds (admission)."""
+    beam = config["input_beam"]
+    n = int(beam.get("n_particle", 0)) or 1_000_000
+    dep = config.get("particle_deposition") or {}
+    nb = max(int(dep.get("xbins", 100)), int(dep.get("zbins", 100)), 100)
+    ul = dep.get("upper_limit")
+    grid = int(ul) if isinstance(ul, int) else 2000
+    slots = steps + 2
+    rec = 48 + (32 if precision == "mixed" else 0)
+    hist = slots * grid * grid * rec
+    raw = slots * 5 * nb * nb * 8
+    particles = n * 8 * (6 * 2 + 4)           # coordinates, tracking temporary, scratch
+    deposit = 4 * nb * nb * 8 * 2 + 3 * nb * nb * 8
+    comp = config.get("CSR_computation") or {}
+    mesh = int(comp.get("xbins", 20)) * int(comp.get("zbins", 30)) * 8 * 4
+    return int(hist + raw + particles + deposit + mesh + (64 << 20))
+
+
+class BatchedWake:
+    """One launch for the wakes of several simulations, optionally replayed
+    from a captured HIP graph (one per (tasks, workgroups) shape)."""
+
+    def __init__(self, device, precision="mixed", use_graph=True, max_tasks=256,
+                 graph_cache=16):
+        import torch
+        self.torch = torch
+        self.device = torch.device(device)
+        self.prec = _lib.DFCSR_PREC_MIXED if precision == "mixed" else _lib.DFCSR_PREC_FP64
+        self.use_graph = use_graph
+        self.lib = _lib.load()
+        self.max_tasks = max_tasks
+        nbytes = int(self.lib.dfcsr_wake_batched_workspace_bytes(max_tasks))
+        self.host = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True)
+        self.dev = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+        self.graphs = collections.OrderedDict()
+        self.graph_cache = graph_cache
+        self.seen = set()
+        self.launches = 0
+        self.replays = 0
+
+    def run(self, items):
+        """items: list of (History, Lattice, WakeParams, out{dE_dct, x_kick})."""
+        torch = self.torch
+        k = len(items)
+        if k == 0:
+            return
+        if k > self.max_tasks:
+            raise _lib.DfcsrError(f"{k} tasks > {self.max_tasks}")
+        H = (_lib.History * k)(*[it[0] for it in items])
+        L = (_lib.Lattice * k)(*[it[1] for it in items])
+        P = (_lib.WakeParams * k)(*[it[2] for it in items])
+        dE = (_lib.c_dp * k)(*[it[3]["dE_dct"].data_ptr() for it in items])
+        xk = (_lib.c_dp * k)(*[it[3]["x_kick"].data_ptr() for it in items])
+        blocks = ctypes.c_int(0)
+        _lib.check(self.lib.dfcsr_wake_tasks_pack(k, H, L, P, self.prec, dE, xk,
+                                                  self.host.data_ptr(), self.host.numel(),
+                                                  ctypes.byref(blocks)), "wake_tasks_pack")
+        nbytes = int(self.lib.dfcsr_wake_batched_workspace_bytes(k))
+        # the previous launch may still read the device task buffer: the copy
+        # is ordered after it on the same stream
+        self.dev[:nbytes].copy_(self.host[:nbytes], non_blocking=True)
+        key = (k, blocks.value)
+        stream = torch.cuda.current_stream(self.device)
+        if self.use_graph and key in self.seen:
+            g = self.graphs.get(key)
+            if g is None:
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g):
+                    _lib.check(self.lib.dfcsr_wake_batched_launch(
+                        self.dev.data_ptr(), k, blocks.value, self.prec,
+                        torch.cuda.current_stream(self.device).cuda_stream), "batched launch")
+                self.graphs[key] = g
+                if len(self.graphs) > self.graph_cache:
+                    self.graphs.popitem(last=False)
+            else:
+                self.graphs.move_to_end(key)
+            g.replay()
+            self.replays += 1
+        else:
+            # first use of a shape runs eagerly (module load stays out of capture)
+            _lib.check(self.lib.dfcsr_wake_batched_launch(self.dev.data_ptr(), k, blocks.value,
+                                                          self.prec, stream.cuda_stream),
+                       "batched launch")
+            self.seen.add(key)
+        self.launches += 1
+
+
+class Server:
+    """Continuous-batching request loop (module docstring)."""
+
+    def __init__(self, device="cuda", hbm_budget=None, max_active=64, precision="mixed",
+                 use_graph=True, sim_factory=None, clock=None, record_wakes=False):
+        self.device = device
+        self.precision = precision
+        self.max_active = max_active
+        self.record_wakes = record_wakes
+        self.waiting = collections.deque()
+        self.active: list[_Slot] = []
+        self.used = 0
+        self.clock = clock or time.perf_counter
+        self.sim_factory = sim_factory or self._make_sim
+        if hbm_budget is None:
+            import torch
+            free, _total = torch.cuda.mem_get_info(torch.device(device))
+            hbm_budget = int(0.85 * free)
+        self.budget = int(hbm_budget)
+        self.batcher = None if sim_factory is not None else BatchedWake(device, precision,
+                                                                         use_graph)
+        self.latencies = []          # per completed simulation step (s)
+        self.queue_waits = []        # arrival -> admission (s)
+        self.points = 0
+        self.steps = 0
+        self.ticks = 0
+        self.finished = {}
+        self.rejected = []
+        self.active_samples = []
+
+    def _make_sim(self, req: Request):
+        from .csr import CSR2D
+        return CSR2D(config=req.config, device=self.device, precision=self.precision,
+                     keep_fp32=(self.precision == "mixed"), history_capacity=req.steps + 2)
+
+    def submit(self, req: Request):
+        if not req.footprint:
+            req.footprint = request_footprint(req.config, req.steps, self.precision)
+        if req.footprint > self.budget:
+            self.rejected.append(req.rid)   # can never fit
+            return False
+        self.waiting.append(req)
+        return True
+
+    def _admit(self, now):
+        while self.waiting and self.waiting[0].arrival <= now:
+            req = self.waiting[0]
+            if len(self.active) >= self.max_active or self.used + req.footprint > self.budget:
+                break
+            self.waiting.popleft()
+            sim = self.sim_factory(req)
+            gen = sim.iter_phases(external_wake=True)
+            self.active.append(_Slot(req, sim, gen, now))
+            self.used += req.footprint
+            self.queue_waits.append(now - req.arrival)
+
+    def _retire(self, slot: _Slot):
+        self.active.remove(slot)
+        self.used -= slot.req.footprint
+        self.finished[slot.req.rid] = slot
+        if not self.record_wakes:
+            slot.sim = None
+            slot.gen = None
+
+    def tick(self, now):
+        """One scheduling round; returns the number of steps completed."""
+        self._admit(now)
+        if not self.active:
+            return 0
+        t0 = self.clock()
+        need, done, ended = [], [], []
+        for slot in list(self.active):
+            try:
+                kind, _step = next(slot.gen)
+            except StopIteration:
+                ended.append(slot)
+                continue
+            (need if kind == "wake" else done).append(slot)
+        if need:
+            items = [s.sim.prepare_wake() for s in need]
+            if self.batcher is not None:
+                self.batcher.run(items)
+            else:
+                for s in need:
+                    s.sim.external_wake(items[need.index(s)])
+            for s in need:
+                self.points += s.sim.integrand_points()
+                kind, _step = next(s.gen)          # kicks, end of step
+                if self.record_wakes:
+                    s.wake_results.append((s.sim.dE_dct.detach().clone(),
+                                           s.sim.x_kick.detach().clone()))
+                done.append(s)
+        if self.batcher is not None:
+            self.batcher.torch.cuda.synchronize()
+        dt = self.clock() - t0
+        for s in done:
+            s.steps_done += 1
+            self.latencies.append(dt)
+            self.steps += 1
+            if s.steps_done >= s.req.steps:
+                ended.append(s)
+        self.active_samples.append(len(self.active))
+        for s in ended:
+            if s in self.active:
+                self._retire(s)
+        self.ticks += 1
+        return len(done)
+
+    def run(self, duration=None, max_ticks=None, simulated_clock=False, tick_dt=0.0):
+        """Runs until every request finished (or duration / max_ticks).
+        simulated_clock: arrivals are compared with a virtual time advanced
+        by each tick's wall time (plus tick_dt), not with real time."""
+        start = self.clock()
+        vnow = 0.0
+        while self.waiting or self.active:
+            now = vnow if simulated_clock else self.clock() - start
+            if duration is not None and (self.clock() - start) > duration:
+                break
+            if max_ticks is not None and self.ticks >= max_ticks:
+                break
+            if not self.active and self.waiting and self.waiting[0].arrival > now:
+                if simulated_clock:
+                    vnow = self.waiting[0].arrival
+                else:
+                    time.sleep(min(0.05, self.waiting[0].arrival - now))
+                continue
+            t0 = self.clock()
+            self.tick(now)
+            vnow += (self.clock() - t0) + tick_dt
+        elapsed = self.clock() - start
+        return self.report(elapsed)
+
+    def report(self, elapsed):
+        lat = np.array(self.latencies) if self.latencies else np.zeros(1)
+        qw = np.array(self.queue_waits) if self.queue_waits else np.zeros(1)
+        return {
+            "elapsed_s": round(elapsed, 3),
+            "ticks": self.ticks,
+            "sim_steps": self.steps,
+            "steps_per_s": round(self.steps / elapsed, 3) if elapsed > 0 else None,
+            "gpts_per_s": round(self.points / elapsed / 1e9, 3) if elapsed > 0 else None,
+            "step_latency_ms_p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
+            "step_latency_ms_p99": round(float(np.percentile(lat, 99)) * 1e3, 3),
+            "queue_wait_s_p50": round(float(np.percentile(qw, 50)), 3),
+            "queue_wait_s_p99": round(float(np.percentile(qw, 99)), 3),
+            "finished": len(self.finished),
+            "rejected": len(self.rejected),
+            "still_waiting": len(self.waiting),
+            "still_active": len(self.active),
+            "mean_active": round(float(np.mean(self.active_samples)), 2)
+            if self.active_samples else 0.0,
+            "batched_launches": self.batcher.launches if self.batcher else 0,
+            "graph_replays": self.batcher.replays if self.batcher else 0,
+            "hbm_budget_bytes": self.budget,
+        }
+
+
+# -- the C4 request mix -------------------------------------------------------------------
+def mix_requests(n=64, rate=2.0, seed=0, steps_range=(100, 800), particles_range=(1e5, 1e7),
+                 grids=(64, 128, 256), meshes=(32, 64, 128), upper_limit=600, Nxz=200,
+                 precision="mixed"):
+    """Seeded request stream: Poisson arrivals at `rate` per second; particle
+    count log-uniform; deposition grid, wake mesh, lattice (single dipole with
+    rho 5-20 m, or a 4-bend chicane at 0.02-0.08 rad), history length, beam
+    sizes (20-100 um) and chirp (-20..0 /m) drawn per request."""
+    from collections import OrderedDict
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for rid in range(n):
+        t += rng.exponential(1.0 / rate)
+        steps = int(rng.integers(steps_range[0], steps_range[1] + 1))
+        npart = int(10 ** rng.uniform(math.log10(particles_range[0]),
+                                      math.log10(particles_range[1])))
+        nb = int(rng.choice(grids))
+        mesh = int(rng.choice(meshes))
+        lat = OrderedDict()
+        if rng.random() < 0.5:
+            rho = rng.uniform(5.0, 20.0)
+            L = 0.5
+            lat["step_size"] = L / steps
+            lat["B1"] = dict(type="dipole", L=L, angle=L / rho, nsep=1)
+        else:
+            ang = rng.uniform(0.02, 0.08)
+            total = 4 * 0.3 + 5.0
+            lat["step_size"] = total / steps
+            lat["B1"] = dict(type="dipole", L=0.3, angle=ang, E1=0.0, E2=ang, nsep=1)
+            lat["D1"] = dict(type="drift", L=2.0, nsep=1)
+            lat["B2"] = dict(type="dipole", L=0.3, angle=-ang, E1=-ang, E2=0.0, nsep=1)
+            lat["D2"] = dict(type="drift", L=1.0, nsep=1)
+            lat["B3"] = dict(type="dipole", L=0.3, angle=-ang, E1=0.0, E2=-ang, nsep=1)
+            lat["D3"] = dict(type="drift", L=2.0, nsep=1)
+            lat["B4"] = dict(type="dipole", L=0.3, angle=ang, E1=ang, E2=0.0, nsep=1)
+        cfg = {
+            "input_beam": dict(style="gaussian", n_particle=npart, charge=1e-9, energy=1e9,
+                               sigma_x=rng.uniform(20e-6, 100e-6),
+                               sigma_z=rng.uniform(20e-6, 100e-6),
+                               chirp=rng.uniform(-20.0, 0.0), sigma_delta=1e-4,
+                               seed=int(rng.integers(1 << 30))),
+            "input_lattice": dict(lattice_input_file=f"<request {rid}>", _config=lat),
+            "particle_deposition": dict(xbins=nb, zbins=nb, xlim=5, zlim=5, filter_order=2,
+                                        filter_window=7, velocity_threhold=1000,
+                                        upper_limit=upper_limit),
+            "CSR_integration": dict(n_formation_length=4, zbins=Nxz, xbins=Nxz),
+            "CSR_computation": dict(xbins=mesh, zbins=mesh, xlim=5, zlim=5, apply_CSR=1,
+                                    transverse_on=1, write_wakes=False),
+        }
+        out.append(Request(rid, cfg, t, steps, precision))
+    return out

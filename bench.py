@@ -70,7 +70,8 @@ def cpu_baseline(sim, gpu_out, budget_s=15.0, max_points=None):
     keep = C._Keep()
     H = C.hist_from_records(rec, tr.head, tr.nt, (float(tr.min_x), float(tr.min_y),
                                                   float(tr.min_z), float(tr.delta_x),
-                                                  float(tr.delta_y), float(tr.delta_z)), keep)
+                                                  float(tr.delta_y), float(tr.delta_z)), keep,
+                                field_stride=4)
     L = C.lat_from_tables(sim.lattice, keep)
     b = sim.beam
     beam = dict(sigma_x=b._sigma_x, sigma_z=b._sigma_z, slope=b._slope, mean_x=b._mean_x)

@@ -38,11 +38,16 @@ enum { DFCSR_PREC_FP64 = 0, DFCSR_PREC_MIXED = 1 };
 /* Order of the five history fields in every slice record. */
 enum { DFCSR_F_DENSITY = 0, DFCSR_F_DENSITY_X = 1, DFCSR_F_DENSITY_Z = 2,
        DFCSR_F_VX = 3, DFCSR_F_VX_X = 4 };
-/* History record strides: FP64 slices hold 6 doubles per (x, z) cell (five
- * fields + one pad so a z-neighbour pair is 16-byte aligned), the FP32
- * shadow holds 8 floats per cell (five fields + three pads). */
-#define DFCSR_HIST_REC64 6
-#define DFCSR_HIST_REC32 8
+/* History records are corner-packed: the record of cell (x, z) holds, for
+ * each field f, the values at (x, z), (x, z1), (x1, z), (x1, z1) with
+ * x1 = min(x + 1, Xi - 1) and z1 = min(z + 1, Zi - 1) -- the 2 x 2
+ * neighbourhood a trilinear lookup reads in one time plane, with the
+ * reference's last-plane clamp (interp3D.py:31-46) built in.  Element
+ * [f * 4 + c], c = 0..3 in that order.  FP64 records are 20 doubles
+ * (160 B: two cache lines), FP32 shadow records 32 floats (20 used, 128 B:
+ * one cache line). */
+#define DFCSR_HIST_REC64 20
+#define DFCSR_HIST_REC32 32
 #define DFCSR_MAX_ELEMENTS 64
 #define DFCSR_MAX_SG_WINDOW 21
 

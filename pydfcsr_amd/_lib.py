@@ -23,8 +23,8 @@ c_f64 = ctypes.c_double
 DFCSR_PREC_FP64 = 0
 DFCSR_PREC_MIXED = 1
 MOMENTS_COUNT = 10
-HIST_REC64 = 6
-HIST_REC32 = 8
+HIST_REC64 = 20   # corner-packed records: field f, corner c at [4 f + c]
+HIST_REC32 = 32
 FIELDS = ("density", "density_x", "density_z", "vx", "vx_x")   # DFCSR_F_* order
 
 

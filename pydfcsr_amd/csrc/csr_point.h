@@ -89,33 +89,16 @@ __device__ inline bool hist_lookup5(const DfcsrHistory &H, const HistAxes &A, do
     return false;
   }
   const double td = T.frac, xd = X.frac, zd = Z.frac;
-  const double *p000 = hist_rec(H, T.i0, X.i0, Z.i0), *p100 = hist_rec(H, T.i1, X.i0, Z.i0);
-  const double *p010 = hist_rec(H, T.i0, X.i1, Z.i0), *p110 = hist_rec(H, T.i1, X.i1, Z.i0);
-  const int dz1 = (Z.i1 - Z.i0) * DFCSR_HIST_REC64;
-#ifdef DFCSR_TRILINEAR_SUM
-  if (kRecip) {
-    // weighted sum of the eight corners, one (t, x) line pair at a time
-    const double w0 = 1 - td, w1 = td, v0 = 1 - xd, v1 = xd, u0 = 1 - zd, u1 = zd;
-    const double *pp[4] = {p000, p100, p010, p110};
-    const double wt[4] = {w0 * v0, w1 * v0, w0 * v1, w1 * v1};
-#pragma unroll
-    for (int f = 0; f < 5; ++f) out[f] = 0.0;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-#pragma unroll
-      for (int f = 0; f < 5; ++f)
-        out[f] = fma(wt[c], fma(u1, pp[c][f + dz1], u0 * pp[c][f]), out[f]);
-    }
-    return true;
-  }
-#endif
+  // corner-packed records: p0 at (t0, x0, z0), p1 at (t1, x0, z0); element
+  // f*4 + {0,1,2,3} = (x0,z0), (x0,z1), (x1,z0), (x1,z1)
+  const double *p0 = hist_rec(H, T.i0, X.i0, Z.i0), *p1 = hist_rec(H, T.i1, X.i0, Z.i0);
 #pragma unroll
   for (int f = 0; f < 5; ++f) {
     // interp3D.py:53-61: lerp along t, then x, then z
-    double c00 = p000[f] * (1 - td) + p100[f] * td;
-    double c01 = p000[f + dz1] * (1 - td) + p100[f + dz1] * td;
-    double c10 = p010[f] * (1 - td) + p110[f] * td;
-    double c11 = p010[f + dz1] * (1 - td) + p110[f + dz1] * td;
+    double c00 = p0[4 * f + 0] * (1 - td) + p1[4 * f + 0] * td;
+    double c01 = p0[4 * f + 1] * (1 - td) + p1[4 * f + 1] * td;
+    double c10 = p0[4 * f + 2] * (1 - td) + p1[4 * f + 2] * td;
+    double c11 = p0[4 * f + 3] * (1 - td) + p1[4 * f + 3] * td;
     double c0 = c00 * (1 - xd) + c10 * xd;
     double c1 = c01 * (1 - xd) + c11 * xd;
     out[f] = c0 * (1 - zd) + c1 * zd;

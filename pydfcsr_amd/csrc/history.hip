@@ -9,10 +9,11 @@
 //   v00(1-y0)(1-y1) + v01(1-y0)y1 + v10 y0(1-y1) + v11 y0 y1,
 // with fill 0 outside the grid (mean(vx_x) of the slice for vx_x).
 //
-// History layout: FP64 records [slot][Xi][Zi][6] (density, density_x,
-// density_z, vx, vx_x, pad) so the two z-neighbours a trilinear lookup needs
-// are 96 contiguous, 16-byte aligned bytes; the optional FP32 shadow uses
-// [slot][Xi][Zi][8].
+// History layout: corner-packed records [slot][Xi][Zi][20] (float64) and
+// [slot][Xi][Zi][32] (float32 shadow): record (x, z) holds field f at
+// (x, z), (x, z1), (x1, z), (x1, z1) in elements f*4 + 0..3 (x1, z1 clamped
+// to the last plane), so a trilinear lookup reads one record per time
+// plane.
 #include "common.h"
 
 namespace dfcsr {
@@ -36,51 +37,66 @@ __global__ __launch_bounds__(kThreads) void vxx_mean_kernel(const double *__rest
   if (threadIdx.x == 0) means[blockIdx.x] = s / (double)N;
 }
 
+// RegularGridInterpolator value of the five fields of job J at (xq, zq).
+__device__ inline void rgi5(const DfcsrRegridJob &J, const double *__restrict__ base,
+                            double fill_vxx, double xq, double zq, double v[5]) {
+#pragma clang fp contract(off)
+  const int64_t Nsrc = (int64_t)J.nbx * J.nbz;
+  bool oob = !(xq >= J.x0 && xq <= J.x1 && zq >= J.z0 && zq <= J.z1);
+  if (oob || J.nbx < 2 || J.nbz < 2) {
+    for (int f = 0; f < 5; ++f) v[f] = (f == DFCSR_F_VX_X) ? fill_vxx : 0.0;
+    return;
+  }
+  int i = rgi_interval(xq, J.x0, J.x1, J.nbx);
+  int j = rgi_interval(zq, J.z0, J.z1, J.nbz);
+  double gi = linspace_at(J.x0, J.x1, J.nbx, i), gi1 = linspace_at(J.x0, J.x1, J.nbx, i + 1);
+  double gj = linspace_at(J.z0, J.z1, J.nbz, j), gj1 = linspace_at(J.z0, J.z1, J.nbz, j + 1);
+  double y0 = (xq - gi) / (gi1 - gi);
+  double y1 = (zq - gj) / (gj1 - gj);
+  int64_t o = (int64_t)i * J.nbz + j;
+  for (int f = 0; f < 5; ++f) {
+    const double *F = base + (int64_t)f * Nsrc;
+    double r = F[o] * (1 - y0) * (1 - y1);
+    r = r + F[o + 1] * (1 - y0) * y1;
+    r = r + F[o + J.nbz] * y0 * (1 - y1);
+    r = r + F[o + J.nbz + 1] * y0 * y1;
+    v[f] = r;
+  }
+}
+
+// One thread per target record: evaluates the four corner points of the
+// record and writes the corner-packed float64 record (and float32 shadow).
 __global__ __launch_bounds__(kThreads) void regrid_kernel(
     const double *__restrict__ src, int64_t slot_stride, const DfcsrRegridJob *__restrict__ jobs,
     const double *__restrict__ means, double *__restrict__ h64, float *__restrict__ h32, int Xi,
     int Zi, double xi0, double xi1, double zi0, double zi1) {
-#pragma clang fp contract(off)
   const DfcsrRegridJob J = jobs[blockIdx.y];
   const int64_t cells = (int64_t)Xi * Zi;
-  const int64_t Nsrc = (int64_t)J.nbx * J.nbz;
   const double *base = src + (int64_t)J.src_slot * slot_stride;
   const double fill_vxx = means[blockIdx.y];
   for (int64_t c = (int64_t)blockIdx.x * kThreads + threadIdx.x; c < cells;
        c += (int64_t)gridDim.x * kThreads) {
-    int ix = (int)(c / Zi), iz = (int)(c % Zi);
-    double xq = linspace_at(xi0, xi1, Xi, ix);
-    double zq = linspace_at(zi0, zi1, Zi, iz);
-    double v[5];
-    bool oob = !(xq >= J.x0 && xq <= J.x1 && zq >= J.z0 && zq <= J.z1);
-    if (oob || J.nbx < 2 || J.nbz < 2) {
-      for (int f = 0; f < 5; ++f) v[f] = (f == DFCSR_F_VX_X) ? fill_vxx : 0.0;
-    } else {
-      int i = rgi_interval(xq, J.x0, J.x1, J.nbx);
-      int j = rgi_interval(zq, J.z0, J.z1, J.nbz);
-      double gi = linspace_at(J.x0, J.x1, J.nbx, i), gi1 = linspace_at(J.x0, J.x1, J.nbx, i + 1);
-      double gj = linspace_at(J.z0, J.z1, J.nbz, j), gj1 = linspace_at(J.z0, J.z1, J.nbz, j + 1);
-      double y0 = (xq - gi) / (gi1 - gi);
-      double y1 = (zq - gj) / (gj1 - gj);
-      int64_t o = (int64_t)i * J.nbz + j;
-      for (int f = 0; f < 5; ++f) {
-        const double *F = base + (int64_t)f * Nsrc;
-        double r = F[o] * (1 - y0) * (1 - y1);
-        r = r + F[o + 1] * (1 - y0) * y1;
-        r = r + F[o + J.nbz] * y0 * (1 - y1);
-        r = r + F[o + J.nbz + 1] * y0 * y1;
-        v[f] = r;
-      }
+    const int ix = (int)(c / Zi), iz = (int)(c % Zi);
+    const int ix1 = ix + 1 < Xi ? ix + 1 : ix, iz1 = iz + 1 < Zi ? iz + 1 : iz;
+    const double x0 = linspace_at(xi0, xi1, Xi, ix), x1 = linspace_at(xi0, xi1, Xi, ix1);
+    const double z0 = linspace_at(zi0, zi1, Zi, iz), z1 = linspace_at(zi0, zi1, Zi, iz1);
+    double v[4][5];
+    rgi5(J, base, fill_vxx, x0, z0, v[0]);
+    rgi5(J, base, fill_vxx, x0, z1, v[1]);
+    rgi5(J, base, fill_vxx, x1, z0, v[2]);
+    rgi5(J, base, fill_vxx, x1, z1, v[3]);
+    double2 *d2 = reinterpret_cast<double2 *>(h64 + ((int64_t)J.dst_slot * cells + c) * DFCSR_HIST_REC64);
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+      d2[2 * f] = make_double2(v[0][f], v[1][f]);
+      d2[2 * f + 1] = make_double2(v[2][f], v[3][f]);
     }
-    double *d = h64 + ((int64_t)J.dst_slot * cells + c) * DFCSR_HIST_REC64;
-    double2 *d2 = reinterpret_cast<double2 *>(d);
-    d2[0] = make_double2(v[0], v[1]);
-    d2[1] = make_double2(v[2], v[3]);
-    d2[2] = make_double2(v[4], 0.0);
     if (h32) {
       float4 *s4 = reinterpret_cast<float4 *>(h32 + ((int64_t)J.dst_slot * cells + c) * DFCSR_HIST_REC32);
-      s4[0] = make_float4((float)v[0], (float)v[1], (float)v[2], (float)v[3]);
-      s4[1] = make_float4((float)v[4], 0.f, 0.f, 0.f);
+#pragma unroll
+      for (int f = 0; f < 5; ++f)
+        s4[f] = make_float4((float)v[0][f], (float)v[1][f], (float)v[2][f], (float)v[3][f]);
+      s4[5] = s4[6] = s4[7] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
   }
 }

@@ -281,8 +281,8 @@ struct Neumaier {
 
 // One source point in mixed precision; false when outside the history.
 __device__ inline bool csr_point_mixed(const DfcsrHistory &H, const HistAxes &A, double t,
-                                       const ColumnMixed &c, double xp, int xi0, int xi1,
-                                       float xd, float &Iz, float &Ix) {
+                                       const ColumnMixed &c, double xp, int xi0, float xd,
+                                       float &Iz, float &Ix) {
   const double rx = c.ax - xp * c.npx;
   const double ry = c.ay - xp * c.npy;
   const double R2 = rx * rx + ry * ry;
@@ -295,16 +295,17 @@ __device__ inline bool csr_point_mixed(const DfcsrHistory &H, const HistAxes &A,
   const TruncIdx T = trunc_index(ut, H.nt), Z = trunc_index(uz, H.Zi);
   if (!(T.ok && Z.ok)) return false;
   const float td = (float)T.frac, zd = (float)Z.frac;
-  const float *p00 = hist_rec32(H, T.i0, xi0, Z.i0), *p10 = hist_rec32(H, T.i1, xi0, Z.i0);
-  const float *p01 = hist_rec32(H, T.i0, xi1, Z.i0), *p11 = hist_rec32(H, T.i1, xi1, Z.i0);
-  const int dz1 = (Z.i1 - Z.i0) * DFCSR_HIST_REC32;
+  // one corner-packed record (one cache line) per time plane
+  const float4 *q0 = reinterpret_cast<const float4 *>(hist_rec32(H, T.i0, xi0, Z.i0));
+  const float4 *q1 = reinterpret_cast<const float4 *>(hist_rec32(H, T.i1, xi0, Z.i0));
   float f[5];
 #pragma unroll
   for (int k = 0; k < 5; ++k) {
-    const float c00 = fmaf(td, p10[k] - p00[k], p00[k]);
-    const float c01 = fmaf(td, p10[k + dz1] - p00[k + dz1], p00[k + dz1]);
-    const float c10 = fmaf(td, p11[k] - p01[k], p01[k]);
-    const float c11 = fmaf(td, p11[k + dz1] - p01[k + dz1], p01[k + dz1]);
+    const float4 a = q0[k], b = q1[k];   // (x0,z0), (x0,z1), (x1,z0), (x1,z1)
+    const float c00 = fmaf(td, b.x - a.x, a.x);
+    const float c01 = fmaf(td, b.y - a.y, a.y);
+    const float c10 = fmaf(td, b.z - a.z, a.z);
+    const float c11 = fmaf(td, b.w - a.w, a.w);
     const float c0 = fmaf(xd, c10 - c00, c00);
     const float c1 = fmaf(xd, c11 - c01, c01);
     f[k] = fmaf(zd, c1 - c0, c0);
@@ -369,7 +370,7 @@ __device__ void wake_point_mixed(const DfcsrHistory &H, const DfcsrLattice &L,
       const TruncIdx X = trunc_index((xp - A.min_x) * A.ix, H.Xi);
       if (!X.ok) continue;   // wave-uniform: rows are shared
       float Iz, Ix;
-      if (!csr_point_mixed(H, A, t, col, xp, X.i0, X.i1, (float)X.frac, Iz, Ix)) continue;
+      if (!csr_point_mixed(H, A, t, col, xp, X.i0, (float)X.frac, Iz, Ix)) continue;
       const float wx = (float)xa.w(j);
       cz.add(wx * Iz);
       cx.add(wx * Ix);

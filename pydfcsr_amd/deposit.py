@@ -9,8 +9,10 @@ interp grid; build_interpolant fixes the axes the wake kernel reads.
 Storage is two rings with one slot per logged slice, on the device for the
 whole run:
   raw   [cap][5][nbmax*nbmax] float64  fields on the slice's own grid
-  hist  [cap][Xi][Zi][6]      float64  re-gridded records (5 fields + pad)
-  hist32[cap][Xi][Zi][8]      float32  shadow read by the mixed-precision wake
+  hist  [cap][Xi][Zi][20]     float64  re-gridded corner-packed records
+  hist32[cap][Xi][Zi][32]     float32  shadow read by the mixed-precision wake
+(record (x, z) holds every field at (x, z), (x, z+1), (x+1, z), (x+1, z+1),
+clamped to the last plane: one record per time plane per lookup)
 Appending writes slot (head + nt) % cap; popping advances head.  Nothing is
 copied per step: the reference's np.array(deque) stacking of the whole
 history (deposit.py:422-426) becomes a descriptor update.
@@ -241,7 +243,7 @@ class DF_tracker:
         (the reference's data_*_interp arrays; for inspection and tests)."""
         idx = torch.tensor([self._slot(i) for i in range(self.nt)], device=self.device)
         h = self.hist.index_select(0, idx)
-        return {f: h[..., k] for k, f in enumerate(_lib.FIELDS)}
+        return {f: h[..., 4 * k] for k, f in enumerate(_lib.FIELDS)}   # corner (x, z)
 
     def footprint_bytes(self):
         b = self.raw.numel() * 8 if self.raw is not None else 0

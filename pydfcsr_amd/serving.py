@@ -65,7 +65,7 @@ def request_footprint(config: dict, steps: int, precision: str = "mixed") -> int
     ul = dep.get("upper_limit")
     grid = int(ul) if isinstance(ul, int) else 2000
     slots = steps + 2
-    rec = 48 + (32 if precision == "mixed" else 0)
+    rec = 160 + (128 if precision == "mixed" else 0)   # corner-packed records
     hist = slots * grid * grid * rec
     raw = slots * 5 * nb * nb * 8
     particles = n * 8 * (6 * 2 + 4)           # coordinates, tracking temporary, scratch

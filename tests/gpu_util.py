@@ -10,6 +10,22 @@ from pydfcsr_amd import _lib
 FIELDS = _lib.FIELDS
 
 
+def corner_records(data, rec_len):
+    """(nt, Xi, Zi, 5) field stacks -> corner-packed records (nt, Xi, Zi, rec_len):
+    element 4 f + c holds field f at (x, z), (x, z1), (x1, z), (x1, z1)."""
+    nt, Xi, Zi, _ = data.shape
+    x1 = np.minimum(np.arange(Xi) + 1, Xi - 1)
+    z1 = np.minimum(np.arange(Zi) + 1, Zi - 1)
+    rec = np.zeros((nt, Xi, Zi, rec_len), dtype=data.dtype)
+    for f in range(5):
+        d = data[..., f]
+        rec[..., 4 * f + 0] = d
+        rec[..., 4 * f + 1] = d[:, :, z1]
+        rec[..., 4 * f + 2] = d[:, x1, :]
+        rec[..., 4 * f + 3] = d[:, x1][:, :, z1]
+    return rec
+
+
 class DeviceHistory:
     """A device ring holding an oracle DFTracker's stacked history.
 
@@ -20,14 +36,15 @@ class DeviceHistory:
         nt, Xi, Zi, _ = data.shape
         cap = nt + (wrap + 1 if wrap else 0)
         head = (cap - wrap) % cap if wrap else 0
+        packed = corner_records(data, _lib.HIST_REC64)
         rec = np.zeros((cap, Xi, Zi, _lib.HIST_REC64))
         for i in range(nt):
-            rec[(head + i) % cap, :, :, :5] = data[i]
+            rec[(head + i) % cap] = packed[i]
         self.h64 = torch.from_numpy(rec).to(device)
         self.h32 = None
         if fp32:
             r32 = np.zeros((cap, Xi, Zi, _lib.HIST_REC32), dtype=np.float32)
-            r32[..., :5] = rec[..., :5].astype(np.float32)
+            r32[..., :20] = rec[..., :20].astype(np.float32)
             self.h32 = torch.from_numpy(r32).to(device)
         a = tracker.axes()
         self.desc = _lib.History(self.h64.data_ptr(),

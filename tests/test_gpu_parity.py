@@ -125,9 +125,10 @@ def test_interpolate_golden_bitexact(K, golden_dir):
     d = _g(golden_dir, "ref_interpolate.npz")
     data = d["data3"]
     nt, nx, nz = data.shape
-    rec = np.zeros((nt, nx, nz, _lib.HIST_REC64))
-    rec[..., 0] = data
-    h = _t(rec)
+    from tests.gpu_util import corner_records
+    stack = np.zeros((nt, nx, nz, 5))
+    stack[..., 0] = data
+    h = _t(corner_records(stack, _lib.HIST_REC64))
     H = _lib.History(h.data_ptr(), None, nt, 0, nt, nx, nz, *[float(v) for v in d["mins"]],
                      *[float(v) for v in d["dels"]])
     r = K.interpolate3D(H, 0, _t(d["q3"]), DEV).cpu().numpy()

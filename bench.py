@@ -17,8 +17,9 @@ are timed.  ``value`` is the integrand throughput of those K steps (points
 evaluated by the retarded integral per second, one point = one (x', s')
 source for one mesh point); ms_per_step is their wall time per step.
 
-The JSON line also carries the roofline of the dominant kernel (the wake
-integral, timed with HIP events on the stream it runs on) and a CPU baseline:
+The JSON line also carries the roofline of the dominant work (the wake
+integral: its four launches timed together with HIP events on the stream they
+run on; traffic from the committed counter runs) and a CPU baseline:
 the oracle's C/OpenMP restatement of the same integral on the same history,
 on a bounded sample of mesh points, on this host's cores.
 """
@@ -70,8 +71,7 @@ def cpu_baseline(sim, gpu_out, budget_s=15.0, max_points=None):
     keep = C._Keep()
     H = C.hist_from_records(rec, tr.head, tr.nt, (float(tr.min_x), float(tr.min_y),
                                                   float(tr.min_z), float(tr.delta_x),
-                                                  float(tr.delta_y), float(tr.delta_z)), keep,
-                                field_stride=4)
+                                                  float(tr.delta_y), float(tr.delta_z)), keep)
     L = C.lat_from_tables(sim.lattice, keep)
     b = sim.beam
     beam = dict(sigma_x=b._sigma_x, sigma_z=b._sigma_z, slope=b._slope, mean_x=b._mean_x)
@@ -284,7 +284,9 @@ def main():
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
                      "traffic": profiled_traffic(),
-                     "kernel": "wake_fp64_kernel", "flops_per_point": FLOPS_PER_POINT},
+                     "kernel": "FP64 wake (wake_observer_kernel, wake_phase_kernel far and "
+                               "near, wake_finalize_kernel; one stream, timed together)",
+                     "flops_per_point": FLOPS_PER_POINT},
         "init_s": round(t0 - t_init, 2),
     }
     res["config"]["s_start_m"] = round(float(sim.beam.position), 4)

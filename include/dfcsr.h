@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define DFCSR_ABI_VERSION 1
+#define DFCSR_ABI_VERSION 2
 
 enum {
   DFCSR_OK = 0,
@@ -43,11 +43,18 @@ enum { DFCSR_F_DENSITY = 0, DFCSR_F_DENSITY_X = 1, DFCSR_F_DENSITY_Z = 2,
  * x1 = min(x + 1, Xi - 1) and z1 = min(z + 1, Zi - 1) -- the 2 x 2
  * neighbourhood a trilinear lookup reads in one time plane, with the
  * reference's last-plane clamp (interp3D.py:31-46) built in.  Element
- * [f * 4 + c], c = 0..3 in that order.  FP64 records are 20 doubles
- * (160 B: two cache lines), FP32 shadow records 32 floats (20 used, 128 B:
- * one cache line). */
+ * [f * 4 + c], c = 0..3 in that order: 20 values per record.
+ * FP64 ring: whole records, time slot innermost, [x][z][slot][20] (slot =
+ * ring slot, see DfcsrHistory): the two time planes of a lookup are
+ * neighbouring records, and the far-region lookups of one wavefront (many
+ * retarded times, nearby z) stay within one x, z block.
+ * FP32 shadow ring: records cut into 5 chunks of 16 bytes (chunk f = the
+ * four corners of field f) stored z fastest, [x][slot][f][z][4], so the
+ * near-region lookups of one wavefront (one time plane, consecutive z) read
+ * consecutive chunks. */
 #define DFCSR_HIST_REC64 20
-#define DFCSR_HIST_REC32 32
+#define DFCSR_HIST_REC32 20
+#define DFCSR_HIST_CHUNKS32 5
 #define DFCSR_MAX_ELEMENTS 64
 #define DFCSR_MAX_SG_WINDOW 21
 
@@ -117,7 +124,8 @@ int dfcsr_df_process(double *density, double *vx, double *density_x, double *den
  * np.linspace(xi0, xi1, Xi) x np.linspace(zi0, zi1, Zi).
  * src: raw slices, src_slot_stride doubles apart, each holding the five
  * fields as [field][nbx][nbz] (field order DFCSR_F_*).
- * jobs: HOST array of njobs descriptors. */
+ * jobs: HOST array of njobs descriptors; hist64/hist32 are rings of cap slots
+ * in the chunked layout above (dst_slot < cap). */
 typedef struct {
   int32_t src_slot;   /* raw slice index in src */
   int32_t dst_slot;   /* ring slot in the history */
@@ -126,7 +134,7 @@ typedef struct {
 } DfcsrRegridJob;
 int64_t dfcsr_regrid_workspace_bytes(int njobs);
 int dfcsr_regrid(const double *src, int64_t src_slot_stride, const DfcsrRegridJob *jobs,
-                 int njobs, double *hist64, float *hist32, int Xi, int Zi, double xi0,
+                 int njobs, double *hist64, float *hist32, int Xi, int Zi, int cap, double xi0,
                  double xi1, double zi0, double zi1, void *ws, int64_t ws_bytes, void *stream);
 
 /* ---- a11/a12: lookup primitives (exposed for parity tests) -----------------
@@ -135,8 +143,8 @@ int dfcsr_regrid(const double *src, int64_t src_slot_stride, const DfcsrRegridJo
  * min_t = oldest time, dt = (t_last - t_first)/(nt - 1), min_x = xi0,
  * dx = (xi1 - xi0)/(Xi - 1), same for z. */
 typedef struct {
-  const double *data;   /* [cap][Xi][Zi][DFCSR_HIST_REC64] */
-  const float *data32;  /* [cap][Xi][Zi][DFCSR_HIST_REC32] or NULL */
+  const double *data;   /* [Xi][Zi][cap][DFCSR_HIST_REC64] */
+  const float *data32;  /* [Xi][cap][DFCSR_HIST_CHUNKS32][Zi][4] or NULL */
   int32_t cap, head, nt, Xi, Zi;
   double min_t, min_x, min_z, dt, dx, dz;
 } DfcsrHistory;
@@ -185,11 +193,14 @@ int dfcsr_csr_integrand(const DfcsrHistory *hist, const DfcsrLattice *lat, doubl
                         double *Iz, double *Ix, void *stream);
 
 /* get_CSR_mesh + calculate_2D_CSR + get_CSR_wake (CSR.py:361-602):
- * xmesh, zmesh (M = mesh_xbins*mesh_zbins, may be NULL), dE_dct, x_kick
- * ([mesh_xbins][mesh_zbins], MeV/m).  precision: DFCSR_PREC_FP64 or
- * DFCSR_PREC_MIXED (FP32 integrand with compensated FP32 accumulation;
- * FP64 retardation geometry and final sum; needs hist->data32). */
-int64_t dfcsr_wake_workspace_bytes(int mesh_points);
+ * xmesh, zmesh (M = mesh_xbins*mesh_zbins, both or neither may be NULL),
+ * dE_dct, x_kick ([mesh_xbins][mesh_zbins], MeV/m).  precision:
+ * DFCSR_PREC_FP64 or DFCSR_PREC_MIXED (FP32 integrand with compensated FP32
+ * accumulation; FP64 retardation geometry and final sum; needs
+ * hist->data32).  The workspace holds per-phase partial sums (a phase is one
+ * integration region and one chunk of its x' rows): its size depends on the
+ * mesh and integration bins and the chirp branch of p. */
+int64_t dfcsr_wake_workspace_bytes(const DfcsrWakeParams *p);
 int dfcsr_wake_mesh(const DfcsrHistory *hist, const DfcsrLattice *lat, const DfcsrWakeParams *p,
                     int precision, double *xmesh, double *zmesh, double *dE_dct, double *x_kick,
                     void *ws, int64_t ws_bytes, void *stream);
@@ -200,18 +211,28 @@ int dfcsr_wake_mesh(const DfcsrHistory *hist, const DfcsrLattice *lat, const Dfc
 int dfcsr_wake_mesh_batched(int k, const DfcsrHistory *hist, const DfcsrLattice *lat,
                             const DfcsrWakeParams *p, int precision, double *const *dE_dct,
                             double *const *x_kick, void *ws, int64_t ws_bytes, void *stream);
-int64_t dfcsr_wake_batched_workspace_bytes(int k);
+int64_t dfcsr_wake_batched_workspace_bytes(int k, const DfcsrWakeParams *p);
 /* The batched form in two halves, so the launch can be captured in a HIP
  * graph and replayed with new task contents: dfcsr_wake_tasks_pack writes
- * the k task descriptors into host memory `tasks` (size
- * dfcsr_wake_batched_workspace_bytes(k)) and the workgroup count into
- * *blocks; the caller copies those bytes to device memory and
- * dfcsr_wake_batched_launch runs them (no host-device copy inside). */
+ * the k task descriptors into host memory `tasks` (at least
+ * dfcsr_wake_task_table_bytes(k) bytes) and the launch shape into *batch;
+ * the caller copies those bytes to device memory and
+ * dfcsr_wake_batched_launch runs them with a device buffer of at least
+ * batch->partial_bytes for the partial sums (no host-device copy inside). */
+typedef struct {
+  int32_t tasks;          /* k */
+  int32_t blocks_far;     /* workgroups of the region-0 launch */
+  int32_t blocks_near;    /* workgroups of the launch over the other regions */
+  int32_t observers;      /* mesh points over all tasks */
+  int64_t partial_bytes;  /* partial-sum buffer the launch needs */
+} DfcsrWakeBatch;
+int64_t dfcsr_wake_task_table_bytes(int k);
 int dfcsr_wake_tasks_pack(int k, const DfcsrHistory *hist, const DfcsrLattice *lat,
                           const DfcsrWakeParams *p, int precision, double *const *dE_dct,
-                          double *const *x_kick, void *tasks, int64_t tasks_bytes, int *blocks);
-int dfcsr_wake_batched_launch(const void *dev_tasks, int k, int blocks, int precision,
-                              void *stream);
+                          double *const *x_kick, void *tasks, int64_t tasks_bytes,
+                          DfcsrWakeBatch *batch);
+int dfcsr_wake_batched_launch(const void *dev_tasks, const DfcsrWakeBatch *batch, int precision,
+                              void *partials, int64_t partials_bytes, void *stream);
 
 /* ---- a16: mesh -> particle kick --------------------------------------------
  * Replaces Beam.apply_wakes (beams.py:108-131): bilinear

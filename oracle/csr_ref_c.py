@@ -75,14 +75,14 @@ def hist_from_tracker(T, keep: _Keep):
                    a["dt"], a["dx"], a["dz"])
 
 
-def hist_from_records(rec: np.ndarray, head, nt, axes, keep: _Keep, field_stride=1):
-    """Ring of interleaved records rec[cap][Xi][Zi][R]; field f of cell
-    (x, z) at element f * field_stride of its record."""
+def hist_from_records(rec: np.ndarray, head, nt, axes, keep: _Keep):
+    """The device's float64 ring rec[Xi][Zi][cap][20] of corner-packed
+    records; field f of cell (x, z) is element 4 f of its record."""
     rec = np.ascontiguousarray(rec, dtype=np.float64)
     keep.refs.append(rec)
-    cap, Xi, Zi, R = rec.shape
-    base = (c_dp * 5)(*[rec.ctypes.data + 8 * f * field_stride for f in range(5)])
-    return RefHist(base, Xi * Zi * R, Zi * R, R, nt, Xi, Zi, cap, head, *axes)
+    Xi, Zi, cap, R = rec.shape
+    base = (c_dp * 5)(*[rec.ctypes.data + 8 * 4 * f for f in range(5)])
+    return RefHist(base, R, Zi * cap * R, cap * R, nt, Xi, Zi, cap, head, *axes)
 
 
 def lat_from_tables(lat, keep: _Keep):

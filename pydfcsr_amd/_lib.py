@@ -20,11 +20,13 @@ c_i32 = ctypes.c_int32
 c_i64 = ctypes.c_int64
 c_f64 = ctypes.c_double
 
+ABI_VERSION = 2   # DFCSR_ABI_VERSION
 DFCSR_PREC_FP64 = 0
 DFCSR_PREC_MIXED = 1
 MOMENTS_COUNT = 10
 HIST_REC64 = 20   # corner-packed records: field f, corner c at [4 f + c]
-HIST_REC32 = 32
+HIST_REC32 = 20
+HIST_CHUNKS32 = 5   # rings: FP64 (Xi, Zi, cap, 20); FP32 (Xi, cap, 5, Zi, 4)
 FIELDS = ("density", "density_x", "density_z", "vx", "vx_x")   # DFCSR_F_* order
 
 
@@ -47,6 +49,11 @@ class History(ctypes.Structure):
 class Lattice(ctypes.Structure):
     _fields_ = [("table", c_dp), ("ns", c_i32), ("nelem", c_i32), ("min_s", c_f64),
                 ("ds", c_f64), ("distance", c_dp), ("rho", c_dp)]
+
+
+class WakeBatch(ctypes.Structure):
+    _fields_ = [("tasks", c_i32), ("blocks_far", c_i32), ("blocks_near", c_i32),
+                ("observers", c_i32), ("partial_bytes", c_i64)]
 
 
 class WakeParams(ctypes.Structure):
@@ -75,16 +82,17 @@ _SIGNATURES = [
     ("dfcsr_savgol_operator", c_i32, [c_i32, c_i32, c_dp, c_dp]),
     ("dfcsr_regrid_workspace_bytes", c_i64, [c_i32]),
     ("dfcsr_regrid", c_i32, [c_dp, c_i64, ctypes.POINTER(RegridJob), c_i32, c_dp, c_dp, c_i32,
-                             c_i32, c_f64, c_f64, c_f64, c_f64, c_dp, c_i64, c_dp]),
+                             c_i32, c_i32, c_f64, c_f64, c_f64, c_f64, c_dp, c_i64, c_dp]),
     ("dfcsr_interpolate3d", c_i32, [ctypes.POINTER(History), c_i32, c_dp, c_i64, c_dp, c_dp]),
     ("dfcsr_interpolate1d", c_i32, [ctypes.POINTER(Lattice), c_i32, c_dp, c_i64, c_dp, c_dp]),
     ("dfcsr_csr_integrand", c_i32, [ctypes.POINTER(History), ctypes.POINTER(Lattice), c_f64,
                                     c_f64, c_f64, c_dp, c_i32, c_dp, c_i32, c_dp, c_dp, c_dp]),
-    ("dfcsr_wake_workspace_bytes", c_i64, [c_i32]),
+    ("dfcsr_wake_workspace_bytes", c_i64, [ctypes.POINTER(WakeParams)]),
     ("dfcsr_wake_mesh", c_i32, [ctypes.POINTER(History), ctypes.POINTER(Lattice),
                                 ctypes.POINTER(WakeParams), c_i32, c_dp, c_dp, c_dp, c_dp, c_dp,
                                 c_i64, c_dp]),
-    ("dfcsr_wake_batched_workspace_bytes", c_i64, [c_i32]),
+    ("dfcsr_wake_batched_workspace_bytes", c_i64, [c_i32, ctypes.POINTER(WakeParams)]),
+    ("dfcsr_wake_task_table_bytes", c_i64, [c_i32]),
     ("dfcsr_wake_mesh_batched", c_i32, [c_i32, ctypes.POINTER(History), ctypes.POINTER(Lattice),
                                         ctypes.POINTER(WakeParams), c_i32,
                                         ctypes.POINTER(c_dp), ctypes.POINTER(c_dp), c_dp, c_i64,
@@ -92,8 +100,9 @@ _SIGNATURES = [
     ("dfcsr_wake_tasks_pack", c_i32, [c_i32, ctypes.POINTER(History), ctypes.POINTER(Lattice),
                                       ctypes.POINTER(WakeParams), c_i32, ctypes.POINTER(c_dp),
                                       ctypes.POINTER(c_dp), c_dp, c_i64,
-                                      ctypes.POINTER(ctypes.c_int)]),
-    ("dfcsr_wake_batched_launch", c_i32, [c_dp, c_i32, c_i32, c_i32, c_dp]),
+                                      ctypes.POINTER(WakeBatch)]),
+    ("dfcsr_wake_batched_launch", c_i32, [c_dp, ctypes.POINTER(WakeBatch), c_i32, c_dp, c_i64,
+                                          c_dp]),
     ("dfcsr_apply_wakes", c_i32, [c_dp, c_dp, c_dp, c_dp, c_i64, c_dp, c_dp, c_i32, c_i32, c_f64,
                                   c_f64, c_f64, c_f64, c_f64, c_f64, c_f64, c_f64, c_i32, c_dp]),
 ]
@@ -121,7 +130,7 @@ def load(path: str | None = None):
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    if lib.dfcsr_abi_version() != 1:
+    if lib.dfcsr_abi_version() != ABI_VERSION:
         raise DfcsrError(f"ABI version mismatch: {lib.dfcsr_abi_version()}")
     if _lib is None:
         _lib = lib

@@ -69,10 +69,15 @@ __host__ __device__ inline HistAxes make_axes(const DfcsrHistory &H) {
   return a;
 }
 
-__device__ inline const double *hist_rec(const DfcsrHistory &H, int ti, int xi, int zi) {
+// Ring slot of logical slice ti.
+__device__ inline int hist_slot(const DfcsrHistory &H, int ti) {
   int slot = H.head + ti;
-  if (slot >= H.cap) slot -= H.cap;
-  return H.data + (((int64_t)slot * H.Xi + xi) * H.Zi + zi) * DFCSR_HIST_REC64;
+  return slot >= H.cap ? slot - H.cap : slot;
+}
+
+// FP64 record (x, z, slot of ti): [x][z][slot][20] layout.
+__device__ inline const double *hist_rec(const DfcsrHistory &H, int ti, int xi, int zi) {
+  return H.data + (((int64_t)xi * H.Zi + zi) * H.cap + hist_slot(H, ti)) * DFCSR_HIST_REC64;
 }
 
 // Trilinear lookup of all five fields at (t, x, z) with interpolate3D's

@@ -9,11 +9,11 @@
 //   v00(1-y0)(1-y1) + v01(1-y0)y1 + v10 y0(1-y1) + v11 y0 y1,
 // with fill 0 outside the grid (mean(vx_x) of the slice for vx_x).
 //
-// History layout: corner-packed records [slot][Xi][Zi][20] (float64) and
-// [slot][Xi][Zi][32] (float32 shadow): record (x, z) holds field f at
-// (x, z), (x, z1), (x1, z), (x1, z1) in elements f*4 + 0..3 (x1, z1 clamped
-// to the last plane), so a trilinear lookup reads one record per time
-// plane.
+// History layout: corner-packed records (record (x, z) holds field f at
+// (x, z), (x, z1), (x1, z), (x1, z1) in elements f*4 + 0..3, x1 and z1
+// clamped to the last plane).  float64: whole records [x][z][slot][20];
+// float32 shadow: 16-byte chunks (one field's four corners each) stored
+// [x][slot][f][z], where threads of consecutive z write consecutive chunks.
 #include "common.h"
 
 namespace dfcsr {
@@ -69,7 +69,7 @@ __device__ inline void rgi5(const DfcsrRegridJob &J, const double *__restrict__ 
 __global__ __launch_bounds__(kThreads) void regrid_kernel(
     const double *__restrict__ src, int64_t slot_stride, const DfcsrRegridJob *__restrict__ jobs,
     const double *__restrict__ means, double *__restrict__ h64, float *__restrict__ h32, int Xi,
-    int Zi, double xi0, double xi1, double zi0, double zi1) {
+    int Zi, int cap, double xi0, double xi1, double zi0, double zi1) {
   const DfcsrRegridJob J = jobs[blockIdx.y];
   const int64_t cells = (int64_t)Xi * Zi;
   const double *base = src + (int64_t)J.src_slot * slot_stride;
@@ -85,18 +85,19 @@ __global__ __launch_bounds__(kThreads) void regrid_kernel(
     rgi5(J, base, fill_vxx, x0, z1, v[1]);
     rgi5(J, base, fill_vxx, x1, z0, v[2]);
     rgi5(J, base, fill_vxx, x1, z1, v[3]);
-    double2 *d2 = reinterpret_cast<double2 *>(h64 + ((int64_t)J.dst_slot * cells + c) * DFCSR_HIST_REC64);
+    double2 *d2 = reinterpret_cast<double2 *>(h64 + (c * cap + J.dst_slot) * DFCSR_HIST_REC64);
 #pragma unroll
     for (int f = 0; f < 5; ++f) {
       d2[2 * f] = make_double2(v[0][f], v[1][f]);
       d2[2 * f + 1] = make_double2(v[2][f], v[3][f]);
     }
     if (h32) {
-      float4 *s4 = reinterpret_cast<float4 *>(h32 + ((int64_t)J.dst_slot * cells + c) * DFCSR_HIST_REC32);
+      const int64_t blk = (int64_t)ix * cap + J.dst_slot;
+      float4 *s4 = reinterpret_cast<float4 *>(h32) + blk * DFCSR_HIST_CHUNKS32 * Zi + iz;
 #pragma unroll
       for (int f = 0; f < 5; ++f)
-        s4[f] = make_float4((float)v[0][f], (float)v[1][f], (float)v[2][f], (float)v[3][f]);
-      s4[5] = s4[6] = s4[7] = make_float4(0.f, 0.f, 0.f, 0.f);
+        s4[f * (int64_t)Zi] = make_float4((float)v[0][f], (float)v[1][f], (float)v[2][f],
+                                          (float)v[3][f]);
     }
   }
 }
@@ -112,19 +113,20 @@ extern "C" int64_t dfcsr_regrid_workspace_bytes(int njobs) {
 }
 
 extern "C" int dfcsr_regrid(const double *src, int64_t src_slot_stride, const DfcsrRegridJob *jobs,
-                            int njobs, double *hist64, float *hist32, int Xi, int Zi, double xi0,
-                            double xi1, double zi0, double zi1, void *ws, int64_t ws_bytes,
-                            void *stream) {
+                            int njobs, double *hist64, float *hist32, int Xi, int Zi, int cap,
+                            double xi0, double xi1, double zi0, double zi1, void *ws,
+                            int64_t ws_bytes, void *stream) {
   if (!src || !jobs || !hist64 || !ws) return set_error(DFCSR_EINVAL, "regrid: NULL pointer");
   if (njobs < 1) return DFCSR_OK;
   if (njobs > kMaxJobs) return set_error(DFCSR_ERANGE, "regrid: %d jobs > %d", njobs, kMaxJobs);
   if (Xi < 2 || Zi < 2) return set_error(DFCSR_EINVAL, "regrid: interp grid %d x %d", Xi, Zi);
+  if (cap < 1) return set_error(DFCSR_EINVAL, "regrid: ring capacity %d", cap);
   if (ws_bytes < dfcsr_regrid_workspace_bytes(njobs))
     return set_error(DFCSR_ENOWS, "regrid: workspace too small");
   for (int k = 0; k < njobs; ++k) {
     const DfcsrRegridJob &J = jobs[k];
     if (J.nbx < 1 || J.nbz < 1 || (int64_t)5 * J.nbx * J.nbz > src_slot_stride || J.src_slot < 0 ||
-        J.dst_slot < 0)
+        J.dst_slot < 0 || J.dst_slot >= cap)
       return set_error(DFCSR_EINVAL, "regrid: bad job %d (nbx %d nbz %d src %d dst %d)", k, J.nbx,
                        J.nbz, J.src_slot, J.dst_slot);
   }
@@ -143,7 +145,7 @@ extern "C" int dfcsr_regrid(const double *src, int64_t src_slot_stride, const Df
   if (gx > 2048) gx = 2048;
   hipLaunchKernelGGL(regrid_kernel, dim3(gx, njobs), dim3(kThreads), 0, s, src, src_slot_stride,
                      (const DfcsrRegridJob *)djobs, (const double *)means, hist64, hist32, Xi, Zi,
-                     xi0, xi1, zi0, zi1);
+                     cap, xi0, xi1, zi0, zi1);
   DFCSR_LAUNCH_CHECK("regrid");
   return DFCSR_OK;
 }

@@ -1,21 +1,28 @@
 // a13-a15: the retarded CSR integral on the wake mesh.
 //
 // Replaces CSR2D.get_CSR_mesh / calculate_2D_CSR / get_CSR_wake /
-// get_CSR_integrand (CSR.py:361-782).  One workgroup integrates one mesh
-// point over all of its regions (3 without chirp, 4 with):
-//   work item  = (region, row chunk, s' column k); lanes of a wavefront take
-//                consecutive columns, so they share x' rows (the x-range test
-//                is wave-uniform) and their retarded lookups land on the same
-//                history rows (z' = s' - t_ret barely moves along s').
+// get_CSR_integrand (CSR.py:361-782).  A mesh point's integral over its
+// regions (3 without chirp, 4 with) is split into phases: one phase is one
+// region and one chunk of its x' rows, over all s' columns.
+//   observer   : lattice lookups at s, vx at the observer and the region
+//                table, once per mesh point (wake_observer_kernel).
+//   workgroup  = one (mesh point, phase); lanes of a wavefront take
+//                consecutive s' columns, so they share x' rows (the x-range
+//                test is wave-uniform) and their retarded lookups are
+//                neighbouring records of the time-innermost history.
 //   per column : lattice lookups at s', rho(s'), the trapezoid weight in s'.
 //   per point  : |r - r'|, t_ret, the trilinear gather of five fields, the
 //                integrand; points whose gather is outside the history add 0
 //                and skip the arithmetic.
-//   reduction  : sum_j,k wx_j wz_k I_jk per lane, then a fixed-order block
-//                tree: deterministic, the same every launch.
-// Workgroups are mapped so each XCD integrates a contiguous range of mesh
-// points (neighbouring observers read the same history lines through that
-// XCD's L2).
+//   reduction  : sum_j,k wx_j wz_k I_jk per lane, a fixed-order block tree
+//                per phase, then the phases of a mesh point in phase order:
+//                deterministic, the same every launch.
+// Launch order: region 0 (the far region) runs first, and the workgroups of
+// one XCD take the same phase for the mesh points of one z line (same x'
+// rows, the same retarded planes and nearly the same z'); the other regions
+// follow with the mesh points of one x line together (same x' rows where
+// the region follows x, overlapping z' sweeps).  Neighbouring workgroups thus
+// read the same history records through their XCD's L2.
 #include "csr_point.h"
 
 namespace dfcsr {
@@ -26,17 +33,19 @@ int check_lattice(const DfcsrLattice *L, const char *who);
 namespace {
 
 constexpr int kThreads = 256;
-#ifndef DFCSR_WAKE_CHUNK
-#define DFCSR_WAKE_CHUNK 64
+#ifndef DFCSR_WAKE_CHUNK_FAR
+#define DFCSR_WAKE_CHUNK_FAR 64
+#endif
+#ifndef DFCSR_WAKE_CHUNK_NEAR
+#define DFCSR_WAKE_CHUNK_NEAR 64
 #endif
 #ifndef DFCSR_WAKE_MINBLOCKS
 #define DFCSR_WAKE_MINBLOCKS 2
 #endif
-#ifndef DFCSR_WAKE_ORDER
-#define DFCSR_WAKE_ORDER 0
-#endif
 
-constexpr int kChunk = DFCSR_WAKE_CHUNK;   // x' rows per work item
+// at most this many x' rows per phase, in region 0 / in the later regions
+constexpr int kChunkFar = DFCSR_WAKE_CHUNK_FAR;
+constexpr int kChunkNear = DFCSR_WAKE_CHUNK_NEAR;
 constexpr int kNumXcd = 8;
 
 struct LinAxis {
@@ -147,66 +156,6 @@ __device__ inline int xcd_remap(int b, int nb) {
   return xcd * q + (xcd < r ? xcd : r) + pos;
 }
 
-// Region table of one mesh point, shared by the workgroup through LDS
-// (dynamic indexing of a private array would put it in scratch memory).
-struct RegionLds {
-  Regions rg;
-  int off[5];
-};
-
-// Integrates mesh point i; every thread returns the block sums.
-__device__ void wake_point_fp64(const DfcsrHistory &H, const DfcsrLattice &L,
-                                const DfcsrWakeParams &P, int i, double *red, RegionLds &R,
-                                double &out_z, double &out_x, double &xm_out, double &zm_out) {
-  double xm, zm;
-  mesh_point(P, i, xm, zm);
-  xm_out = xm;
-  zm_out = zm;
-  const double t = P.t;
-  const double s = t + zm, x = xm;
-  const HistAxes A = make_axes(H);
-  const double inv_ds = 1.0 / L.ds;
-  const Observer o = make_observer<true>(H, A, L, s, x, t);
-  if (threadIdx.x == 0) {
-    make_regions(P, s, x, t, R.rg);
-    R.off[0] = 0;
-    for (int r = 0; r < R.rg.n; ++r)
-      R.off[r + 1] = R.off[r] + ((R.rg.xa[r].n + kChunk - 1) / kChunk) * R.rg.sa[r].n;
-  }
-  __syncthreads();
-  const int nreg = R.rg.n;
-  const int total = R.off[nreg];
-  double acc_z = 0.0, acc_x = 0.0;
-  for (int it = threadIdx.x; it < total; it += blockDim.x) {
-    int r = 0;
-    while (r + 1 < nreg && it >= R.off[r + 1]) ++r;
-    const int rem = it - R.off[r];
-    const LinAxis sa = R.rg.sa[r];
-    const LinAxis xa = R.rg.xa[r];
-    const int c = rem / sa.n, k = rem % sa.n;
-    const double sp = sa.at(k);
-    const double wz = sa.w(k);
-    const Column col = make_column<true>(L, o, sp, inv_ds);
-    const int j0 = c * kChunk;
-    const int j1 = min(j0 + kChunk, xa.n);
-    double cz = 0.0, cx = 0.0;
-    for (int j = j0; j < j1; ++j) {
-      const double xp = xa.at(j);
-      const double ux = (xp - A.min_x) * A.ix;
-      if (!(ux > -1.0 && ux < (double)H.Xi)) continue;   // wave-uniform: rows are shared
-      double Iz, Ix;
-      if (!csr_point<true>(H, A, o, col, xp, Iz, Ix)) continue;
-      const double wx = xa.w(j);
-      cz += wx * Iz;
-      cx += wx * Ix;
-    }
-    acc_z += wz * cz;
-    acc_x += wz * cx;
-  }
-  out_z = block_sum(acc_z, red);
-  out_x = block_sum(acc_x, red);
-}
-
 // ---- mixed precision --------------------------------------------------------
 //
 // Same integral, regrouped so that FP32 only sees well-conditioned terms.
@@ -221,10 +170,10 @@ __device__ void wake_point_fp64(const DfcsrHistory &H, const DfcsrLattice &L,
 // and q = (r-r').(n-n')/R - n.tau' (whose two terms cancel to leading order)
 // are float64; the gathered fields, the coefficients' products and the
 // integrand are float32.  1/R comes from an FP32 reciprocal square root with
-// one FP64 Newton step.  Each 32-row chunk is summed in FP32 with Neumaier's
-// compensated summation (Neumaier 1974, the Kahan-Babuska improvement); chunk
-// sums are weighted and accumulated in float64, and the block reduction is
-// float64.
+// one FP64 Newton step.  The rows of one column within a phase are summed
+// in FP32 with Neumaier's compensated summation (Neumaier 1974, the
+// Kahan-Babuska improvement); those sums are weighted and accumulated in
+// float64, and the block and phase reductions are float64.
 struct ColumnMixed {
   double sp, ax, ay, npx, npy, P0, P1, p2;
   float rho, D0, D1, D2, G0, G1, G2, e, a, b, A, B;
@@ -263,10 +212,11 @@ __device__ inline ColumnMixed make_column_mixed(const DfcsrLattice &L, const Obs
   return m;
 }
 
-__device__ inline const float *hist_rec32(const DfcsrHistory &H, int ti, int xi, int zi) {
-  int slot = H.head + ti;
-  if (slot >= H.cap) slot -= H.cap;
-  return H.data32 + (((int64_t)slot * H.Xi + xi) * H.Zi + zi) * DFCSR_HIST_REC32;
+// Chunk 0 (field 0's four corners) of the FP32 record (x, slot of ti, z);
+// field f's chunk is f * Zi float4 further on.
+__device__ inline const float4 *hist_rec32(const DfcsrHistory &H, int ti, int xi, int zi) {
+  const int64_t blk = ((int64_t)xi * H.cap + hist_slot(H, ti)) * DFCSR_HIST_CHUNKS32;
+  return reinterpret_cast<const float4 *>(H.data32) + blk * H.Zi + zi;
 }
 
 struct Neumaier {
@@ -295,13 +245,14 @@ __device__ inline bool csr_point_mixed(const DfcsrHistory &H, const HistAxes &A,
   const TruncIdx T = trunc_index(ut, H.nt), Z = trunc_index(uz, H.Zi);
   if (!(T.ok && Z.ok)) return false;
   const float td = (float)T.frac, zd = (float)Z.frac;
-  // one corner-packed record (one cache line) per time plane
-  const float4 *q0 = reinterpret_cast<const float4 *>(hist_rec32(H, T.i0, xi0, Z.i0));
-  const float4 *q1 = reinterpret_cast<const float4 *>(hist_rec32(H, T.i1, xi0, Z.i0));
+  // one corner-packed record per time plane, one 16-byte chunk per field
+  const float4 *q0 = hist_rec32(H, T.i0, xi0, Z.i0);
+  const float4 *q1 = hist_rec32(H, T.i1, xi0, Z.i0);
+  const int64_t cs = H.Zi;
   float f[5];
 #pragma unroll
   for (int k = 0; k < 5; ++k) {
-    const float4 a = q0[k], b = q1[k];   // (x0,z0), (x0,z1), (x1,z0), (x1,z1)
+    const float4 a = q0[k * cs], b = q1[k * cs];   // (x0,z0), (x0,z1), (x1,z0), (x1,z1)
     const float c00 = fmaf(td, b.x - a.x, a.x);
     const float c01 = fmaf(td, b.y - a.y, a.y);
     const float c10 = fmaf(td, b.z - a.z, a.z);
@@ -330,141 +281,301 @@ __device__ inline bool csr_point_mixed(const DfcsrHistory &H, const HistAxes &A,
   return true;
 }
 
-__device__ void wake_point_mixed(const DfcsrHistory &H, const DfcsrLattice &L,
-                                 const DfcsrWakeParams &P, int i, double *red, RegionLds &R,
-                                 double &out_z, double &out_x, double &xm_out, double &zm_out) {
+// ---- phase decomposition ----------------------------------------------------
+//
+// Region 0 has 2 Nx rows, the others Nx (make_regions); each region's rows
+// are split into ceil(rows / kChunk*) chunks of equal length.  Phase order of
+// a mesh point: the chunks of region 0, then those of regions 1, 2 (, 3).
+struct PhasePlan {
+  int nreg;                 // 3 without chirp, 4 with
+  int rows0, ch0, per0;     // region 0: rows, chunks, rows per chunk
+  int rows1, ch1, per1;     // each later region
+  int nA, nB;               // phases of region 0 / of the later regions
+};
+
+__host__ __device__ inline PhasePlan make_plan(const DfcsrWakeParams &P) {
+  PhasePlan pl;
+  pl.nreg = (fabs(P.slope0) <= 1) ? 3 : 4;   // the branch test of make_regions
+  pl.rows0 = 2 * P.Nx;
+  pl.ch0 = (pl.rows0 + kChunkFar - 1) / kChunkFar;
+  pl.per0 = (pl.rows0 + pl.ch0 - 1) / pl.ch0;
+  pl.rows1 = P.Nx;
+  pl.ch1 = (pl.rows1 + kChunkNear - 1) / kChunkNear;
+  pl.per1 = (pl.rows1 + pl.ch1 - 1) / pl.ch1;
+  pl.nA = pl.ch0;
+  pl.nB = (pl.nreg - 1) * pl.ch1;
+  return pl;
+}
+
+// Region and row range [j0, j1) of phase ph.
+__host__ __device__ inline void phase_rows(const PhasePlan &pl, int ph, int &r, int &j0, int &j1) {
+  if (ph < pl.ch0) {
+    r = 0;
+    j0 = ph * pl.per0;
+    j1 = min(j0 + pl.per0, pl.rows0);
+  } else {
+    const int q = ph - pl.ch0;
+    r = 1 + q / pl.ch1;
+    j0 = (q % pl.ch1) * pl.per1;
+    j1 = min(j0 + pl.per1, pl.rows1);
+  }
+}
+
+// One simulation's wake: descriptors, outputs and its place in a launch.
+struct WakeTask {
+  DfcsrHistory H;
+  DfcsrLattice L;
+  DfcsrWakeParams P;
+  double *dE, *xk;        // outputs [M]
+  double *xmesh, *zmesh;  // optional outputs [M] (NULL: not written)
+  int64_t part;           // first of its M * (nA + nB) * 2 partial sums
+  int64_t obs;            // first of its M * kObsStride observer values
+  int M, nA, nB;
+  int blockA0, blockB0, obs0;   // first workgroup / observer in a batched launch
+};
+
+// Observer data of one mesh point, computed once by wake_observer_kernel
+// and read by every phase of that point (kObsStride doubles):
+//   [0] s, [1] x, [2..7] lattice X0, Y0, n_x, n_y, tau_x, tau_y at s,
+//   [8] vx at the observer, [16 + 4 r] x' axis of region r (a, b, step, n),
+//   [32 + 4 r] s' axis of region r.
+constexpr int kObsStride = 48;
+
+__device__ inline void store_axis(double *o, const LinAxis &a) {
+  o[0] = a.a;
+  o[1] = a.b;
+  o[2] = a.step;
+  o[3] = (double)a.n;
+}
+
+__device__ inline LinAxis load_axis(const double *o) {
+  LinAxis a;
+  a.a = o[0];
+  a.b = o[1];
+  a.step = o[2];
+  a.n = (int)o[3];
+  return a;
+}
+
+__device__ inline void observer_point(const WakeTask &T, int i, double *__restrict__ od) {
   double xm, zm;
-  mesh_point(P, i, xm, zm);
-  xm_out = xm;
-  zm_out = zm;
-  const double t = P.t;
+  mesh_point(T.P, i, xm, zm);
+  const double t = T.P.t;
   const double s = t + zm, x = xm;
+  const HistAxes A = make_axes(T.H);
+  const Observer o = make_observer<true>(T.H, A, T.L, s, x, t);
+  Regions rg = {};
+  make_regions(T.P, s, x, t, rg);
+  od[0] = s;
+  od[1] = x;
+  od[2] = o.L.X0;
+  od[3] = o.L.Y0;
+  od[4] = o.L.nx;
+  od[5] = o.L.ny;
+  od[6] = o.L.tx;
+  od[7] = o.L.ty;
+  od[8] = o.vx;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    store_axis(od + 16 + 4 * r, rg.xa[r]);
+    store_axis(od + 32 + 4 * r, rg.sa[r]);
+  }
+}
+
+// Integrates phase ph of the mesh point whose observer data is od; every
+// thread returns the block sums of wz_k wx_j (I_z, I_x) over the phase.
+template <bool kMixed>
+__device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int ph, double *red,
+                           double &out_z, double &out_x) {
+  const DfcsrHistory &H = T.H;
+  const DfcsrLattice &L = T.L;
+  const double t = T.P.t;
   const HistAxes A = make_axes(H);
   const double inv_ds = 1.0 / L.ds;
-  const Observer o = make_observer<true>(H, A, L, s, x, t);
-  if (threadIdx.x == 0) {
-    make_regions(P, s, x, t, R.rg);
-    R.off[0] = 0;
-    for (int r = 0; r < R.rg.n; ++r)
-      R.off[r + 1] = R.off[r] + ((R.rg.xa[r].n + kChunk - 1) / kChunk) * R.rg.sa[r].n;
+  int r, j0, j1;
+  phase_rows(make_plan(T.P), ph, r, j0, j1);
+  const LinAxis xa = load_axis(od + 16 + 4 * r);
+  const LinAxis sa = load_axis(od + 32 + 4 * r);
+  // hull of the rows whose x' lies inside the history (x' is monotone in j);
+  // a phase without one adds exactly 0 and skips its column setup
+  int v0 = j1, v1 = j0;
+  for (int j = j0; j < j1; ++j) {
+    const double ux = (xa.at(j) - A.min_x) * A.ix;
+    if (ux > -1.0 && ux < (double)H.Xi) {
+      v0 = min(v0, j);
+      v1 = j + 1;
+    }
   }
-  __syncthreads();
-  const int nreg = R.rg.n;
-  const int total = R.off[nreg];
+  if (v0 >= v1) {   // uniform over the workgroup
+    out_z = out_x = 0.0;
+    return;
+  }
+  Observer o;
+  o.s = od[0];
+  o.x = od[1];
+  o.t = t;
+  o.L.X0 = od[2];
+  o.L.Y0 = od[3];
+  o.L.nx = od[4];
+  o.L.ny = od[5];
+  o.L.tx = od[6];
+  o.L.ty = od[7];
+  o.vx = od[8];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  const int ngroup = (sa.n + 63) >> 6;
   double acc_z = 0.0, acc_x = 0.0;
-  for (int it = threadIdx.x; it < total; it += blockDim.x) {
-    int r = 0;
-    while (r + 1 < nreg && it >= R.off[r + 1]) ++r;
-    const int rem = it - R.off[r];
-    const LinAxis sa = R.rg.sa[r];
-    const LinAxis xa = R.rg.xa[r];
-    const int c = rem / sa.n, k = rem % sa.n;
+  for (int g = wave; g < ngroup; g += nwave) {
+    // a group of m < 64 columns (the last one) spreads its rows over 64 / m
+    // lane sets, so the wave still does a full wave's work
+    const int m = min(64, sa.n - 64 * g);
+    const int nq = 64 / m;
+    const int q = lane / m;
+    if (q >= nq) continue;
+    const int k = 64 * g + (lane - q * m);
     const double sp = sa.at(k);
     const double wz = sa.w(k);
-    const ColumnMixed col = make_column_mixed(L, o, sp, inv_ds);
-    const int j0 = c * kChunk;
-    const int j1 = min(j0 + kChunk, xa.n);
-    Neumaier cz, cx;
-    for (int j = j0; j < j1; ++j) {
-      const double xp = xa.at(j);
-      const TruncIdx X = trunc_index((xp - A.min_x) * A.ix, H.Xi);
-      if (!X.ok) continue;   // wave-uniform: rows are shared
-      float Iz, Ix;
-      if (!csr_point_mixed(H, A, t, col, xp, X.i0, (float)X.frac, Iz, Ix)) continue;
-      const float wx = (float)xa.w(j);
-      cz.add(wx * Iz);
-      cx.add(wx * Ix);
+    if (kMixed) {
+      const ColumnMixed col = make_column_mixed(L, o, sp, inv_ds);
+      Neumaier cz, cx;
+      for (int j = v0 + q; j < v1; j += nq) {
+        const double xp = xa.at(j);
+        const TruncIdx X = trunc_index((xp - A.min_x) * A.ix, H.Xi);
+        if (!X.ok) continue;
+        float Iz, Ix;
+        if (!csr_point_mixed(H, A, t, col, xp, X.i0, (float)X.frac, Iz, Ix)) continue;
+        const float wx = (float)xa.w(j);
+        cz.add(wx * Iz);
+        cx.add(wx * Ix);
+      }
+      acc_z += wz * (double)cz.value();
+      acc_x += wz * (double)cx.value();
+    } else {
+      const Column col = make_column<true>(L, o, sp, inv_ds);
+      double cz = 0.0, cx = 0.0;
+      for (int j = v0 + q; j < v1; j += nq) {
+        const double xp = xa.at(j);
+        const double ux = (xp - A.min_x) * A.ix;
+        if (!(ux > -1.0 && ux < (double)H.Xi)) continue;
+        double Iz, Ix;
+        if (!csr_point<true>(H, A, o, col, xp, Iz, Ix)) continue;
+        const double wx = xa.w(j);
+        cz += wx * Iz;
+        cx += wx * Ix;
+      }
+      acc_z += wz * cz;
+      acc_x += wz * cx;
     }
-    acc_z += wz * (double)cz.value();
-    acc_x += wz * (double)cx.value();
   }
   out_z = block_sum(acc_z, red);
   out_x = block_sum(acc_x, red);
 }
 
-// Mesh point of block b.  Blocks of one XCD integrate a contiguous range of
-// mesh points (row-major: neighbouring z at one x, whose retarded lookups
-// overlap), so they share that XCD's L2.  DFCSR_WAKE_ORDER=1 instead runs
-// over x first (measured slower on the chicane workload).
-__device__ inline int mesh_of_block(const DfcsrWakeParams &P, int b, int nb) {
-  const int l = xcd_remap(b, nb);
-  if (DFCSR_WAKE_ORDER == 0 || P.observers) return l;
-  const int ix = l % P.mesh_xbins, zt = l / P.mesh_xbins;
-  int iz = zt;
-  if (DFCSR_WAKE_ORDER == 2 && P.mesh_zbins % kNumXcd == 0) {
-    // XCD k owns z-lines k, k+8, ...: the same z at all x runs together on
-    // one XCD while every XCD gets observers from the whole bunch
-    const int q = P.mesh_zbins / kNumXcd;
-    iz = (zt % q) * kNumXcd + zt / q;
+// Workgroup l of a task's far (kNear = false) or near launch -> (mesh
+// point, phase); writes the phase's partial sums.
+template <bool kMixed, bool kNear>
+__device__ inline void run_phase_block(const WakeTask &T, int l, double *__restrict__ partials,
+                                       double *red) {
+  const int xb = T.P.mesh_xbins, zb = T.P.mesh_zbins;
+  int i, ph;
+  if (!kNear) {   // z line major: the mesh points of one z line, phase by phase
+    const int per_line = T.nA * xb;
+    const int zl = l / per_line, rem = l - zl * per_line;
+    ph = rem / xb;
+    i = (rem - ph * xb) * zb + zl;
+  } else {        // x line major
+    const int per_line = T.nB * zb;
+    const int xl = l / per_line, rem = l - xl * per_line;
+    const int q = rem / zb;
+    i = xl * zb + (rem - q * zb);
+    ph = T.nA + q;
   }
-  return ix * P.mesh_zbins + iz;
-}
-
-__global__ __launch_bounds__(kThreads, DFCSR_WAKE_MINBLOCKS) void wake_fp64_kernel(DfcsrHistory H, DfcsrLattice L,
-                                                             DfcsrWakeParams P,
-                                                             double *__restrict__ xmesh,
-                                                             double *__restrict__ zmesh,
-                                                             double *__restrict__ dE,
-                                                             double *__restrict__ xk) {
-  __shared__ double red[kThreads / 64];
-  const int i = mesh_of_block(P, blockIdx.x, gridDim.x);
-  __shared__ RegionLds R;
-  double sz, sx, xm, zm;
-  wake_point_fp64(H, L, P, i, red, R, sz, sx, xm, zm);
+  double sz, sx;
+  wake_phase<kMixed>(T, partials + T.obs + (int64_t)i * kObsStride, ph, red, sz, sx);
   if (threadIdx.x == 0) {
-    dE[i] = -P.csr_scaling * sz;
-    xk[i] = P.csr_scaling * sx;
-    if (xmesh) xmesh[i] = xm;
-    if (zmesh) zmesh[i] = zm;
+    double *p = partials + T.part + ((int64_t)i * (T.nA + T.nB) + ph) * 2;
+    p[0] = sz;
+    p[1] = sx;
   }
 }
 
-__global__ __launch_bounds__(kThreads, DFCSR_WAKE_MINBLOCKS) void wake_mixed_kernel(
-    DfcsrHistory H, DfcsrLattice L, DfcsrWakeParams P, double *__restrict__ xmesh,
-    double *__restrict__ zmesh, double *__restrict__ dE, double *__restrict__ xk) {
+template <bool kMixed, bool kNear>
+__global__ __launch_bounds__(kThreads, DFCSR_WAKE_MINBLOCKS) void wake_phase_kernel(
+    WakeTask T, double *__restrict__ partials) {
   __shared__ double red[kThreads / 64];
-  const int i = mesh_of_block(P, blockIdx.x, gridDim.x);
-  __shared__ RegionLds R;
-  double sz, sx, xm, zm;
-  wake_point_mixed(H, L, P, i, red, R, sz, sx, xm, zm);
-  if (threadIdx.x == 0) {
-    dE[i] = -P.csr_scaling * sz;
-    xk[i] = P.csr_scaling * sx;
-    if (xmesh) xmesh[i] = xm;
-    if (zmesh) zmesh[i] = zm;
-  }
+  run_phase_block<kMixed, kNear>(T, xcd_remap(blockIdx.x, gridDim.x), partials, red);
 }
 
-struct WakeTask {
-  DfcsrHistory H;
-  DfcsrLattice L;
-  DfcsrWakeParams P;
-  double *dE, *xk;
-  int block0;   // first block of this task in the flattened launch
-  int M;
-};
-
-template <bool kMixed>
-__global__ __launch_bounds__(kThreads, DFCSR_WAKE_MINBLOCKS) void wake_batched_kernel(
-    const WakeTask *__restrict__ tasks, int ntask) {
-  __shared__ double red[kThreads / 64];
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
+// Last task whose first index (workgroup or observer) is <= b.
+template <typename F>
+__device__ inline int find_task(const WakeTask *__restrict__ tasks, int ntask, int b, F first) {
   int lo = 0, hi = ntask - 1;
-  while (lo < hi) {   // last task with block0 <= b
-    int mid = (lo + hi + 1) >> 1;
-    if (tasks[mid].block0 <= b) lo = mid;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (first(tasks[mid]) <= b) lo = mid;
     else hi = mid - 1;
   }
-  const WakeTask &T = tasks[lo];
-  const int i = b - T.block0;
-  __shared__ RegionLds R;
-  double sz, sx, xm, zm;
-  if (kMixed) wake_point_mixed(T.H, T.L, T.P, i, red, R, sz, sx, xm, zm);
-  else wake_point_fp64(T.H, T.L, T.P, i, red, R, sz, sx, xm, zm);
-  if (threadIdx.x == 0) {
-    T.dE[i] = -T.P.csr_scaling * sz;
-    T.xk[i] = T.P.csr_scaling * sx;
+  return lo;
+}
+
+template <bool kMixed, bool kNear>
+__global__ __launch_bounds__(kThreads, DFCSR_WAKE_MINBLOCKS) void wake_phase_batched_kernel(
+    const WakeTask *__restrict__ tasks, int ntask, double *__restrict__ partials) {
+  __shared__ double red[kThreads / 64];
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int j = find_task(tasks, ntask, b,
+                          [](const WakeTask &T) { return kNear ? T.blockB0 : T.blockA0; });
+  const WakeTask &T = tasks[j];
+  run_phase_block<kMixed, kNear>(T, b - (kNear ? T.blockB0 : T.blockA0), partials, red);
+}
+
+__global__ __launch_bounds__(kThreads) void wake_observer_kernel(WakeTask T,
+                                                                 double *__restrict__ partials) {
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  if (i < T.M) observer_point(T, i, partials + T.obs + (int64_t)i * kObsStride);
+}
+
+__global__ __launch_bounds__(kThreads) void wake_observer_batched_kernel(
+    const WakeTask *__restrict__ tasks, int ntask, int nobs, double *__restrict__ partials) {
+  const int g = blockIdx.x * kThreads + threadIdx.x;
+  if (g >= nobs) return;
+  const int j = find_task(tasks, ntask, g, [](const WakeTask &T) { return T.obs0; });
+  const WakeTask &T = tasks[j];
+  const int i = g - T.obs0;
+  observer_point(T, i, partials + T.obs + (int64_t)i * kObsStride);
+}
+
+// Sum of the phases of mesh point i in phase order; the reference's signs
+// and scaling (CSR.py:599-602).
+__device__ inline void finalize_point(const WakeTask &T, int i, const double *__restrict__ partials) {
+  const int nph = T.nA + T.nB;
+  const double *p = partials + T.part + (int64_t)i * nph * 2;
+  double sz = 0.0, sx = 0.0;
+  for (int ph = 0; ph < nph; ++ph) {
+    sz += p[2 * ph];
+    sx += p[2 * ph + 1];
   }
+  T.dE[i] = -T.P.csr_scaling * sz;
+  T.xk[i] = T.P.csr_scaling * sx;
+  if (T.xmesh) {
+    double xm, zm;
+    mesh_point(T.P, i, xm, zm);
+    T.xmesh[i] = xm;
+    T.zmesh[i] = zm;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void wake_finalize_kernel(WakeTask T,
+                                                                 const double *__restrict__ partials) {
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  if (i < T.M) finalize_point(T, i, partials);
+}
+
+__global__ __launch_bounds__(kThreads) void wake_finalize_batched_kernel(
+    const WakeTask *__restrict__ tasks, int ntask, int nobs, const double *__restrict__ partials) {
+  const int g = blockIdx.x * kThreads + threadIdx.x;
+  if (g >= nobs) return;
+  const int j = find_task(tasks, ntask, g, [](const WakeTask &T) { return T.obs0; });
+  finalize_point(tasks[j], g - tasks[j].obs0, partials);
 }
 
 int check_params(const DfcsrWakeParams *p, const char *who) {
@@ -478,14 +589,55 @@ int check_params(const DfcsrWakeParams *p, const char *who) {
   return DFCSR_OK;
 }
 
+// check_params without setting the error message (for the size queries).
+bool params_ok(const DfcsrWakeParams &p) {
+  return p.Nx >= 1 && p.Nz >= 1 && p.mesh_xbins >= 1 && p.mesh_zbins >= 1 &&
+         (int64_t)p.mesh_xbins * p.mesh_zbins <= ((int64_t)1 << 24);
+}
+
+// Partial sums and observer data of one simulation, in doubles.
+int64_t partial_doubles(const DfcsrWakeParams &p) {
+  const PhasePlan pl = make_plan(p);
+  return (int64_t)p.mesh_xbins * p.mesh_zbins * ((pl.nA + pl.nB) * 2 + kObsStride);
+}
+
+// Fills a task; rejects launches whose workgroup counts overflow int.
+int make_task(const DfcsrHistory &H, const DfcsrLattice &L, const DfcsrWakeParams &P,
+              double *dE, double *xk, double *xmesh, double *zmesh, WakeTask &T, const char *who) {
+  const PhasePlan pl = make_plan(P);
+  T.H = H;
+  T.L = L;
+  T.P = P;
+  T.dE = dE;
+  T.xk = xk;
+  T.xmesh = xmesh;
+  T.zmesh = zmesh;
+  T.M = P.mesh_xbins * P.mesh_zbins;
+  T.part = 0;
+  T.obs = (int64_t)T.M * (pl.nA + pl.nB) * 2;
+  T.nA = pl.nA;
+  T.nB = pl.nB;
+  T.blockA0 = T.blockB0 = T.obs0 = 0;
+  if ((int64_t)T.M * (pl.nA + pl.nB) > ((int64_t)1 << 30))
+    return set_error(DFCSR_ERANGE, "%s: %d mesh points x %d phases is too many workgroups", who,
+                     T.M, pl.nA + pl.nB);
+  return DFCSR_OK;
+}
+
+constexpr int64_t kTableAlign = 256;
+
+int64_t table_bytes(int k) {
+  return (((int64_t)k * (int64_t)sizeof(WakeTask) + kTableAlign - 1) / kTableAlign) * kTableAlign;
+}
+
 }  // namespace
 }  // namespace dfcsr
 
 using namespace dfcsr;
 
-extern "C" int64_t dfcsr_wake_workspace_bytes(int mesh_points) {
-  (void)mesh_points;
-  return 256;
+extern "C" int64_t dfcsr_wake_workspace_bytes(const DfcsrWakeParams *p) {
+  if (!p || !params_ok(*p)) return 256;
+  return partial_doubles(*p) * (int64_t)sizeof(double) + 256;
 }
 
 extern "C" int dfcsr_wake_mesh(const DfcsrHistory *hist, const DfcsrLattice *lat,
@@ -497,42 +649,56 @@ extern "C" int dfcsr_wake_mesh(const DfcsrHistory *hist, const DfcsrLattice *lat
   if ((rc = check_lattice(lat, "wake_mesh"))) return rc;
   if ((rc = check_params(p, "wake_mesh"))) return rc;
   if (!dE_dct || !x_kick) return set_error(DFCSR_EINVAL, "wake_mesh: NULL output");
-  (void)ws;
-  (void)ws_bytes;
-  const int M = p->mesh_xbins * p->mesh_zbins;
-  if (precision == DFCSR_PREC_FP64) {
-    hipLaunchKernelGGL(wake_fp64_kernel, dim3(M), dim3(kThreads), 0, as_stream(stream), *hist,
-                       *lat, *p, xmesh, zmesh, dE_dct, x_kick);
-  } else if (precision == DFCSR_PREC_MIXED) {
-    if (!hist->data32)
-      return set_error(DFCSR_EINVAL, "wake_mesh: mixed precision needs the FP32 history");
-    hipLaunchKernelGGL(wake_mixed_kernel, dim3(M), dim3(kThreads), 0, as_stream(stream), *hist,
-                       *lat, *p, xmesh, zmesh, dE_dct, x_kick);
-  } else {
+  if ((xmesh == nullptr) != (zmesh == nullptr))
+    return set_error(DFCSR_EINVAL, "wake_mesh: xmesh and zmesh must both be given or both NULL");
+  if (precision != DFCSR_PREC_FP64 && precision != DFCSR_PREC_MIXED)
     return set_error(DFCSR_EINVAL, "wake_mesh: unknown precision %d", precision);
+  if (precision == DFCSR_PREC_MIXED && !hist->data32)
+    return set_error(DFCSR_EINVAL, "wake_mesh: mixed precision needs the FP32 history");
+  if (!ws || ws_bytes < dfcsr_wake_workspace_bytes(p))
+    return set_error(DFCSR_ENOWS, "wake_mesh: workspace of %lld bytes, %lld needed",
+                     (long long)ws_bytes, (long long)dfcsr_wake_workspace_bytes(p));
+  WakeTask T;
+  if ((rc = make_task(*hist, *lat, *p, dE_dct, x_kick, xmesh, zmesh, T, "wake_mesh"))) return rc;
+  double *part = static_cast<double *>(ws);
+  hipStream_t s = as_stream(stream);
+  const dim3 ga(T.M * T.nA), gb(T.M * T.nB), gf((T.M + kThreads - 1) / kThreads);
+  hipLaunchKernelGGL(wake_observer_kernel, gf, dim3(kThreads), 0, s, T, part);
+  if (precision == DFCSR_PREC_MIXED) {
+    hipLaunchKernelGGL((wake_phase_kernel<true, false>), ga, dim3(kThreads), 0, s, T, part);
+    hipLaunchKernelGGL((wake_phase_kernel<true, true>), gb, dim3(kThreads), 0, s, T, part);
+  } else {
+    hipLaunchKernelGGL((wake_phase_kernel<false, false>), ga, dim3(kThreads), 0, s, T, part);
+    hipLaunchKernelGGL((wake_phase_kernel<false, true>), gb, dim3(kThreads), 0, s, T, part);
   }
+  hipLaunchKernelGGL(wake_finalize_kernel, gf, dim3(kThreads), 0, s, T, (const double *)part);
   DFCSR_LAUNCH_CHECK("wake_mesh");
   return DFCSR_OK;
 }
 
-extern "C" int64_t dfcsr_wake_batched_workspace_bytes(int k) {
-  if (k < 1) k = 1;
-  return (int64_t)k * (int64_t)sizeof(WakeTask) + 256;
+extern "C" int64_t dfcsr_wake_task_table_bytes(int k) { return table_bytes(k < 1 ? 1 : k); }
+
+extern "C" int64_t dfcsr_wake_batched_workspace_bytes(int k, const DfcsrWakeParams *p) {
+  if (k < 1 || !p) return table_bytes(1);
+  int64_t n = 0;
+  for (int j = 0; j < k; ++j)
+    if (params_ok(p[j])) n += partial_doubles(p[j]);
+  return table_bytes(k) + n * (int64_t)sizeof(double);
 }
 
 extern "C" int dfcsr_wake_tasks_pack(int k, const DfcsrHistory *hist, const DfcsrLattice *lat,
                                      const DfcsrWakeParams *p, int precision,
                                      double *const *dE_dct, double *const *x_kick, void *tasks,
-                                     int64_t tasks_bytes, int *blocks) {
+                                     int64_t tasks_bytes, DfcsrWakeBatch *batch) {
   if (k < 1 || k > 4096) return set_error(DFCSR_ERANGE, "wake_tasks_pack: %d tasks", k);
-  if (!hist || !lat || !p || !dE_dct || !x_kick || !tasks || !blocks)
+  if (!hist || !lat || !p || !dE_dct || !x_kick || !tasks || !batch)
     return set_error(DFCSR_EINVAL, "wake_tasks_pack: NULL pointer");
   if (tasks_bytes < (int64_t)k * (int64_t)sizeof(WakeTask))
-    return set_error(DFCSR_ENOWS, "wake_tasks_pack: buffer too small");
+    return set_error(DFCSR_ENOWS, "wake_tasks_pack: task buffer too small");
   if (precision != DFCSR_PREC_FP64 && precision != DFCSR_PREC_MIXED)
     return set_error(DFCSR_EINVAL, "wake_tasks_pack: unknown precision %d", precision);
   WakeTask *th = static_cast<WakeTask *>(tasks);
-  int64_t nb = 0;
+  int64_t na = 0, nb = 0, nobs = 0, part = 0;
   for (int j = 0; j < k; ++j) {
     int rc;
     if ((rc = check_history(&hist[j], "wake_tasks_pack"))) return rc;
@@ -541,32 +707,57 @@ extern "C" int dfcsr_wake_tasks_pack(int k, const DfcsrHistory *hist, const Dfcs
     if (!dE_dct[j] || !x_kick[j]) return set_error(DFCSR_EINVAL, "wake_tasks_pack: NULL output");
     if (precision == DFCSR_PREC_MIXED && !hist[j].data32)
       return set_error(DFCSR_EINVAL, "wake_tasks_pack: task %d lacks the FP32 history", j);
-    th[j].H = hist[j];
-    th[j].L = lat[j];
-    th[j].P = p[j];
-    th[j].dE = dE_dct[j];
-    th[j].xk = x_kick[j];
-    th[j].block0 = (int)nb;
-    th[j].M = p[j].mesh_xbins * p[j].mesh_zbins;
-    nb += th[j].M;
-    if (nb > ((int64_t)1 << 30)) return set_error(DFCSR_ERANGE, "wake_tasks_pack: too many blocks");
+    WakeTask &T = th[j];
+    if ((rc = make_task(hist[j], lat[j], p[j], dE_dct[j], x_kick[j], nullptr, nullptr, T,
+                        "wake_tasks_pack")))
+      return rc;
+    T.obs += part;
+    T.part = part;
+    T.blockA0 = (int)na;
+    T.blockB0 = (int)nb;
+    T.obs0 = (int)nobs;
+    part += partial_doubles(p[j]);
+    na += (int64_t)T.M * T.nA;
+    nb += (int64_t)T.M * T.nB;
+    nobs += T.M;
+    if (na > ((int64_t)1 << 30) || nb > ((int64_t)1 << 30))
+      return set_error(DFCSR_ERANGE, "wake_tasks_pack: too many workgroups");
   }
-  *blocks = (int)nb;
+  batch->tasks = k;
+  batch->blocks_far = (int32_t)na;
+  batch->blocks_near = (int32_t)nb;
+  batch->observers = (int32_t)nobs;
+  batch->partial_bytes = part * (int64_t)sizeof(double);
   return DFCSR_OK;
 }
 
-extern "C" int dfcsr_wake_batched_launch(const void *dev_tasks, int k, int blocks, int precision,
+extern "C" int dfcsr_wake_batched_launch(const void *dev_tasks, const DfcsrWakeBatch *batch,
+                                         int precision, void *partials, int64_t partials_bytes,
                                          void *stream) {
-  if (!dev_tasks || k < 1 || blocks < 1)
+  if (!dev_tasks || !batch || batch->tasks < 1 || batch->blocks_far < 1 ||
+      batch->blocks_near < 1 || batch->observers < 1)
     return set_error(DFCSR_EINVAL, "wake_batched_launch: bad arguments");
+  if (!partials || partials_bytes < batch->partial_bytes)
+    return set_error(DFCSR_ENOWS, "wake_batched_launch: partial-sum buffer too small");
   hipStream_t s = as_stream(stream);
   const WakeTask *dt = static_cast<const WakeTask *>(dev_tasks);
-  if (precision == DFCSR_PREC_MIXED)
-    hipLaunchKernelGGL(wake_batched_kernel<true>, dim3(blocks), dim3(kThreads), 0, s, dt, k);
-  else if (precision == DFCSR_PREC_FP64)
-    hipLaunchKernelGGL(wake_batched_kernel<false>, dim3(blocks), dim3(kThreads), 0, s, dt, k);
-  else
+  double *part = static_cast<double *>(partials);
+  const int k = batch->tasks;
+  const dim3 ga(batch->blocks_far), gb(batch->blocks_near);
+  const dim3 gf((batch->observers + kThreads - 1) / kThreads);
+  hipLaunchKernelGGL(wake_observer_batched_kernel, gf, dim3(kThreads), 0, s, dt, k,
+                     batch->observers, part);
+  if (precision == DFCSR_PREC_MIXED) {
+    hipLaunchKernelGGL((wake_phase_batched_kernel<true, false>), ga, dim3(kThreads), 0, s, dt, k, part);
+    hipLaunchKernelGGL((wake_phase_batched_kernel<true, true>), gb, dim3(kThreads), 0, s, dt, k, part);
+  } else if (precision == DFCSR_PREC_FP64) {
+    hipLaunchKernelGGL((wake_phase_batched_kernel<false, false>), ga, dim3(kThreads), 0, s, dt, k, part);
+    hipLaunchKernelGGL((wake_phase_batched_kernel<false, true>), gb, dim3(kThreads), 0, s, dt, k, part);
+  } else {
     return set_error(DFCSR_EINVAL, "wake_batched_launch: unknown precision %d", precision);
+  }
+  hipLaunchKernelGGL(wake_finalize_batched_kernel, gf, dim3(kThreads), 0, s, dt, k,
+                     batch->observers, (const double *)part);
   DFCSR_LAUNCH_CHECK("wake_batched_launch");
   return DFCSR_OK;
 }
@@ -576,16 +767,19 @@ extern "C" int dfcsr_wake_mesh_batched(int k, const DfcsrHistory *hist, const Df
                                        double *const *dE_dct, double *const *x_kick, void *ws,
                                        int64_t ws_bytes, void *stream) {
   if (k < 1) return DFCSR_OK;
-  if (!ws) return set_error(DFCSR_EINVAL, "wake_mesh_batched: NULL workspace");
-  if (ws_bytes < dfcsr_wake_batched_workspace_bytes(k))
-    return set_error(DFCSR_ENOWS, "wake_mesh_batched: workspace too small");
   if (k > 256) return set_error(DFCSR_ERANGE, "wake_mesh_batched: %d tasks > 256", k);
+  if (!ws) return set_error(DFCSR_EINVAL, "wake_mesh_batched: NULL workspace");
+  if (!p) return set_error(DFCSR_EINVAL, "wake_mesh_batched: NULL params");
+  if (ws_bytes < dfcsr_wake_batched_workspace_bytes(k, p))
+    return set_error(DFCSR_ENOWS, "wake_mesh_batched: workspace too small");
   WakeTask tasks_h[256];
-  int blocks = 0;
+  DfcsrWakeBatch batch;
   int rc = dfcsr_wake_tasks_pack(k, hist, lat, p, precision, dE_dct, x_kick, tasks_h,
-                                 (int64_t)sizeof(tasks_h), &blocks);
+                                 (int64_t)sizeof(tasks_h), &batch);
   if (rc) return rc;
   hipStream_t s = as_stream(stream);
   DFCSR_CHECK_HIP(hipMemcpyAsync(ws, tasks_h, (size_t)k * sizeof(WakeTask), hipMemcpyHostToDevice, s));
-  return dfcsr_wake_batched_launch(ws, k, blocks, precision, stream);
+  const int64_t tb = table_bytes(k);
+  return dfcsr_wake_batched_launch(ws, &batch, precision, static_cast<char *>(ws) + tb,
+                                   ws_bytes - tb, stream);
 }

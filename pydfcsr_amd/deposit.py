@@ -9,10 +9,13 @@ interp grid; build_interpolant fixes the axes the wake kernel reads.
 Storage is two rings with one slot per logged slice, on the device for the
 whole run:
   raw   [cap][5][nbmax*nbmax] float64  fields on the slice's own grid
-  hist  [cap][Xi][Zi][20]     float64  re-gridded corner-packed records
-  hist32[cap][Xi][Zi][32]     float32  shadow read by the mixed-precision wake
+  hist  [Xi][Zi][cap][20]     float64  re-gridded corner-packed records
+  hist32[Xi][cap][5][Zi][4]   float32  shadow read by the mixed-precision wake
 (record (x, z) holds every field at (x, z), (x, z+1), (x+1, z), (x+1, z+1),
-clamped to the last plane: one record per time plane per lookup)
+clamped to the last plane: one record per time plane per lookup.  The
+float64 ring keeps the slot innermost (the two planes of a lookup are
+neighbours); the float32 ring stores each field's four corners as one
+16-byte chunk, z fastest, so neighbouring z of one plane are neighbours)
 Appending writes slot (head + nt) % cap; popping advances head.  Nothing is
 copied per step: the reference's np.array(deque) stacking of the whole
 history (deposit.py:422-426) becomes a descriptor update.
@@ -93,33 +96,34 @@ class DF_tracker:
         raw = torch.empty((new_cap, self._raw_stride()), dtype=torch.float64, device=self.device)
         hist = hist32 = None
         if self.Xi:
-            hist = torch.empty((new_cap, self.Xi, self.Zi, _lib.HIST_REC64), dtype=torch.float64,
-                               device=self.device)
-            if self.keep_fp32:
-                hist32 = torch.empty((new_cap, self.Xi, self.Zi, _lib.HIST_REC32),
-                                     dtype=torch.float32, device=self.device)
+            hist, hist32 = self._rings(self.Xi, self.Zi, new_cap)
         if self.cap:
             # keep the ring's logical order, pending slot included (index nt)
             for i in range(min(self.nt + 1, self.cap)):
                 s = self._slot(i)
                 raw[i].copy_(self.raw[s])
                 if hist is not None and self.hist is not None:
-                    hist[i].copy_(self.hist[s])
+                    hist[:, :, i].copy_(self.hist[:, :, s])
                     if hist32 is not None and self.hist32 is not None:
-                        hist32[i].copy_(self.hist32[s])
+                        hist32[:, i].copy_(self.hist32[:, s])
         self.raw, self.hist, self.hist32 = raw, hist, hist32
         self.cap = new_cap
         self.head = 0
+
+    def _rings(self, Xi, Zi, cap):
+        hist = torch.empty((Xi, Zi, cap, _lib.HIST_REC64), dtype=torch.float64,
+                           device=self.device)
+        hist32 = None
+        if self.keep_fp32:
+            hist32 = torch.empty((Xi, cap, _lib.HIST_CHUNKS32, Zi, 4), dtype=torch.float32,
+                                 device=self.device)
+        return hist, hist32
 
     def _alloc_hist(self, Xi, Zi):
         if self.hist is None or self.Xi != Xi or self.Zi != Zi:
             self.hist = None
             self.hist32 = None
-            self.hist = torch.empty((self.cap, Xi, Zi, _lib.HIST_REC64), dtype=torch.float64,
-                                    device=self.device)
-            if self.keep_fp32:
-                self.hist32 = torch.empty((self.cap, Xi, Zi, _lib.HIST_REC32),
-                                          dtype=torch.float32, device=self.device)
+            self.hist, self.hist32 = self._rings(Xi, Zi, self.cap)
         self.Xi, self.Zi = Xi, Zi
 
     def raw_fields(self, i=None):
@@ -242,8 +246,8 @@ class DF_tracker:
         """Stacked (nt, Xi, Zi) copies of the five fields in logical order
         (the reference's data_*_interp arrays; for inspection and tests)."""
         idx = torch.tensor([self._slot(i) for i in range(self.nt)], device=self.device)
-        h = self.hist.index_select(0, idx)
-        return {f: h[..., 4 * k] for k, f in enumerate(_lib.FIELDS)}   # corner (x, z)
+        h = self.hist[..., 0::4].index_select(2, idx)   # corner (x, z) of each field
+        return {f: h[..., k].permute(2, 0, 1).contiguous() for k, f in enumerate(_lib.FIELDS)}
 
     def footprint_bytes(self):
         b = self.raw.numel() * 8 if self.raw is not None else 0

@@ -122,15 +122,20 @@ def savgol_operator(window, order):
 
 def regrid(src: torch.Tensor, slot_stride: int, jobs, hist64, hist32, Xi, Zi, xi0, xi1, zi0, zi1,
            ws: _lib.Workspace):
-    """jobs: list of (src_slot, dst_slot, nbx, nbz, x0, x1, z0, z1)."""
+    """jobs: list of (src_slot, dst_slot, nbx, nbz, x0, x1, z0, z1); hist64 is the
+    (Xi, Zi, cap, 20) float64 ring (hist32 the (Xi, cap, 5, Zi, 4) float32 one or None)."""
     lib = _lib.load()
     n = len(jobs)
     if n == 0:
         return
     arr = (_lib.RegridJob * n)(*[_lib.RegridJob(*j) for j in jobs])
     b = ws.get("regrid", lib.dfcsr_regrid_workspace_bytes(n))
+    cap = int(hist64.shape[2])
+    if (tuple(hist64.shape) != (Xi, Zi, cap, _lib.HIST_REC64) or
+            (hist32 is not None and tuple(hist32.shape) != (Xi, cap, _lib.HIST_CHUNKS32, Zi, 4))):
+        raise DfcsrError(f"regrid: history rings {tuple(hist64.shape)} do not match {Xi} x {Zi}")
     check(lib.dfcsr_regrid(ptr(src), int(slot_stride), arr, n, ptr(hist64), ptr(hist32), int(Xi),
-                           int(Zi), float(xi0), float(xi1), float(zi0), float(zi1), ptr(b),
+                           int(Zi), cap, float(xi0), float(xi1), float(zi0), float(zi1), ptr(b),
                            b.numel(), stream_handle(src.device)), "regrid")
 
 
@@ -170,7 +175,7 @@ def wake_mesh(hist: _lib.History, lat: _lib.Lattice, p: _lib.WakeParams, precisi
     if out is None:
         out = {k: torch.empty(M, dtype=torch.float64, device=device)
                for k in ("xmesh", "zmesh", "dE_dct", "x_kick")}
-    b = ws.get("wake", lib.dfcsr_wake_workspace_bytes(M))
+    b = ws.get("wake", lib.dfcsr_wake_workspace_bytes(ctypes.byref(p)))
     check(lib.dfcsr_wake_mesh(ctypes.byref(hist), ctypes.byref(lat), ctypes.byref(p),
                               int(precision), ptr(out["xmesh"]) if want_mesh else None,
                               ptr(out["zmesh"]) if want_mesh else None, ptr(out["dE_dct"]),
@@ -187,7 +192,7 @@ def wake_mesh_batched(hists, lats, params, precision, outs, device, ws: _lib.Wor
     P = (_lib.WakeParams * k)(*params)
     dE = (_lib.c_dp * k)(*[o["dE_dct"].data_ptr() for o in outs])
     xk = (_lib.c_dp * k)(*[o["x_kick"].data_ptr() for o in outs])
-    b = ws.get("wake_batched", lib.dfcsr_wake_batched_workspace_bytes(k))
+    b = ws.get("wake_batched", lib.dfcsr_wake_batched_workspace_bytes(k, P))
     check(lib.dfcsr_wake_mesh_batched(k, H, L, P, int(precision), dE, xk, ptr(b), b.numel(),
                                       stream_handle(device)), "wake_mesh_batched")
 

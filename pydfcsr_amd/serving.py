@@ -11,7 +11,7 @@ in a FIFO queue.  At every tick the server
      (tracking, deposition, density functions, history update, mesh) up to
      its wake;
   3. runs the wakes of all of them in ONE batched launch
-     (dfcsr_wake_batched_launch: one workgroup per mesh point across all
+     (dfcsr_wake_batched_launch: workgroups of (mesh point, phase) over all
      simulations), replayed from a HIP graph captured once per active-set
      shape;
   4. resumes each simulation to apply its kicks, retires finished ones and
@@ -65,7 +65,7 @@ def request_footprint(config: dict, steps: int, precision: str = "mixed") -> int
     ul = dep.get("upper_limit")
     grid = int(ul) if isinstance(ul, int) else 2000
     slots = steps + 2
-    rec = 160 + (128 if precision == "mixed" else 0)   # corner-packed records
+    rec = 160 + (80 if precision == "mixed" else 0)   # corner-packed records
     hist = slots * grid * grid * rec
     raw = slots * 5 * nb * nb * 8
     particles = n * 8 * (6 * 2 + 4)           # coordinates, tracking temporary, scratch
@@ -76,8 +76,11 @@ def request_footprint(config: dict, steps: int, precision: str = "mixed") -> int
 
 
 class BatchedWake:
-    """One launch for the wakes of several simulations, optionally replayed
-    from a captured HIP graph (one per (tasks, workgroups) shape)."""
+    """One batched launch for the wakes of several simulations, optionally
+    replayed from a captured HIP graph (one per launch shape).  The task
+    table lives in a fixed device buffer rewritten before every launch; the
+    partial sums in a device buffer that only grows (a reallocation drops the
+    captured graphs, which hold its address)."""
 
     def __init__(self, device, precision="mixed", use_graph=True, max_tasks=256,
                  graph_cache=16):
@@ -88,9 +91,10 @@ class BatchedWake:
         self.use_graph = use_graph
         self.lib = _lib.load()
         self.max_tasks = max_tasks
-        nbytes = int(self.lib.dfcsr_wake_batched_workspace_bytes(max_tasks))
+        nbytes = int(self.lib.dfcsr_wake_task_table_bytes(max_tasks))
         self.host = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True)
         self.dev = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+        self.part = torch.empty(0, dtype=torch.uint8, device=self.device)
         self.graphs = collections.OrderedDict()
         self.graph_cache = graph_cache
         self.seen = set()
@@ -110,24 +114,36 @@ class BatchedWake:
         P = (_lib.WakeParams * k)(*[it[2] for it in items])
         dE = (_lib.c_dp * k)(*[it[3]["dE_dct"].data_ptr() for it in items])
         xk = (_lib.c_dp * k)(*[it[3]["x_kick"].data_ptr() for it in items])
-        blocks = ctypes.c_int(0)
+        batch = _lib.WakeBatch()
         _lib.check(self.lib.dfcsr_wake_tasks_pack(k, H, L, P, self.prec, dE, xk,
                                                   self.host.data_ptr(), self.host.numel(),
-                                                  ctypes.byref(blocks)), "wake_tasks_pack")
-        nbytes = int(self.lib.dfcsr_wake_batched_workspace_bytes(k))
+                                                  ctypes.byref(batch)), "wake_tasks_pack")
+        if batch.partial_bytes > self.part.numel():
+            # freeing the old buffer is stream-ordered: torch's allocator hands it
+            # out again only to work queued after the previous launch
+            self.part = torch.empty(max(int(batch.partial_bytes), 2 * self.part.numel()),
+                                    dtype=torch.uint8, device=self.device)
+            self.graphs.clear()
+            self.seen.clear()
+        nbytes = int(self.lib.dfcsr_wake_task_table_bytes(k))
         # the previous launch may still read the device task buffer: the copy
         # is ordered after it on the same stream
         self.dev[:nbytes].copy_(self.host[:nbytes], non_blocking=True)
-        key = (k, blocks.value)
+        key = (k, batch.blocks_far, batch.blocks_near, batch.observers)
         stream = torch.cuda.current_stream(self.device)
+
+        def launch(s):
+            _lib.check(self.lib.dfcsr_wake_batched_launch(self.dev.data_ptr(), ctypes.byref(batch),
+                                                          self.prec, self.part.data_ptr(),
+                                                          self.part.numel(), s),
+                       "batched launch")
+
         if self.use_graph and key in self.seen:
             g = self.graphs.get(key)
             if g is None:
                 g = torch.cuda.CUDAGraph()
                 with torch.cuda.graph(g):
-                    _lib.check(self.lib.dfcsr_wake_batched_launch(
-                        self.dev.data_ptr(), k, blocks.value, self.prec,
-                        torch.cuda.current_stream(self.device).cuda_stream), "batched launch")
+                    launch(torch.cuda.current_stream(self.device).cuda_stream)
                 self.graphs[key] = g
                 if len(self.graphs) > self.graph_cache:
                     self.graphs.popitem(last=False)
@@ -137,9 +153,7 @@ class BatchedWake:
             self.replays += 1
         else:
             # first use of a shape runs eagerly (module load stays out of capture)
-            _lib.check(self.lib.dfcsr_wake_batched_launch(self.dev.data_ptr(), k, blocks.value,
-                                                          self.prec, stream.cuda_stream),
-                       "batched launch")
+            launch(stream.cuda_stream)
             self.seen.add(key)
         self.launches += 1
 

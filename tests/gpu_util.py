@@ -10,7 +10,7 @@ from pydfcsr_amd import _lib
 FIELDS = _lib.FIELDS
 
 
-def corner_records(data, rec_len):
+def corner_records(data, rec_len=20):
     """(nt, Xi, Zi, 5) field stacks -> corner-packed records (nt, Xi, Zi, rec_len):
     element 4 f + c holds field f at (x, z), (x, z1), (x1, z), (x1, z1)."""
     nt, Xi, Zi, _ = data.shape
@@ -26,6 +26,28 @@ def corner_records(data, rec_len):
     return rec
 
 
+def ring64(records, cap, head):
+    """Corner-packed float64 records (nt, Xi, Zi, 20) -> the device ring
+    (Xi, Zi, cap, 20), logical slice i in slot (head + i) % cap."""
+    nt, Xi, Zi, n = records.shape
+    ring = np.zeros((Xi, Zi, cap, n), dtype=records.dtype)
+    for i in range(nt):
+        ring[:, :, (head + i) % cap] = records[i]
+    return ring
+
+
+def device_ring(records, cap, head, width=4):
+    """Corner-packed records (nt, Xi, Zi, 20) -> the chunked float32 ring
+    (Xi, cap, 20 // width, Zi, width): 16-byte chunks of `width` values,
+    z fastest, logical slice i in slot (head + i) % cap."""
+    nt, Xi, Zi, n = records.shape
+    ch = records.reshape(nt, Xi, Zi, n // width, width).transpose(1, 0, 3, 2, 4)
+    ring = np.zeros((Xi, cap, n // width, Zi, width), dtype=records.dtype)
+    for i in range(nt):
+        ring[:, (head + i) % cap] = ch[:, i]
+    return ring
+
+
 class DeviceHistory:
     """A device ring holding an oracle DFTracker's stacked history.
 
@@ -36,16 +58,12 @@ class DeviceHistory:
         nt, Xi, Zi, _ = data.shape
         cap = nt + (wrap + 1 if wrap else 0)
         head = (cap - wrap) % cap if wrap else 0
-        packed = corner_records(data, _lib.HIST_REC64)
-        rec = np.zeros((cap, Xi, Zi, _lib.HIST_REC64))
-        for i in range(nt):
-            rec[(head + i) % cap] = packed[i]
-        self.h64 = torch.from_numpy(rec).to(device)
+        packed = corner_records(data)
+        self.h64 = torch.from_numpy(ring64(packed, cap, head)).to(device)
         self.h32 = None
         if fp32:
-            r32 = np.zeros((cap, Xi, Zi, _lib.HIST_REC32), dtype=np.float32)
-            r32[..., :20] = rec[..., :20].astype(np.float32)
-            self.h32 = torch.from_numpy(r32).to(device)
+            self.h32 = torch.from_numpy(device_ring(packed.astype(np.float32), cap, head,
+                                                    4)).to(device)
         a = tracker.axes()
         self.desc = _lib.History(self.h64.data_ptr(),
                                  self.h32.data_ptr() if self.h32 is not None else None,

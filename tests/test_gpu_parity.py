@@ -125,10 +125,10 @@ def test_interpolate_golden_bitexact(K, golden_dir):
     d = _g(golden_dir, "ref_interpolate.npz")
     data = d["data3"]
     nt, nx, nz = data.shape
-    from tests.gpu_util import corner_records
+    from tests.gpu_util import corner_records, ring64
     stack = np.zeros((nt, nx, nz, 5))
     stack[..., 0] = data
-    h = _t(corner_records(stack, _lib.HIST_REC64))
+    h = _t(ring64(corner_records(stack), nt, 0))
     H = _lib.History(h.data_ptr(), None, nt, 0, nt, nx, nz, *[float(v) for v in d["mins"]],
                      *[float(v) for v in d["dels"]])
     r = K.interpolate3D(H, 0, _t(d["q3"]), DEV).cpu().numpy()

@@ -1,0 +1,139 @@
+"""Driver-level parity: CSR2D's run loop (CSR.py:202-358) against the oracle,
+step by step, on a short dipole + drift lattice.
+
+Tracking is outside the accelerated path (its transfer maps are not pinned),
+so at every step the oracle is fed the particles the driver tracked: after
+the driver's history update (external_wake hook) the oracle repeats
+get_DF -> append_DF -> append_interpolant -> build_interpolant
+(deposit.py:145-426) on the same particles, the mesh (CSR.py:361-394), the
+wake of every mesh point (CSR.py:397-602) and, after the driver's kick, the
+kick itself (beams.py:108-131).  Checked at every step: history axes and
+fields, mesh axes, dE_dct / x_kick, px / pz.  The initial deposition of
+CSR2D.initialization (CSR.py:70-81) is replayed the same way.
+"""
+from collections import OrderedDict
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda"
+
+
+def tiny_config(n=20000):
+    lat = OrderedDict(step_size=0.005)
+    lat["B1"] = dict(type="dipole", L=0.05, angle=0.01, E1=0.0, E2=0.0, nsep=1)
+    lat["D1"] = dict(type="drift", L=0.03, nsep=1)
+    return {
+        "input_beam": dict(style="gaussian", n_particle=n, charge=1e-9, energy=1e9,
+                           sigma_x=50e-6, sigma_z=50e-6, chirp=-10.0, sigma_delta=1e-4,
+                           seed=3),
+        "input_lattice": dict(lattice_input_file="<tiny>", _config=lat),
+        "particle_deposition": dict(xbins=32, zbins=32, xlim=5, zlim=5, filter_order=2,
+                                    filter_window=5, velocity_threhold=1000, upper_limit=160),
+        "CSR_integration": dict(n_formation_length=4, zbins=16, xbins=16),
+        "CSR_computation": dict(xbins=8, zbins=8, xlim=5, zlim=5, apply_CSR=1, transverse_on=1,
+                                write_wakes=False),
+    }
+
+
+def _np(t):
+    return t.detach().cpu().numpy().astype(np.float64)
+
+
+def _rel(a, b):
+    """max |a - b| / max |b| over the finite entries; NaN where the reference
+    has NaN must be NaN here too (and only there)."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    na, nb = np.isnan(a), np.isnan(b)
+    assert np.array_equal(na, nb), f"NaN at {np.argwhere(na != nb)[:5].tolist()} " \
+        f"(mine {int(na.sum())}, reference {int(nb.sum())})"
+    ok = ~nb
+    if not ok.any():
+        return 0.0
+    den = np.abs(b[ok]).max()
+    d = np.abs(a[ok] - b[ok]).max()
+    return float(d / den) if den > 0 else float(d)
+
+
+def test_run_loop_matches_oracle_step_by_step():
+    import torch
+    from oracle import cpu_ref as O
+    from pydfcsr_amd import kernels
+    from pydfcsr_amd.csr import CSR2D
+
+    cfg = tiny_config()
+    sim = CSR2D(config=cfg, device=DEV, precision="fp64")
+    dep = cfg["particle_deposition"]
+    ip, cp = cfg["CSR_integration"], cfg["CSR_computation"]
+    ora = O.DFTracker(xbins=dep["xbins"], zbins=dep["zbins"], xlim=dep["xlim"],
+                      zlim=dep["zlim"], filter_order=dep["filter_order"],
+                      filter_window=dep["filter_window"],
+                      velocity_threhold=dep["velocity_threhold"],
+                      upper_limit=dep["upper_limit"])
+    b = sim.beam
+    ora.get_DF(_np(b.x), _np(b.z), _np(b.px), float(b.position))
+    ora.append_DF()
+    ora.append_interpolant(float("inf"), ip["n_formation_length"])
+    lat = O.LatticeTables([dict(v) for k, v in cfg["input_lattice"]["_config"].items()
+                           if k != "step_size"])
+    step_size = cfg["input_lattice"]["_config"]["step_size"]
+
+    steps = 0
+    pending = None
+    worst = dict(hist=0.0, wake=0.0, kick=0.0)
+    for kind, step in sim.iter_phases(external_wake=True):
+        if kind == "wake":
+            x, z, px, pz = _np(b.x), _np(b.z), _np(b.px), _np(b.pz)
+            t = float(b.position)
+            ora.get_DF(x, z, px, t)
+            ora.append_DF()
+            ora.append_interpolant(sim.formation_length, ip["n_formation_length"])
+            ora.build_interpolant()
+            tr = sim.DF_tracker
+            assert tr.nt == len(ora.time_interp), (step, tr.nt, len(ora.time_interp))
+            for mine, ref in ((tr.min_x, ora.min_x), (tr.delta_x, ora.delta_x),
+                              (tr.min_y, ora.min_y), (tr.delta_y, ora.delta_y),
+                              (tr.min_z, ora.min_z), (tr.delta_z, ora.delta_z)):
+                assert abs(mine - ref) <= 1e-12 * max(abs(ref), 1e-30), (step, mine, ref)
+            hf = tr.history_fields()
+            for f in O.FIELDS:
+                e = _rel(_np(hf[f]), ora.data[f])
+                worst["hist"] = max(worst["hist"], e)
+                assert e <= 1e-9, (step, f, e)
+
+            mesh = O.csr_mesh(x, z, cp["xbins"], cp["zbins"], cp["xlim"], cp["zlim"])
+            assert _rel(sim.CSR_zrange, mesh["zrange"]) <= 1e-12
+            assert _rel(sim.CSR_xrange_transformed, mesh["xrange"]) <= 1e-11
+            H, L, P, out = sim.prepare_wake()
+            kernels.wake_mesh(H, L, P, 0, sim.device, sim.ws, out=out, want_mesh=False)
+            torch.cuda.synchronize()
+            m = O.beam_moments(x, z)
+            beam = dict(sigma_x=m["sigma_x"], sigma_z=m["sigma_z"], slope=m["slope"],
+                        mean_x=m["mean_x"])
+            dE, xk = O.calculate_2D_CSR(t, mesh, beam, sim.formation_length,
+                                        ip["n_formation_length"], ip["xbins"], ip["zbins"], lat,
+                                        ora, sim.CSR_scaling, cp["xbins"], cp["zbins"])
+            e = max(_rel(_np(out["dE_dct"]).reshape(dE.shape), dE),
+                    _rel(_np(out["x_kick"]).reshape(xk.shape), xk))
+            worst["wake"] = max(worst["wake"], e)
+            assert e <= 1e-9, (step, e)
+            pending = (x, z, px, pz, dE, xk, mesh)
+        else:
+            steps = step
+            if pending is None:
+                continue
+            x, z, px, pz, dE, xk, mesh = pending
+            px_o, pz_o = O.apply_wakes(x, z, px, pz, dE, xk, mesh["xrange"], mesh["zrange"],
+                                       step_size, float(b.init_energy), True)
+            # relative to the largest kick; NaN kicks (a wake of 0/0 at mesh
+            # points past the lattice end, as in the reference) must coincide
+            e = max(_rel(_np(b.px) - px, px_o - px), _rel(_np(b.pz) - pz, pz_o - pz))
+            worst["kick"] = max(worst["kick"], e)
+            assert e <= 1e-8, (step, e)
+            pending = None
+    # positions 0, 0.005, ..., 0.08: 16 steps (total_steps counts position 0 too)
+    assert steps == int(sum(sim.lattice.steps_per_element)) == sim.lattice.total_steps - 1 == 16
+    print("worst relative differences over", steps, "steps:", worst)

@@ -61,6 +61,38 @@ def chicane_config(n_particle=1_000_000, nbins=128, window=7, mesh=64, Nxz=200):
     }
 
 
+def dipole_config(n_particle=200_000, nbins=64, window=5, mesh=32, Nxz=200):
+    """Single dipole, plain method: 2e5 macroparticles, 1 nC, 5 GeV, Gaussian
+    sigma_z = sigma_x = 50 um, no transverse momentum or energy spread, seed 0;
+    rho = 10 m, L = 0.5 m (0.05 rad), 100 steps; deposition 64 x 64 over
+    +/-5 sigma with Savitzky-Golay 5/2; wake mesh 32 x 32; integration
+    200 x 200, n_formation_length 4 (the reference's defaults)."""
+    lat = OrderedDict(step_size=0.5 / 100)
+    lat["B1"] = dict(type="dipole", L=0.5, angle=0.05, E1=0.0, E2=0.0, nsep=1)
+    return {
+        "input_beam": dict(style="gaussian", n_particle=n_particle, charge=1e-9, energy=5e9,
+                           sigma_x=50e-6, sigma_z=50e-6, chirp=0.0, sigma_delta=0.0, seed=0),
+        "input_lattice": dict(lattice_input_file="<dipole>", _config=lat),
+        "particle_deposition": dict(xbins=nbins, zbins=nbins, xlim=5, zlim=5, filter_order=2,
+                                    filter_window=window, velocity_threhold=1000),
+        "CSR_integration": dict(n_formation_length=4, zbins=Nxz, xbins=Nxz),
+        "CSR_computation": dict(xbins=mesh, zbins=mesh, xlim=5, zlim=5, apply_CSR=1,
+                                transverse_on=1, write_wakes=False),
+    }
+
+
+WORKLOADS = {
+    # name: (config, default warm-up steps, description)
+    "chicane": (chicane_config, 200,
+                "chicane bunch compressor: 1e6 particles, 1 nC, 1 GeV, 4 bends x 0.3 m at "
+                "0.05 rad, drifts 2/1/2 m, 400 steps; deposition 128x128 SG 7/2; wake mesh "
+                "64x64; integration 200x200; n_fl 4"),
+    "dipole": (dipole_config, 90,
+               "single dipole: 2e5 particles, 1 nC, 5 GeV, rho 10 m, L 0.5 m, 100 steps; "
+               "deposition 64x64 SG 5/2; wake mesh 32x32; integration 200x200; n_fl 4"),
+}
+
+
 def cpu_baseline(sim, gpu_out, budget_s=15.0, max_points=None):
     """The oracle's C/OpenMP integral on the simulation's current history and
     wake parameters; gpu_out is the FP64 device wake for the same inputs."""
@@ -100,7 +132,7 @@ def cpu_baseline(sim, gpu_out, budget_s=15.0, max_points=None):
             "kind": "port",
             "sample": f"{k} of {obs.shape[0]} mesh points of the last timed step "
                       f"({per_point} integrand points each, {dt:.1f} s)",
-            "max_rel_diff_vs_gpu_dE": err}
+            "max_rel_diff_vs_gpu_dE": err}, dE
 
 
 def profiled_traffic():
@@ -195,8 +227,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--workload", default="chicane", choices=["chicane", "serving"])
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed steps before the timed ones (default: 200 chicane, 90 dipole)")
+    ap.add_argument("--workload", default="chicane", choices=["chicane", "dipole", "serving"])
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrivals per second")
     ap.add_argument("--serve-seconds", type=float, default=90.0)
@@ -229,7 +262,10 @@ def main():
     if args.workload == "serving":
         serve_main(args, dev)
         return
-    cfg = chicane_config()
+    make_cfg, default_warmup, description = WORKLOADS[args.workload]
+    if args.warmup is None:
+        args.warmup = default_warmup
+    cfg = make_cfg()
     t_init = time.perf_counter()
     sim = CSR2D(config=cfg, device=dev, precision=args.precision, keep_fp32=True)
     it = sim.iter_steps()
@@ -268,11 +304,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32+f64",
-        "data": "synthetic (seeded Gaussian beam, chicane lattice)",
-        "config": {"workload": "chicane bunch compressor: 1e6 particles, 1 nC, 1 GeV, "
-                               "4 bends x 0.3 m at 0.05 rad, drifts 2/1/2 m, 400 steps; "
-                               "deposition 128x128 SG 7/2; wake mesh 64x64; integration "
-                               "200x200; n_fl 4",
+        "data": f"synthetic (seeded Gaussian beam, {args.workload} lattice)",
+        "config": {"workload": description,
                    "precision": args.precision,
                    "steps_timed": f"{ev[0][5] if ev else 0} mesh points x "
                                   f"{(pts[0] // ev[0][5]) if ev else 0} points per step",
@@ -296,7 +329,14 @@ def main():
     if not args.no_cpu_baseline:
         print("cpu baseline: oracle C/OpenMP integral on the same history", file=sys.stderr,
               flush=True)
-        res["cpu_baseline"] = cpu_baseline(sim, outs["fp64"], budget_s=args.cpu_budget)
+        res["cpu_baseline"], dE_cpu = cpu_baseline(sim, outs["fp64"], budget_s=args.cpu_budget)
+        # the mixed form against the FP64 CPU reference on the same sample
+        k = dE_cpu.shape[0]
+        d = outs["mixed"]["dE_dct"][:k] - dE_cpu
+        m = np.abs(dE_cpu).max()
+        res["precision_compare"]["mixed"]["max_rel_err_dE_dct_vs_cpu"] = float(np.abs(d).max() / m)
+        res["precision_compare"]["mixed"]["rms_rel_err_dE_dct_vs_cpu"] = float(
+            np.sqrt(np.mean(d * d)) / m)
     print(json.dumps(res), flush=True)
 
 

@@ -153,7 +153,9 @@ __device__ inline Column make_column(const DfcsrLattice &L, const Observer &o, d
 
 // One integrand point (I_z, I_x) at source (x', s'), CSR.py:645-775 in the
 // reference's operation order.  Returns false (I = 0) when the retarded
-// lookup falls outside the history, where every gathered field is zero.
+// lookup falls outside the history, where every gathered field is zero and
+// the reference's formula gives exactly 0 -- except at R == 0, where its
+// divisions give 0/0: that point is evaluated (with zero fields) as it is.
 template <bool kRecip>
 __device__ inline bool csr_point(const DfcsrHistory &H, const HistAxes &A, const Observer &o,
                                  const Column &c, double xp, double &Iz, double &Ix) {
@@ -162,7 +164,7 @@ __device__ inline bool csr_point(const DfcsrHistory &H, const HistAxes &A, const
   const double R = sqrt(rx * rx + ry * ry);
   const double t_ret = o.t - R;
   double f[5];
-  if (!hist_lookup5<kRecip>(H, A, t_ret, xp, c.sp - t_ret, f)) {
+  if (!hist_lookup5<kRecip>(H, A, t_ret, xp, c.sp - t_ret, f) && R != 0.0) {
     Iz = 0.0;
     Ix = 0.0;
     return false;
@@ -183,7 +185,7 @@ __device__ inline bool csr_point(const DfcsrHistory &H, const HistAxes &A, const
   const double p1 = rx * nmx + ry * nmy;
   const double p2 = o.L.nx * c.L.tx + o.L.ny * c.L.ty;
   const double pd = -(vr_x * nab_x + vr_y * nab_y) - dens * div_v;
-  if (kRecip) {
+  if (kRecip && R != 0.0) {
     const double iR = 1.0 / R;
     const double iR2 = iR * iR;
     Iz = (num1 + num2) * iR;

@@ -47,12 +47,7 @@ __global__ __launch_bounds__(kThreads) void integrand_kernel(DfcsrHistory H, Dfc
     int i = (int)(p / nsp), k = (int)(p % nsp);
     Column c = make_column<false>(L, o, sp[k], 0.0);
     double a, b;
-    if (!csr_point<false>(H, A, o, c, xp[i], a, b)) {
-      // Outside the history every field is 0; the reference's formula then
-      // gives 0 except at R == 0 (0/0).
-      const double rx = c.ax - xp[i] * c.L.nx, ry = c.ay - xp[i] * c.L.ny;
-      if (rx == 0.0 && ry == 0.0) a = b = NAN;
-    }
+    csr_point<false>(H, A, o, c, xp[i], a, b);
     Iz[p] = a;
     Ix[p] = b;
   }

@@ -236,6 +236,10 @@ __device__ inline bool csr_point_mixed(const DfcsrHistory &H, const HistAxes &A,
   const double rx = c.ax - xp * c.npx;
   const double ry = c.ay - xp * c.npy;
   const double R2 = rx * rx + ry * ry;
+  if (R2 == 0.0) {   // the reference's 0/0 (or +-inf): not a finite contribution
+    Iz = Ix = NAN;
+    return true;
+  }
   double y = (double)rsqrtf((float)R2);
   y = y * (1.5 - 0.5 * R2 * y * y);               // one Newton step: ~1e-14 relative
   const double R = R2 * y;
@@ -337,7 +341,8 @@ struct WakeTask {
 // Observer data of one mesh point, computed once by wake_observer_kernel
 // and read by every phase of that point (kObsStride doubles):
 //   [0] s, [1] x, [2..7] lattice X0, Y0, n_x, n_y, tau_x, tau_y at s,
-//   [8] vx at the observer, [16 + 4 r] x' axis of region r (a, b, step, n),
+//   [8] vx at the observer, [9] 1 if s lies outside the lattice table (its
+//   lookups are 0 there), [16 + 4 r] x' axis of region r (a, b, step, n),
 //   [32 + 4 r] s' axis of region r.
 constexpr int kObsStride = 48;
 
@@ -375,6 +380,7 @@ __device__ inline void observer_point(const WakeTask &T, int i, double *__restri
   od[6] = o.L.tx;
   od[7] = o.L.ty;
   od[8] = o.vx;
+  od[9] = trunc_index((s - T.L.min_s) * (1.0 / T.L.ds), T.L.ns).ok ? 0.0 : 1.0;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     store_axis(od + 16 + 4 * r, rg.xa[r]);
@@ -396,6 +402,23 @@ __device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int
   phase_rows(make_plan(T.P), ph, r, j0, j1);
   const LinAxis xa = load_axis(od + 16 + 4 * r);
   const LinAxis sa = load_axis(od + 32 + 4 * r);
+  // (all branches below are uniform over the workgroup)
+  if (xa.n < 2 || sa.n < 2) {   // trapz over a single sample is 0, NaN or not
+    out_z = out_x = 0.0;
+    return;
+  }
+  if (od[9] != 0.0) {
+    // Observer and a source column both outside the lattice table: both
+    // positions and normals look up as 0, so R == 0 on every row of that
+    // column and the reference's integrand is 0/0 there -- the region's
+    // integral is NaN.  s' is monotone, so its end nodes decide.
+    const bool in0 = trunc_index((sa.at(0) - L.min_s) * inv_ds, L.ns).ok;
+    const bool in1 = trunc_index((sa.at(sa.n - 1) - L.min_s) * inv_ds, L.ns).ok;
+    if (!in0 || !in1) {
+      out_z = out_x = NAN;
+      return;
+    }
+  }
   // hull of the rows whose x' lies inside the history (x' is monotone in j);
   // a phase without one adds exactly 0 and skips its column setup
   int v0 = j1, v1 = j0;

@@ -1,9 +1,12 @@
 // a4-a6: density-function post-processing of one deposited slice.
 //
-// Replaces deposit.py:183-235.  The grids are small (<= 512^2) and the steps
-// depend on each other through global reductions (max, trapezoid sum, masked
-// mean), so the whole sequence runs as ONE workgroup of 1024 threads with
-// barriers between phases; the arrays stay in L2.
+// Replaces deposit.py:183-235 as nine launches on one stream: the
+// element-wise steps (smoothing stencils, gradients, thresholds) run over
+// the whole grid, and the global steps between them (the maxima, the
+// trapezoid normalisation, the masked mean of vx_x) read partials or run as
+// one fixed-order workgroup, so results are the same every launch.  The
+// velocity division and the normalisation are applied where their inputs
+// are read, in the reference's operation order.
 //
 // Savitzky-Golay: scipy.signal.savgol_filter(mode='interp') is linear: the
 // interior is the symmetric smoothing stencil c[] (the centre row of the
@@ -89,171 +92,304 @@ static int make_sg(int W, int order, SGOp *op) {
 
 namespace {
 
-constexpr int kThreads = 1024;
+constexpr int kBlock = 1024;   // single-workgroup reductions (fixed order)
+constexpr int kT = 256;        // grid-wide element kernels
+constexpr int kMaxGrid = 1024;
 
-// One Savitzky-Golay pass along `axis` (0: along x rows, 1: along z) of a
-// [nx][nz] array.
-__device__ void sg_pass(const SGOp &op, const double *__restrict__ in, double *__restrict__ out,
-                        int nx, int nz, int axis) {
+// Scalars passed between the launches (workspace tail).
+enum { S_THR = 0, S_DSUM = 1, S_THR2 = 2, S_FILL = 3, S_COUNT = 8 };
+
+__host__ __device__ inline int grid_of(int64_t N) {
+  int64_t g = (N + kT - 1) / kT;
+  return (int)(g < 1 ? 1 : (g > kMaxGrid ? kMaxGrid : g));
+}
+
+// vx after the velocity division (deposit.py:183-184).
+struct VelDiv {
+  const double *vx, *dens;
+  double thr;
+  __device__ double operator()(int64_t e) const {
+    const double d = dens[e];
+    return d > thr ? vx[e] / d : vx[e];
+  }
+};
+
+struct Plain {
+  const double *a;
+  __device__ double operator()(int64_t e) const { return a[e]; }
+};
+
+// Savitzky-Golay output at element e of a [nx][nz] array along `axis`
+// (0: along x rows, 1: along z); in(e) gives the input at element e.
+template <typename In>
+__device__ inline double sg_at(const SGOp &op, const In &in, int nx, int nz, int axis, int64_t e) {
+#pragma clang fp contract(off)
   const int W = op.W, h = op.h;
   const int n = axis == 0 ? nx : nz;
-  const int64_t N = (int64_t)nx * nz;
-  const int64_t stride_ax = axis == 0 ? nz : 1;
-  for (int64_t e = threadIdx.x; e < N; e += blockDim.x) {
-    int i = axis == 0 ? (int)(e / nz) : (int)(e % nz);   // position along the axis
-    const double *line = in + (e - (int64_t)i * stride_ax);   // element 0 of this line
-    double acc = 0.0;
-    if (i < h) {
-      for (int k = 0; k < W; ++k) acc += op.E[i][k] * line[k * stride_ax];
-    } else if (i >= n - h) {
-      int r = n - 1 - i;   // mirror row of the left edge
-      for (int k = 0; k < W; ++k) acc += op.E[r][W - 1 - k] * line[(int64_t)(n - W + k) * stride_ax];
-    } else {
-      for (int k = 0; k < W; ++k) acc += op.c[k] * line[(int64_t)(i - h + k) * stride_ax];
-    }
-    out[e] = acc;
+  const int64_t st = axis == 0 ? nz : 1;
+  const int i = axis == 0 ? (int)(e / nz) : (int)(e % nz);   // position along the axis
+  const int64_t e0 = e - (int64_t)i * st;                     // element 0 of this line
+  double acc = 0.0;
+  if (i < h) {
+    for (int k = 0; k < W; ++k) acc += op.E[i][k] * in(e0 + k * st);
+  } else if (i >= n - h) {
+    const int r = n - 1 - i;   // mirror row of the left edge
+    for (int k = 0; k < W; ++k) acc += op.E[r][W - 1 - k] * in(e0 + (int64_t)(n - W + k) * st);
+  } else {
+    for (int k = 0; k < W; ++k) acc += op.c[k] * in(e0 + (int64_t)(i - h + k) * st);
   }
+  return acc;
 }
 
-__device__ inline bool axis_uniform(double g0, double g1, int n) {
-  // np.gradient collapses to the scalar-spacing formula when every diff of
-  // the coordinate array equals the first one.
+// np.gradient spacing test: the coordinate diffs all equal the first one
+// (then numpy uses the scalar-spacing formula).  Evaluated on the host.
+__host__ __device__ inline bool axis_uniform(double g0, double g1, int n) {
+#pragma clang fp contract(off)
   if (n < 2) return true;
-  double d0 = linspace_at(g0, g1, n, 1) - linspace_at(g0, g1, n, 0);
-  int ok = 1;
-  for (int i = threadIdx.x; i < n - 1; i += blockDim.x)
-    if (linspace_at(g0, g1, n, i + 1) - linspace_at(g0, g1, n, i) != d0) ok = 0;
-  return __syncthreads_and(ok);
+  const double d0 = linspace_at(g0, g1, n, 1) - linspace_at(g0, g1, n, 0);
+  for (int i = 1; i < n - 1; ++i)
+    if (linspace_at(g0, g1, n, i + 1) - linspace_at(g0, g1, n, i) != d0) return false;
+  return true;
 }
 
-// np.gradient(f, g) along `axis` (edge_order 1), numpy/lib/_function_base_impl.py.
-__device__ void gradient_pass(const double *__restrict__ f, double *__restrict__ out, int nx,
-                              int nz, int axis, double g0, double g1, bool uniform) {
+// np.gradient(f, g) along `axis` at element e (edge_order 1),
+// numpy/lib/_function_base_impl.py.
+template <typename In>
+__device__ inline double gradient_at(const In &f, int nx, int nz, int axis, double g0, double g1,
+                                     bool uniform, int64_t e) {
 #pragma clang fp contract(off)
   const int n = axis == 0 ? nx : nz;
-  const int64_t N = (int64_t)nx * nz;
   const int64_t st = axis == 0 ? nz : 1;
+  const int i = axis == 0 ? (int)(e / nz) : (int)(e % nz);
   const double dxu = linspace_at(g0, g1, n, 1) - linspace_at(g0, g1, n, 0);
-  for (int64_t e = threadIdx.x; e < N; e += blockDim.x) {
-    int i = axis == 0 ? (int)(e / nz) : (int)(e % nz);
-    double r;
-    if (n < 2) {
-      r = 0.0;
-    } else if (i == 0) {
-      r = (f[e + st] - f[e]) / dxu;
-    } else if (i == n - 1) {
-      double dn = uniform ? dxu : linspace_at(g0, g1, n, n - 1) - linspace_at(g0, g1, n, n - 2);
-      r = (f[e] - f[e - st]) / dn;
-    } else if (uniform) {
-      r = (f[e + st] - f[e - st]) / (2. * dxu);
-    } else {
-      double xm = linspace_at(g0, g1, n, i - 1), x0 = linspace_at(g0, g1, n, i),
-             xp = linspace_at(g0, g1, n, i + 1);
-      double dx1 = x0 - xm, dx2 = xp - x0;
-      double a = -(dx2) / (dx1 * (dx1 + dx2));
-      double b = (dx2 - dx1) / (dx1 * dx2);
-      double c = dx1 / (dx2 * (dx1 + dx2));
-      r = a * f[e - st] + b * f[e] + c * f[e + st];
-    }
-    out[e] = r;
+  if (n < 2) return 0.0;
+  if (i == 0) return (f(e + st) - f(e)) / dxu;
+  if (i == n - 1) {
+    const double dn = uniform ? dxu : linspace_at(g0, g1, n, n - 1) - linspace_at(g0, g1, n, n - 2);
+    return (f(e) - f(e - st)) / dn;
+  }
+  if (uniform) return (f(e + st) - f(e - st)) / (2. * dxu);
+  const double xm = linspace_at(g0, g1, n, i - 1), x0 = linspace_at(g0, g1, n, i),
+               xp = linspace_at(g0, g1, n, i + 1);
+  const double dx1 = x0 - xm, dx2 = xp - x0;
+  const double a = -(dx2) / (dx1 * (dx1 + dx2));
+  const double b = (dx2 - dx1) / (dx1 * dx2);
+  const double c = dx1 / (dx2 * (dx1 + dx2));
+  return a * f(e - st) + b * f(e) + c * f(e + st);
+}
+
+// Block max of a grid-strided pass, one partial per workgroup.
+__device__ inline void max_partial(double m, double *part) {
+  __shared__ double red[kT / 64];
+  m = block_max(m, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = m;
+}
+
+__device__ inline double max_of(const double *part, int n) {
+  double m = -INFINITY;
+  for (int k = 0; k < n; ++k) m = fmax(m, part[k]);
+  return m;
+}
+
+// 1. max of the raw density (deposit.py:183)
+__global__ __launch_bounds__(kT) void df_max_kernel(const double *__restrict__ dens, int64_t N,
+                                                     double *__restrict__ part) {
+  double m = -INFINITY;
+  for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < N; e += (int64_t)gridDim.x * kT)
+    m = fmax(m, dens[e]);
+  max_partial(m, part);
+}
+
+// 2. velocity division fused into the axis-0 smoothing of density and vx
+// (deposit.py:183-184, 195-199)
+__global__ __launch_bounds__(kT) void df_sg0_kernel(SGOp op, const double *__restrict__ dens,
+                                                    const double *__restrict__ vx, int nx, int nz,
+                                                    double vt, const double *__restrict__ part,
+                                                    int npart, double *__restrict__ T0,
+                                                    double *__restrict__ T1,
+                                                    double *__restrict__ scal) {
+  const double thr = max_of(part, npart) / vt;
+  if (blockIdx.x == 0 && threadIdx.x == 0) scal[S_THR] = thr;
+  const int64_t N = (int64_t)nx * nz;
+  const VelDiv v{vx, dens, thr};
+  const Plain d{dens};
+  for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < N; e += (int64_t)gridDim.x * kT) {
+    T0[e] = sg_at(op, d, nx, nz, 0, e);
+    T1[e] = sg_at(op, v, nx, nz, 0, e);
   }
 }
 
-__global__ __launch_bounds__(kThreads) void df_kernel(
-    SGOp op, double *__restrict__ dens, double *__restrict__ vx, double *__restrict__ dxo,
-    double *__restrict__ dzo, double *__restrict__ vxxo, int nx, int nz, double x0, double x1,
-    double z0, double z1, double vt, double *__restrict__ T0, double *__restrict__ T1,
-    double *__restrict__ T2, double *__restrict__ info) {
-#pragma clang fp contract(off)
-  __shared__ double red[kThreads / 64];
+// 3. axis-1 smoothing of both into density and vx (deposit.py:195-199)
+__global__ __launch_bounds__(kT) void df_sg1_kernel(SGOp op, const double *__restrict__ T0,
+                                                    const double *__restrict__ T1, int nx, int nz,
+                                                    double *__restrict__ dens,
+                                                    double *__restrict__ vx) {
   const int64_t N = (int64_t)nx * nz;
+  const Plain a{T0}, b{T1};
+  for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < N; e += (int64_t)gridDim.x * kT) {
+    dens[e] = sg_at(op, a, nx, nz, 1, e);
+    vx[e] = sg_at(op, b, nx, nz, 1, e);
+  }
+}
+
+// 4. dsum = trapz(trapz(density, x_grids, axis=0), z_grids) (deposit.py:201):
+// the inner trapz adds the rows of each column in order (numpy reduces axis
+// 0 row by row), the outer one is a fixed-order block sum.  One workgroup.
+__global__ __launch_bounds__(kBlock) void df_dsum_kernel(const double *__restrict__ dens, int nx,
+                                                         int nz, double x0, double x1, double z0,
+                                                         double z1, double *__restrict__ colsum,
+                                                         double *__restrict__ scal) {
+#pragma clang fp contract(off)
+  __shared__ double red[kBlock / 64];
   const int tid = threadIdx.x;
-
-  // deposit.py:183-184 velocity = sum(px)/count above max/vt
-  double m = -INFINITY;
-  for (int64_t e = tid; e < N; e += kThreads) m = fmax(m, dens[e]);
-  const double dmax_raw = block_max(m, red);
-  const double thr = dmax_raw / vt;
-  for (int64_t e = tid; e < N; e += kThreads)
-    if (dens[e] > thr) vx[e] = vx[e] / dens[e];
-  __syncthreads();
-
-  // deposit.py:195-199 savgol axis 0 then axis 1
-  sg_pass(op, dens, T0, nx, nz, 0);
-  sg_pass(op, vx, T1, nx, nz, 0);
-  __syncthreads();
-  sg_pass(op, T0, dens, nx, nz, 1);
-  sg_pass(op, T1, vx, nx, nz, 1);
-  __syncthreads();
-
-  // deposit.py:201 dsum = trapz(trapz(density, x_grids, axis=0), z_grids):
-  // the inner trapz adds the rows in order (numpy reduces axis 0 row by row).
-  for (int j = tid; j < nz; j += kThreads) {
+  for (int j = tid; j < nz; j += kBlock) {
     double col = 0.0;
+#pragma unroll 8
     for (int i = 0; i + 1 < nx; ++i) {
-      double d = linspace_at(x0, x1, nx, i + 1) - linspace_at(x0, x1, nx, i);
-      double t = d * (dens[(int64_t)i * nz + j] + dens[(int64_t)(i + 1) * nz + j]) / 2.0;
+      const double d = linspace_at(x0, x1, nx, i + 1) - linspace_at(x0, x1, nx, i);
+      const double t = d * (dens[(int64_t)i * nz + j] + dens[(int64_t)(i + 1) * nz + j]) / 2.0;
       col = (i == 0) ? t : col + t;
     }
-    T2[j] = col;
+    colsum[j] = col;
   }
   __syncthreads();
   double part = 0.0;
-  for (int j = tid; j + 1 < nz; j += kThreads) {
-    double d = linspace_at(z0, z1, nz, j + 1) - linspace_at(z0, z1, nz, j);
-    part += d * (T2[j] + T2[j + 1]) / 2.0;
+  for (int j = tid; j + 1 < nz; j += kBlock) {
+    const double d = linspace_at(z0, z1, nz, j + 1) - linspace_at(z0, z1, nz, j);
+    part += d * (colsum[j] + colsum[j + 1]) / 2.0;
   }
   const double dsum = block_sum(part, red);
-  for (int64_t e = tid; e < N; e += kThreads) {
-    double d = dens[e] / dsum;
+  if (tid == 0) scal[S_DSUM] = dsum;
+}
+
+// Normalised density and its thresholded vx (deposit.py:202, 204), as read
+// by the gradients before they are stored.
+struct Normed {
+  const double *dens;
+  double dsum;
+  __device__ double operator()(int64_t e) const { return dens[e] / dsum; }
+};
+struct VxCut {
+  const double *vx, *dens;
+  double dsum, thr;
+  __device__ double operator()(int64_t e) const { return (dens[e] / dsum <= thr) ? 0.0 : vx[e]; }
+};
+
+// 5. gradients of the normalised density and of vx (deposit.py:212-213)
+__global__ __launch_bounds__(kT) void df_grad_kernel(const double *__restrict__ dens,
+                                                     const double *__restrict__ vx, int nx, int nz,
+                                                     double x0, double x1, double z0, double z1,
+                                                     int ux, int uz,
+                                                     const double *__restrict__ scal,
+                                                     double *__restrict__ G0,
+                                                     double *__restrict__ G1,
+                                                     double *__restrict__ G2) {
+  const int64_t N = (int64_t)nx * nz;
+  const double dsum = scal[S_DSUM], thr = scal[S_THR];
+  const Normed d{dens, dsum};
+  const VxCut v{vx, dens, dsum, thr};
+  for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < N; e += (int64_t)gridDim.x * kT) {
+    G0[e] = gradient_at(d, nx, nz, 0, x0, x1, ux, e);
+    G1[e] = gradient_at(d, nx, nz, 1, z0, z1, uz, e);
+    G2[e] = gradient_at(v, nx, nz, 0, x0, x1, ux, e);
+  }
+}
+
+// 6. axis-0 smoothing of the three derivatives (deposit.py:215-224); the
+// density and vx take their normalised / thresholded values in place.
+__global__ __launch_bounds__(kT) void df_sg0_grad_kernel(SGOp op, const double *__restrict__ G0,
+                                                         const double *__restrict__ G1,
+                                                         const double *__restrict__ G2, int nx,
+                                                         int nz, const double *__restrict__ scal,
+                                                         double *__restrict__ dens,
+                                                         double *__restrict__ vx,
+                                                         double *__restrict__ o0,
+                                                         double *__restrict__ o1,
+                                                         double *__restrict__ o2) {
+#pragma clang fp contract(off)
+  const int64_t N = (int64_t)nx * nz;
+  const double dsum = scal[S_DSUM], thr = scal[S_THR];
+  const Plain a{G0}, b{G1}, c{G2};
+  for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < N; e += (int64_t)gridDim.x * kT) {
+    o0[e] = sg_at(op, a, nx, nz, 0, e);
+    o1[e] = sg_at(op, b, nx, nz, 0, e);
+    o2[e] = sg_at(op, c, nx, nz, 0, e);
+    const double d = dens[e] / dsum;
     dens[e] = d;
     if (d <= thr) vx[e] = 0.0;   // raw-count threshold on the normalised density (deposit.py:204)
   }
-  __syncthreads();
+}
 
-  // deposit.py:212-213 gradients
-  const bool ux = axis_uniform(x0, x1, nx);
-  const bool uz = axis_uniform(z0, z1, nz);
-  gradient_pass(dens, T0, nx, nz, 0, x0, x1, ux);
-  gradient_pass(dens, T1, nx, nz, 1, z0, z1, uz);
-  gradient_pass(vx, T2, nx, nz, 0, x0, x1, ux);
-  __syncthreads();
+// 7. axis-1 smoothing of the derivatives; max partials of the density
+__global__ __launch_bounds__(kT) void df_sg1_grad_kernel(SGOp op, const double *__restrict__ i0,
+                                                         const double *__restrict__ i1,
+                                                         const double *__restrict__ i2, int nx,
+                                                         int nz, const double *__restrict__ dens,
+                                                         double *__restrict__ G0,
+                                                         double *__restrict__ G1,
+                                                         double *__restrict__ G2,
+                                                         double *__restrict__ part) {
+  const int64_t N = (int64_t)nx * nz;
+  const Plain a{i0}, b{i1}, c{i2};
+  double m = -INFINITY;
+  for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < N; e += (int64_t)gridDim.x * kT) {
+    G0[e] = sg_at(op, a, nx, nz, 1, e);
+    G1[e] = sg_at(op, b, nx, nz, 1, e);
+    G2[e] = sg_at(op, c, nx, nz, 1, e);
+    m = fmax(m, dens[e]);
+  }
+  max_partial(m, part);
+}
 
-  // deposit.py:215-224 smoothing of the derivatives
-  sg_pass(op, T0, dxo, nx, nz, 0);
-  sg_pass(op, T1, dzo, nx, nz, 0);
-  sg_pass(op, T2, vxxo, nx, nz, 0);
-  __syncthreads();
-  sg_pass(op, dxo, T0, nx, nz, 1);
-  sg_pass(op, dzo, T1, nx, nz, 1);
-  sg_pass(op, vxxo, T2, nx, nz, 1);
-  __syncthreads();
-
-  // deposit.py:233-235 vx_x below 8*max/vt takes the mean of vx_x above it
-  m = -INFINITY;
-  for (int64_t e = tid; e < N; e += kThreads) m = fmax(m, dens[e]);
-  const double dmax = block_max(m, red);
+// 8. vx_x below 8*max/vt takes the mean of vx_x above it (deposit.py:233-235):
+// fixed-order sums in one workgroup.
+__global__ __launch_bounds__(kBlock) void df_fill_kernel(const double *__restrict__ dens,
+                                                         const double *__restrict__ G2, int64_t N,
+                                                         double vt, const double *__restrict__ part,
+                                                         int npart, double *__restrict__ scal,
+                                                         double *__restrict__ info) {
+  __shared__ double red[kBlock / 64];
+  const double dmax = max_of(part, npart);
   const double thr2 = dmax / vt * 8;
   double s = 0.0, c = 0.0;
-  for (int64_t e = tid; e < N; e += kThreads)
+  for (int64_t e = threadIdx.x; e < N; e += kBlock)
     if (dens[e] > thr2) {
-      s += T2[e];
+      s += G2[e];
       c += 1.0;
     }
   const double ssum = block_sum(s, red);
   const double cnt = block_sum(c, red);
   const double fill = cnt > 0.0 ? ssum / cnt : NAN;   // np.mean of an empty selection
-  for (int64_t e = tid; e < N; e += kThreads) {
-    dxo[e] = T0[e];
-    dzo[e] = T1[e];
-    vxxo[e] = (dens[e] < thr2) ? fill : T2[e];
+  if (threadIdx.x == 0) {
+    scal[S_THR2] = thr2;
+    scal[S_FILL] = fill;
+    if (info) {
+      info[1] = scal[S_DSUM];
+      info[2] = dmax;
+      info[3] = fill;
+    }
   }
-  if (tid == 0 && info) {
-    info[0] = dmax_raw;
-    info[1] = dsum;
-    info[2] = dmax;
-    info[3] = fill;
+}
+
+// 9. outputs
+__global__ __launch_bounds__(kT) void df_out_kernel(const double *__restrict__ dens,
+                                                    const double *__restrict__ G0,
+                                                    const double *__restrict__ G1,
+                                                    const double *__restrict__ G2, int64_t N,
+                                                    const double *__restrict__ scal,
+                                                    double *__restrict__ dxo,
+                                                    double *__restrict__ dzo,
+                                                    double *__restrict__ vxxo) {
+  const double thr2 = scal[S_THR2], fill = scal[S_FILL];
+  for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < N; e += (int64_t)gridDim.x * kT) {
+    dxo[e] = G0[e];
+    dzo[e] = G1[e];
+    vxxo[e] = (dens[e] < thr2) ? fill : G2[e];
   }
+}
+
+__global__ void df_info_kernel(const double *__restrict__ part, int npart, double *__restrict__ info) {
+  if (threadIdx.x == 0) info[0] = max_of(part, npart);
 }
 
 }  // namespace
@@ -263,8 +399,9 @@ using namespace dfcsr;
 
 extern "C" int64_t dfcsr_df_workspace_bytes(int nbins_x, int nbins_z) {
   if (nbins_x < 1 || nbins_z < 1) return 0;
-  int64_t N = (int64_t)nbins_x * nbins_z;
-  return 3 * N * 8 + 64;
+  const int64_t N = (int64_t)nbins_x * nbins_z;
+  // three grid temporaries, column sums, two sets of max partials, scalars
+  return (3 * N + nbins_z + 2 * (int64_t)kMaxGrid + S_COUNT) * 8 + 64;
 }
 
 extern "C" int dfcsr_df_process(double *density, double *vx, double *density_x,
@@ -287,12 +424,33 @@ extern "C" int dfcsr_df_process(double *density, double *vx, double *density_x,
   SGOp op;
   int rc = make_sg(window, order, &op);
   if (rc) return rc;
-  int64_t N = (int64_t)nbins_x * nbins_z;
+  const int nx = nbins_x, nz = nbins_z;
+  const int64_t N = (int64_t)nx * nz;
   double *T0 = static_cast<double *>(ws);
-  double *T1 = T0 + N, *T2 = T1 + N;
-  hipLaunchKernelGGL(df_kernel, dim3(1), dim3(kThreads), 0, as_stream(stream), op, density, vx,
-                     density_x, density_z, vx_x, nbins_x, nbins_z, x_start, x_end, z_start,
-                     z_end, velocity_threshold, T0, T1, T2, info);
+  double *T1 = T0 + N, *T2 = T1 + N, *col = T2 + N;
+  double *partA = col + nz, *partB = partA + kMaxGrid, *scal = partB + kMaxGrid;
+  const int g = grid_of(N);
+  hipStream_t s = as_stream(stream);
+  // T0/T1: smoothed along x; density/vx: smoothed; T0,T1,T2: gradients;
+  // density_x/z, vx_x: gradients smoothed along x; T0,T1,T2: along z.
+  hipLaunchKernelGGL(df_max_kernel, dim3(g), dim3(kT), 0, s, density, N, partA);
+  hipLaunchKernelGGL(df_sg0_kernel, dim3(g), dim3(kT), 0, s, op, density, vx, nx, nz,
+                     velocity_threshold, partA, g, T0, T1, scal);
+  hipLaunchKernelGGL(df_sg1_kernel, dim3(g), dim3(kT), 0, s, op, T0, T1, nx, nz, density, vx);
+  hipLaunchKernelGGL(df_dsum_kernel, dim3(1), dim3(kBlock), 0, s, density, nx, nz, x_start,
+                     x_end, z_start, z_end, col, scal);
+  const int ux = axis_uniform(x_start, x_end, nx), uz = axis_uniform(z_start, z_end, nz);
+  hipLaunchKernelGGL(df_grad_kernel, dim3(g), dim3(kT), 0, s, density, vx, nx, nz, x_start, x_end,
+                     z_start, z_end, ux, uz, scal, T0, T1, T2);
+  hipLaunchKernelGGL(df_sg0_grad_kernel, dim3(g), dim3(kT), 0, s, op, T0, T1, T2, nx, nz, scal,
+                     density, vx, density_x, density_z, vx_x);
+  hipLaunchKernelGGL(df_sg1_grad_kernel, dim3(g), dim3(kT), 0, s, op, density_x, density_z, vx_x,
+                     nx, nz, density, T0, T1, T2, partB);
+  hipLaunchKernelGGL(df_fill_kernel, dim3(1), dim3(kBlock), 0, s, density, T2, N,
+                     velocity_threshold, partB, g, scal, info);
+  hipLaunchKernelGGL(df_out_kernel, dim3(g), dim3(kT), 0, s, density, T0, T1, T2, N, scal,
+                     density_x, density_z, vx_x);
+  if (info) hipLaunchKernelGGL(df_info_kernel, dim3(1), dim3(64), 0, s, partA, g, info);
   DFCSR_LAUNCH_CHECK("df_process");
   return DFCSR_OK;
 }

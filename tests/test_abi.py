@@ -72,3 +72,68 @@ def test_savgol_operator_rejects_bad_window():
     assert lib.dfcsr_savgol_operator(4, 2, c.ctypes.data, c.ctypes.data) == -1
     assert lib.dfcsr_savgol_operator(5, 5, c.ctypes.data, c.ctypes.data) == -1
     assert lib.dfcsr_savgol_operator(0, 0, c.ctypes.data, c.ctypes.data) == -1
+
+
+def _params(slope0, mesh=64, nxz=200):
+    return _lib.WakeParams(3.0, 1e-4, 1e-4, slope0, 0.0, 0.0, 0.7, 4.0, 1e-5, nxz, nxz, mesh, mesh,
+                           -1e-4, 1e-4, -1e-4, 1e-4, None)
+
+
+@pytest.mark.parametrize("slope0,phases", [(-1.25, 7 + 3 * 4), (0.5, 7 + 2 * 4), (2.0, 19)])
+def test_wake_workspace_follows_the_phase_plan(slope0, phases):
+    """Region 0 has 2 Nx rows in ceil(400/64) = 7 chunks, every later region
+    Nx rows in 4 chunks; 3 regions without chirp (|slope| <= 1), 4 with.  Per
+    mesh point: 2 partial sums per phase + 48 doubles of observer data."""
+    import ctypes
+    lib = _lib.load()
+    P = _params(slope0)
+    M = 64 * 64
+    assert lib.dfcsr_wake_workspace_bytes(ctypes.byref(P)) == M * (2 * phases + 48) * 8 + 256
+
+
+def test_wake_tasks_pack_host_logic():
+    """Packing is host-only: counts workgroups and partial-sum bytes per task
+    and rejects bad tasks before any device work (fake device pointers)."""
+    import ctypes
+    lib = _lib.load()
+    k = 3
+    hist = (_lib.History * k)(*[_lib.History(1, 2, 8, 0, 5, 16, 16, 0, 0, 0, 1, 1, 1)
+                                 for _ in range(k)])
+    lat = (_lib.Lattice * k)(*[_lib.Lattice(3, 100, 1, 0.0, 0.01, 4, 5) for _ in range(k)])
+    P = (_lib.WakeParams * k)(_params(-1.25, 8), _params(0.5, 16), _params(-1.25, 4))
+    out = (_lib.c_dp * k)(*[8 + j for j in range(k)])
+    nb = int(lib.dfcsr_wake_task_table_bytes(k))
+    tbl = ctypes.create_string_buffer(nb)
+    batch = _lib.WakeBatch()
+    rc = lib.dfcsr_wake_tasks_pack(k, hist, lat, P, _lib.DFCSR_PREC_MIXED, out, out, tbl, nb,
+                                   ctypes.byref(batch))
+    assert rc == 0, lib.dfcsr_last_error()
+    M = [64, 256, 16]
+    ph = [(7, 12), (7, 8), (7, 12)]
+    assert batch.tasks == k and batch.observers == sum(M)
+    assert batch.blocks_far == sum(m * a for m, (a, _) in zip(M, ph))
+    assert batch.blocks_near == sum(m * b for m, (_, b) in zip(M, ph))
+    assert batch.partial_bytes == sum(m * (2 * (a + b) + 48) * 8 for m, (a, b) in zip(M, ph))
+    assert lib.dfcsr_wake_batched_workspace_bytes(k, P) == nb + batch.partial_bytes
+    # mixed precision needs every task's FP32 history
+    hist[1].data32 = None
+    rc = lib.dfcsr_wake_tasks_pack(k, hist, lat, P, _lib.DFCSR_PREC_MIXED, out, out, tbl, nb,
+                                   ctypes.byref(batch))
+    assert rc == -1 and b"FP32" in lib.dfcsr_last_error()
+    rc = lib.dfcsr_wake_tasks_pack(k, hist, lat, P, _lib.DFCSR_PREC_FP64, out, out, tbl, 16,
+                                   ctypes.byref(batch))
+    assert rc != 0 and b"too small" in lib.dfcsr_last_error()
+
+
+def test_bench_workload_lattices():
+    """The three bench workloads: 400-step chicane, 100-step dipole, 1000-step large run."""
+    import bench
+    from pydfcsr_amd.lattice import Lattice
+    for cfg, steps in ((bench.chicane_config(), 400), (bench.dipole_config(), 100),
+                       (bench.large_config(), 1000)):
+        lat = Lattice(cfg["input_lattice"])
+        assert int(sum(lat.steps_per_element)) == steps == lat.total_steps - 1
+    big = bench.large_config()
+    assert big["input_beam"]["two_gaussian"]["ratio"] == (0.3, 0.7)
+    assert big["particle_deposition"]["filter_window"] == 9
+    assert big["particle_deposition"]["filter_order"] == 3

@@ -132,7 +132,11 @@ def test_bench_workload_lattices():
     for cfg, steps in ((bench.chicane_config(), 400), (bench.dipole_config(), 100),
                        (bench.large_config(), 1000)):
         lat = Lattice(cfg["input_lattice"])
-        assert int(sum(lat.steps_per_element)) == steps == lat.total_steps - 1
+        assert lat.total_steps - 1 == steps
+        # whole steps per element (lattice.py:152-173); an element boundary
+        # between two positions turns one step into the split partial steps
+        # of the run loop (CSR.py:227-236)
+        assert steps - 1 <= int(sum(lat.steps_per_element)) <= steps
     big = bench.large_config()
     assert big["input_beam"]["two_gaussian"]["ratio"] == (0.3, 0.7)
     assert big["particle_deposition"]["filter_window"] == 9

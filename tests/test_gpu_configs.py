@@ -107,3 +107,58 @@ def test_dipole_full_run_fp64_and_mixed_vs_oracle():
             assert report[(step, "fp64", k)][0] <= 1e-8, (step, k, report)
             assert report[(step, "mixed", k)][0] <= 1e-4, (step, k, report)
     assert check <= {k[0] for k in report}
+
+
+def test_large_run_sampled_mesh_points_vs_oracle():
+    """Full-size large configuration (1e7 particles, deposition 256 x 256 SG
+    9/3, wake mesh 128 x 128, 200 x 200 integration), 100 steps.  At the final
+    state: the density of the last slice integrates to 1 (deposit.py:201-202);
+    the FP64 and mixed wakes at 64 seeded mesh points equal the oracle's
+    C/OpenMP integral on the same history within 1e-9 / 1e-4 of max|W|."""
+    import torch
+    from bench import large_config
+    from oracle import csr_ref_c as C
+    from pydfcsr_amd import _lib, kernels
+    from pydfcsr_amd.csr import CSR2D
+
+    cfg = large_config()
+    sim = CSR2D(config=cfg, device=DEV, precision="fp64", keep_fp32=True)
+    sim.run(max_steps=100)
+    torch.cuda.synchronize()
+    tr = sim.DF_tracker
+    c = tr.log[-1]
+    d = _np(tr.raw_fields(tr.nt - 1)[0])
+    xg = np.linspace(c.x0, c.x1, c.nbx)
+    zg = np.linspace(c.z0, c.z1, c.nbz)
+    integral = np.trapz(np.trapz(d, xg, axis=0), zg)
+    assert abs(integral - 1.0) <= 1e-12, integral
+
+    sim.get_CSR_mesh()
+    H = tr.descriptor()
+    L = sim.lattice.device_tables(sim.device)
+    P = sim.wake_params()
+    fp64 = kernels.wake_mesh(H, L, P, _lib.DFCSR_PREC_FP64, sim.device, sim.ws)
+    mixed = kernels.wake_mesh(H, L, P, _lib.DFCSR_PREC_MIXED, sim.device, sim.ws,
+                              want_mesh=False)
+    torch.cuda.synchronize()
+    M = P.mesh_xbins * P.mesh_zbins
+    rows = np.sort(np.random.default_rng(5).choice(M, 64, replace=False))
+    keep = C._Keep()
+    hist = C.hist_from_records(tr.hist.cpu().numpy(), tr.head, tr.nt,
+                               (float(tr.min_x), float(tr.min_y), float(tr.min_z),
+                                float(tr.delta_x), float(tr.delta_y), float(tr.delta_z)), keep)
+    latc = C.lat_from_tables(sim.lattice, keep)
+    b = sim.beam
+    beam = dict(sigma_x=b._sigma_x, sigma_z=b._sigma_z, slope=b._slope, mean_x=b._mean_x)
+    obs = np.stack([_np(fp64["zmesh"])[rows], _np(fp64["xmesh"])[rows]], axis=1)
+    ip = cfg["CSR_integration"]
+    dE, xk = C.wake(hist, latc, P.t, beam, P.formation_length, P.n_formation_length, ip["xbins"],
+                    ip["zbins"], P.csr_scaling, obs, C.threads())
+    report = {}
+    for name, res in (("fp64", fp64), ("mixed", mixed)):
+        for k, ref in (("dE_dct", dE), ("x_kick", xk)):
+            report[(name, k)] = _max_rms(_np(res[k])[rows], ref)
+    print(f"large run, s = {P.t:.3f} m, {tr.nt} slices on {tr.Xi}x{tr.Zi}:", report, flush=True)
+    for k in ("dE_dct", "x_kick"):
+        assert report[("fp64", k)][0] <= 1e-9, report
+        assert report[("mixed", k)][0] <= 1e-4, report

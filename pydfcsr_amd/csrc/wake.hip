@@ -17,12 +17,13 @@
 //   reduction  : sum_j,k wx_j wz_k I_jk per lane, a fixed-order block tree
 //                per phase, then the phases of a mesh point in phase order:
 //                deterministic, the same every launch.
-// Launch order: region 0 (the far region) runs first, and the workgroups of
-// one XCD take the same phase for the mesh points of one z line (same x'
-// rows, the same retarded planes and nearly the same z'); the other regions
-// follow with the mesh points of one x line together (same x' rows where
-// the region follows x, overlapping z' sweeps).  Neighbouring workgroups thus
-// read the same history records through their XCD's L2.
+// Launch order: both launches run the mesh points of one z line phase by
+// phase.  Region 0 (the far region) runs first and each XCD takes a
+// contiguous range (its L2 holds what a z line shares: the same x' rows and
+// retarded planes, nearly the same z').  The near regions' z line shares
+// ~130 MB, more than an L2 but within the 256 MB last-level cache: all XCDs
+// sweep the same z line together, each taking groups of 16 neighbouring
+// workgroups.
 #include "csr_point.h"
 
 namespace dfcsr {
@@ -47,6 +48,7 @@ constexpr int kThreads = 256;
 constexpr int kChunkFar = DFCSR_WAKE_CHUNK_FAR;
 constexpr int kChunkNear = DFCSR_WAKE_CHUNK_NEAR;
 constexpr int kNumXcd = 8;
+constexpr int kNearGroup = 16;   // consecutive near-phase workgroups per XCD
 
 struct LinAxis {
   double a, b, step;
@@ -148,6 +150,16 @@ __device__ inline void mesh_point(const DfcsrWakeParams &P, int i, double &xm, d
   double pv = P.slope0 * zm;
   pv = pv + P.slope1;
   xm = xt + pv;
+}
+
+// Groups of G consecutive logical workgroups on one XCD, consecutive groups
+// on consecutive XCDs: the XCDs together sweep the logical order while each
+// L2 sees G neighbours.  Workgroups past the last full round keep their index.
+__device__ inline int xcd_group(int b, int nb, int G) {
+  const int round = kNumXcd * G;
+  if (b >= (nb / round) * round) return b;
+  const int q = b / kNumXcd, x = b % kNumXcd;
+  return ((q / G) * kNumXcd + x) * G + q % G;
 }
 
 __device__ inline int xcd_remap(int b, int nb) {
@@ -500,18 +512,14 @@ __device__ inline void run_phase_block(const WakeTask &T, int l, double *__restr
                                        double *red) {
   const int xb = T.P.mesh_xbins, zb = T.P.mesh_zbins;
   int i, ph;
-  if (!kNear) {   // z line major: the mesh points of one z line, phase by phase
-    const int per_line = T.nA * xb;
-    const int zl = l / per_line, rem = l - zl * per_line;
-    ph = rem / xb;
-    i = (rem - ph * xb) * zb + zl;
-  } else {        // x line major
-    const int per_line = T.nB * zb;
-    const int xl = l / per_line, rem = l - xl * per_line;
-    const int q = rem / zb;
-    i = xl * zb + (rem - q * zb);
-    ph = T.nA + q;
-  }
+  // z line major in both launches: the mesh points of one z line, phase by
+  // phase
+  const int np = kNear ? T.nB : T.nA;
+  const int per_line = np * xb;
+  const int zl = l / per_line, rem = l - zl * per_line;
+  const int q = rem / xb;
+  i = (rem - q * xb) * zb + zl;
+  ph = kNear ? T.nA + q : q;
   double sz, sx;
   wake_phase<kMixed>(T, partials + T.obs + (int64_t)i * kObsStride, ph, red, sz, sx);
   if (threadIdx.x == 0) {
@@ -525,7 +533,9 @@ template <bool kMixed, bool kNear>
 __global__ __launch_bounds__(kThreads, DFCSR_WAKE_MINBLOCKS) void wake_phase_kernel(
     WakeTask T, double *__restrict__ partials) {
   __shared__ double red[kThreads / 64];
-  run_phase_block<kMixed, kNear>(T, xcd_remap(blockIdx.x, gridDim.x), partials, red);
+  const int b = blockIdx.x, nb = gridDim.x;
+  run_phase_block<kMixed, kNear>(T, kNear ? xcd_group(b, nb, kNearGroup) : xcd_remap(b, nb),
+                                 partials, red);
 }
 
 // Last task whose first index (workgroup or observer) is <= b.
@@ -544,7 +554,8 @@ template <bool kMixed, bool kNear>
 __global__ __launch_bounds__(kThreads, DFCSR_WAKE_MINBLOCKS) void wake_phase_batched_kernel(
     const WakeTask *__restrict__ tasks, int ntask, double *__restrict__ partials) {
   __shared__ double red[kThreads / 64];
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int b = kNear ? xcd_group(blockIdx.x, gridDim.x, kNearGroup)
+                      : xcd_remap(blockIdx.x, gridDim.x);
   const int j = find_task(tasks, ntask, b,
                           [](const WakeTask &T) { return kNear ? T.blockB0 : T.blockA0; });
   const WakeTask &T = tasks[j];

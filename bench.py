@@ -314,7 +314,9 @@ def main():
         args.warmup = default_warmup
     cfg = make_cfg()
     t_init = time.perf_counter()
-    sim = CSR2D(config=cfg, device=dev, precision=args.precision, keep_fp32=True)
+    # both precisions' rings: the precision comparison runs on the same state
+    sim = CSR2D(config=cfg, device=dev, precision=args.precision, keep_fp32=True,
+                keep_fp64=True)
     it = sim.iter_steps()
     for i in range(args.warmup):
         next(it)

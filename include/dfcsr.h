@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define DFCSR_ABI_VERSION 2
+#define DFCSR_ABI_VERSION 3
 
 enum {
   DFCSR_OK = 0,
@@ -48,10 +48,13 @@ enum { DFCSR_F_DENSITY = 0, DFCSR_F_DENSITY_X = 1, DFCSR_F_DENSITY_Z = 2,
  * ring slot, see DfcsrHistory): the two time planes of a lookup are
  * neighbouring records, and the far-region lookups of one wavefront (many
  * retarded times, nearby z) stay within one x, z block.
- * FP32 shadow ring: records cut into 5 chunks of 16 bytes (chunk f = the
- * four corners of field f) stored z fastest, [x][slot][f][z][4], so the
- * near-region lookups of one wavefront (one time plane, consecutive z) read
- * consecutive chunks. */
+ * FP32 rings (mixed precision), two layouts of the same values:
+ *   chunked: records cut into 5 chunks of 16 bytes (chunk f = the four
+ *   corners of field f) stored z fastest, [x][slot][f][z][4], so the
+ *   near-region lookups of one wavefront (one time plane, consecutive z)
+ *   read consecutive chunks;
+ *   time-innermost: whole records [x][z][slot][20] like the FP64 ring, so a
+ *   far-region lookup (two neighbouring planes) is one 160-byte span. */
 #define DFCSR_HIST_REC64 20
 #define DFCSR_HIST_REC32 20
 #define DFCSR_HIST_CHUNKS32 5
@@ -124,8 +127,9 @@ int dfcsr_df_process(double *density, double *vx, double *density_x, double *den
  * np.linspace(xi0, xi1, Xi) x np.linspace(zi0, zi1, Zi).
  * src: raw slices, src_slot_stride doubles apart, each holding the five
  * fields as [field][nbx][nbz] (field order DFCSR_F_*).
- * jobs: HOST array of njobs descriptors; hist64/hist32 are rings of cap slots
- * in the chunked layout above (dst_slot < cap). */
+ * jobs: HOST array of njobs descriptors; hist64 (FP64), hist32 (FP32
+ * chunked) and hist32t (FP32 time-innermost) are rings of cap slots in the
+ * layouts above (dst_slot < cap); each may be NULL, but not all three. */
 typedef struct {
   int32_t src_slot;   /* raw slice index in src */
   int32_t dst_slot;   /* ring slot in the history */
@@ -134,17 +138,22 @@ typedef struct {
 } DfcsrRegridJob;
 int64_t dfcsr_regrid_workspace_bytes(int njobs);
 int dfcsr_regrid(const double *src, int64_t src_slot_stride, const DfcsrRegridJob *jobs,
-                 int njobs, double *hist64, float *hist32, int Xi, int Zi, int cap, double xi0,
-                 double xi1, double zi0, double zi1, void *ws, int64_t ws_bytes, void *stream);
+                 int njobs, double *hist64, float *hist32, float *hist32t, int Xi, int Zi,
+                 int cap, double xi0, double xi1, double zi0, double zi1, void *ws,
+                 int64_t ws_bytes, void *stream);
 
 /* ---- a11/a12: lookup primitives (exposed for parity tests) -----------------
  * History descriptor: ring of cap slots, logical slice i lives in slot
  * (head + i) % cap; axes as DF_tracker.build_interpolant (deposit.py:416-421):
  * min_t = oldest time, dt = (t_last - t_first)/(nt - 1), min_x = xi0,
- * dx = (xi1 - xi0)/(Xi - 1), same for z. */
+ * dx = (xi1 - xi0)/(Xi - 1), same for z.  The FP64 ring is needed by the
+ * lookup primitives, the integrand and FP64 wakes; a mixed-precision wake
+ * needs at least one FP32 ring (with both, its region-0 phases read the
+ * time-innermost one and the other phases the chunked one). */
 typedef struct {
-  const double *data;   /* [Xi][Zi][cap][DFCSR_HIST_REC64] */
-  const float *data32;  /* [Xi][cap][DFCSR_HIST_CHUNKS32][Zi][4] or NULL */
+  const double *data;    /* [Xi][Zi][cap][DFCSR_HIST_REC64] or NULL */
+  const float *data32;   /* [Xi][cap][DFCSR_HIST_CHUNKS32][Zi][4] or NULL */
+  const float *data32t;  /* [Xi][Zi][cap][DFCSR_HIST_REC32] or NULL */
   int32_t cap, head, nt, Xi, Zi;
   double min_t, min_x, min_z, dt, dx, dz;
 } DfcsrHistory;
@@ -196,8 +205,9 @@ int dfcsr_csr_integrand(const DfcsrHistory *hist, const DfcsrLattice *lat, doubl
  * xmesh, zmesh (M = mesh_xbins*mesh_zbins, both or neither may be NULL),
  * dE_dct, x_kick ([mesh_xbins][mesh_zbins], MeV/m).  precision:
  * DFCSR_PREC_FP64 or DFCSR_PREC_MIXED (FP32 integrand with compensated FP32
- * accumulation; FP64 retardation geometry and final sum; needs
- * hist->data32).  The workspace holds per-phase partial sums (a phase is one
+ * accumulation; FP64 retardation geometry and final sum; needs an FP32
+ * ring; vx at the observer comes from the FP64 ring when present, else
+ * from an FP32 ring).  The workspace holds per-phase partial sums (a phase is one
  * integration region and one chunk of its x' rows): its size depends on the
  * mesh and integration bins and the chirp branch of p. */
 int64_t dfcsr_wake_workspace_bytes(const DfcsrWakeParams *p);
@@ -225,6 +235,8 @@ typedef struct {
   int32_t blocks_near;    /* workgroups of the launch over the other regions */
   int32_t observers;      /* mesh points over all tasks */
   int64_t partial_bytes;  /* partial-sum buffer the launch needs */
+  int32_t precision;      /* the precision the tasks were checked for; the
+                             launch must use the same */
 } DfcsrWakeBatch;
 int64_t dfcsr_wake_task_table_bytes(int k);
 int dfcsr_wake_tasks_pack(int k, const DfcsrHistory *hist, const DfcsrLattice *lat,

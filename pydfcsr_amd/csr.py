@@ -31,15 +31,19 @@ CSR_SCALING_PER_COULOMB = 8.98755e3   # MeV/m per (C/m^2), CSR.py:80
 
 class CSR2D:
     def __init__(self, input_file=None, parallel=False, device=None, precision="fp64",
-                 config=None, verbose=False, keep_fp32=None, history_capacity=16):
+                 config=None, verbose=False, keep_fp32=None, history_capacity=16,
+                 keep_fp64=None):
         """input_file: YAML path (or a parsed dict via ``config``).  parallel is
         accepted for interface parity: the reference's MPI split over mesh
-        points is the kernel's launch grid here."""
+        points is the kernel's launch grid here.  precision "fp64" or "mixed";
+        the history keeps the rings that precision reads unless keep_fp64 /
+        keep_fp32 ask for more (e.g. to compare both precisions on one state)."""
         if device is None:
             device = "cuda"
         self.device = torch.device(device)
         self.precision = precision
         self.keep_fp32 = (precision == "mixed") if keep_fp32 is None else bool(keep_fp32)
+        self.keep_fp64 = (precision == "fp64") if keep_fp64 is None else bool(keep_fp64)
         self.history_capacity = int(history_capacity)
         self.verbose = verbose
         self.timestamp = time.strftime("%Y-%m-%dT%H_%M_%S")
@@ -66,7 +70,8 @@ class CSR2D:
         self.lattice = Lattice(config["input_lattice"], device=self.device, base_dir=base)
         self.DF_tracker = DF_tracker(config.get("particle_deposition") or {}, device=self.device,
                                      keep_fp32=self.keep_fp32,
-                                     capacity=self.history_capacity)
+                                     capacity=self.history_capacity,
+                                     keep_fp64=self.keep_fp64)
         self.integration_params = Integration_params(config.get("CSR_integration") or {})
         self.CSR_params = CSR_params(config.get("CSR_computation") or {})
 

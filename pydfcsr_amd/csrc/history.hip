@@ -12,8 +12,10 @@
 // History layout: corner-packed records (record (x, z) holds field f at
 // (x, z), (x, z1), (x1, z), (x1, z1) in elements f*4 + 0..3, x1 and z1
 // clamped to the last plane).  float64: whole records [x][z][slot][20];
-// float32 shadow: 16-byte chunks (one field's four corners each) stored
-// [x][slot][f][z], where threads of consecutive z write consecutive chunks.
+// float32 chunked: 16-byte chunks (one field's four corners each) stored
+// [x][slot][f][z], where threads of consecutive z write consecutive chunks;
+// float32 time-innermost: whole records [x][z][slot][20].  Any of the three
+// rings may be absent.
 #include "common.h"
 
 namespace dfcsr {
@@ -65,11 +67,12 @@ __device__ inline void rgi5(const DfcsrRegridJob &J, const double *__restrict__ 
 }
 
 // One thread per target record: evaluates the four corner points of the
-// record and writes the corner-packed float64 record (and float32 shadow).
+// record and writes the corner-packed record into each ring present.
 __global__ __launch_bounds__(kThreads) void regrid_kernel(
     const double *__restrict__ src, int64_t slot_stride, const DfcsrRegridJob *__restrict__ jobs,
-    const double *__restrict__ means, double *__restrict__ h64, float *__restrict__ h32, int Xi,
-    int Zi, int cap, double xi0, double xi1, double zi0, double zi1) {
+    const double *__restrict__ means, double *__restrict__ h64, float *__restrict__ h32,
+    float *__restrict__ h32t, int Xi, int Zi, int cap, double xi0, double xi1, double zi0,
+    double zi1) {
   const DfcsrRegridJob J = jobs[blockIdx.y];
   const int64_t cells = (int64_t)Xi * Zi;
   const double *base = src + (int64_t)J.src_slot * slot_stride;
@@ -85,11 +88,19 @@ __global__ __launch_bounds__(kThreads) void regrid_kernel(
     rgi5(J, base, fill_vxx, x0, z1, v[1]);
     rgi5(J, base, fill_vxx, x1, z0, v[2]);
     rgi5(J, base, fill_vxx, x1, z1, v[3]);
-    double2 *d2 = reinterpret_cast<double2 *>(h64 + (c * cap + J.dst_slot) * DFCSR_HIST_REC64);
+    if (h64) {
+      double2 *d2 = reinterpret_cast<double2 *>(h64 + (c * cap + J.dst_slot) * DFCSR_HIST_REC64);
 #pragma unroll
-    for (int f = 0; f < 5; ++f) {
-      d2[2 * f] = make_double2(v[0][f], v[1][f]);
-      d2[2 * f + 1] = make_double2(v[2][f], v[3][f]);
+      for (int f = 0; f < 5; ++f) {
+        d2[2 * f] = make_double2(v[0][f], v[1][f]);
+        d2[2 * f + 1] = make_double2(v[2][f], v[3][f]);
+      }
+    }
+    if (h32t) {
+      float4 *t4 = reinterpret_cast<float4 *>(h32t + (c * cap + J.dst_slot) * DFCSR_HIST_REC32);
+#pragma unroll
+      for (int f = 0; f < 5; ++f)
+        t4[f] = make_float4((float)v[0][f], (float)v[1][f], (float)v[2][f], (float)v[3][f]);
     }
     if (h32) {
       const int64_t blk = (int64_t)ix * cap + J.dst_slot;
@@ -113,10 +124,11 @@ extern "C" int64_t dfcsr_regrid_workspace_bytes(int njobs) {
 }
 
 extern "C" int dfcsr_regrid(const double *src, int64_t src_slot_stride, const DfcsrRegridJob *jobs,
-                            int njobs, double *hist64, float *hist32, int Xi, int Zi, int cap,
-                            double xi0, double xi1, double zi0, double zi1, void *ws,
-                            int64_t ws_bytes, void *stream) {
-  if (!src || !jobs || !hist64 || !ws) return set_error(DFCSR_EINVAL, "regrid: NULL pointer");
+                            int njobs, double *hist64, float *hist32, float *hist32t, int Xi,
+                            int Zi, int cap, double xi0, double xi1, double zi0, double zi1,
+                            void *ws, int64_t ws_bytes, void *stream) {
+  if (!src || !jobs || !ws) return set_error(DFCSR_EINVAL, "regrid: NULL pointer");
+  if (!hist64 && !hist32 && !hist32t) return set_error(DFCSR_EINVAL, "regrid: no history ring");
   if (njobs < 1) return DFCSR_OK;
   if (njobs > kMaxJobs) return set_error(DFCSR_ERANGE, "regrid: %d jobs > %d", njobs, kMaxJobs);
   if (Xi < 2 || Zi < 2) return set_error(DFCSR_EINVAL, "regrid: interp grid %d x %d", Xi, Zi);
@@ -144,8 +156,8 @@ extern "C" int dfcsr_regrid(const double *src, int64_t src_slot_stride, const Df
   int gx = (int)((cells + kThreads - 1) / kThreads);
   if (gx > 2048) gx = 2048;
   hipLaunchKernelGGL(regrid_kernel, dim3(gx, njobs), dim3(kThreads), 0, s, src, src_slot_stride,
-                     (const DfcsrRegridJob *)djobs, (const double *)means, hist64, hist32, Xi, Zi,
-                     cap, xi0, xi1, zi0, zi1);
+                     (const DfcsrRegridJob *)djobs, (const double *)means, hist64, hist32, hist32t,
+                     Xi, Zi, cap, xi0, xi1, zi0, zi1);
   DFCSR_LAUNCH_CHECK("regrid");
   return DFCSR_OK;
 }

@@ -63,7 +63,9 @@ int grid_for(int64_t n) {
 }  // namespace
 
 int check_history(const DfcsrHistory *H, const char *who) {
-  if (!H || !H->data) return set_error(DFCSR_EINVAL, "%s: NULL history", who);
+  if (!H) return set_error(DFCSR_EINVAL, "%s: NULL history", who);
+  if (!H->data && !H->data32 && !H->data32t)
+    return set_error(DFCSR_EINVAL, "%s: history without a ring", who);
   if (H->nt < 2)
     return set_error(DFCSR_EINVAL,
                      "%s: history holds %d slice(s); the time axis needs two "
@@ -118,6 +120,7 @@ extern "C" int dfcsr_csr_integrand(const DfcsrHistory *hist, const DfcsrLattice 
                                    int nsp, double *Iz, double *Ix, void *stream) {
   int rc = check_history(hist, "csr_integrand");
   if (rc) return rc;
+  if (!hist->data) return set_error(DFCSR_EINVAL, "csr_integrand: needs the FP64 ring");
   if ((rc = check_lattice(lat, "csr_integrand"))) return rc;
   if (!xp || !sp || !Iz || !Ix) return set_error(DFCSR_EINVAL, "csr_integrand: NULL pointer");
   if (nxp < 1 || nsp < 1) return set_error(DFCSR_EINVAL, "csr_integrand: empty grid");

@@ -224,11 +224,43 @@ __device__ inline ColumnMixed make_column_mixed(const DfcsrLattice &L, const Obs
   return m;
 }
 
-// Chunk 0 (field 0's four corners) of the FP32 record (x, slot of ti, z);
-// field f's chunk is f * Zi float4 further on.
+// Chunk 0 (field 0's four corners) of the FP32 record (x, slot of ti, z)
+// in the chunked ring; field f's chunk is f * Zi float4 further on.
 __device__ inline const float4 *hist_rec32(const DfcsrHistory &H, int ti, int xi, int zi) {
   const int64_t blk = ((int64_t)xi * H.cap + hist_slot(H, ti)) * DFCSR_HIST_CHUNKS32;
   return reinterpret_cast<const float4 *>(H.data32) + blk * H.Zi + zi;
+}
+
+// The same record in the time-innermost ring: five consecutive chunks.
+__device__ inline const float4 *hist_rec32t(const DfcsrHistory &H, int ti, int xi, int zi) {
+  const int64_t rec = ((int64_t)xi * H.Zi + zi) * H.cap + hist_slot(H, ti);
+  return reinterpret_cast<const float4 *>(H.data32t) + rec * DFCSR_HIST_CHUNKS32;
+}
+
+// vx at (tq, xq, zq) from an FP32 ring (the time-innermost one when present),
+// with interpolate3D's truncation and validity rules and its lerp order.
+__device__ inline double lookup_vx32(const DfcsrHistory &H, const HistAxes &A, double tq,
+                                     double xq, double zq) {
+  const TruncIdx T = trunc_index((tq - A.min_t) * A.it, H.nt);
+  const TruncIdx X = trunc_index((xq - A.min_x) * A.ix, H.Xi);
+  const TruncIdx Z = trunc_index((zq - A.min_z) * A.iz, H.Zi);
+  if (!(T.ok && X.ok && Z.ok)) return 0.0;
+  float4 a, b;
+  if (H.data32t) {
+    a = hist_rec32t(H, T.i0, X.i0, Z.i0)[DFCSR_F_VX];
+    b = hist_rec32t(H, T.i1, X.i0, Z.i0)[DFCSR_F_VX];
+  } else {
+    a = hist_rec32(H, T.i0, X.i0, Z.i0)[(int64_t)DFCSR_F_VX * H.Zi];
+    b = hist_rec32(H, T.i1, X.i0, Z.i0)[(int64_t)DFCSR_F_VX * H.Zi];
+  }
+  const double td = T.frac, xd = X.frac, zd = Z.frac;
+  const double c00 = (double)a.x * (1 - td) + (double)b.x * td;
+  const double c01 = (double)a.y * (1 - td) + (double)b.y * td;
+  const double c10 = (double)a.z * (1 - td) + (double)b.z * td;
+  const double c11 = (double)a.w * (1 - td) + (double)b.w * td;
+  const double c0 = c00 * (1 - xd) + c10 * xd;
+  const double c1 = c01 * (1 - xd) + c11 * xd;
+  return c0 * (1 - zd) + c1 * zd;
 }
 
 struct Neumaier {
@@ -242,6 +274,8 @@ struct Neumaier {
 };
 
 // One source point in mixed precision; false when outside the history.
+// kTI: read the time-innermost FP32 ring, else the chunked one.
+template <bool kTI>
 __device__ inline bool csr_point_mixed(const DfcsrHistory &H, const HistAxes &A, double t,
                                        const ColumnMixed &c, double xp, int xi0, float xd,
                                        float &Iz, float &Ix) {
@@ -262,9 +296,9 @@ __device__ inline bool csr_point_mixed(const DfcsrHistory &H, const HistAxes &A,
   if (!(T.ok && Z.ok)) return false;
   const float td = (float)T.frac, zd = (float)Z.frac;
   // one corner-packed record per time plane, one 16-byte chunk per field
-  const float4 *q0 = hist_rec32(H, T.i0, xi0, Z.i0);
-  const float4 *q1 = hist_rec32(H, T.i1, xi0, Z.i0);
-  const int64_t cs = H.Zi;
+  const float4 *q0 = kTI ? hist_rec32t(H, T.i0, xi0, Z.i0) : hist_rec32(H, T.i0, xi0, Z.i0);
+  const float4 *q1 = kTI ? hist_rec32t(H, T.i1, xi0, Z.i0) : hist_rec32(H, T.i1, xi0, Z.i0);
+  const int64_t cs = kTI ? 1 : H.Zi;
   float f[5];
 #pragma unroll
   for (int k = 0; k < 5; ++k) {
@@ -380,7 +414,16 @@ __device__ inline void observer_point(const WakeTask &T, int i, double *__restri
   const double t = T.P.t;
   const double s = t + zm, x = xm;
   const HistAxes A = make_axes(T.H);
-  const Observer o = make_observer<true>(T.H, A, T.L, s, x, t);
+  Observer o;
+  if (T.H.data) {
+    o = make_observer<true>(T.H, A, T.L, s, x, t);
+  } else {   // FP32 rings only (mixed precision)
+    o.s = s;
+    o.x = x;
+    o.t = t;
+    o.L = lattice_at<true>(T.L, s, 1.0 / T.L.ds);
+    o.vx = lookup_vx32(T.H, A, t, x, s - t);
+  }
   Regions rg = {};
   make_regions(T.P, s, x, t, rg);
   od[0] = s;
@@ -402,7 +445,8 @@ __device__ inline void observer_point(const WakeTask &T, int i, double *__restri
 
 // Integrates phase ph of the mesh point whose observer data is od; every
 // thread returns the block sums of wz_k wx_j (I_z, I_x) over the phase.
-template <bool kMixed>
+// kTI (mixed only): read the time-innermost FP32 ring.
+template <bool kMixed, bool kTI>
 __device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int ph, double *red,
                            double &out_z, double &out_x) {
   const DfcsrHistory &H = T.H;
@@ -477,7 +521,7 @@ __device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int
         const TruncIdx X = trunc_index((xp - A.min_x) * A.ix, H.Xi);
         if (!X.ok) continue;
         float Iz, Ix;
-        if (!csr_point_mixed(H, A, t, col, xp, X.i0, (float)X.frac, Iz, Ix)) continue;
+        if (!csr_point_mixed<kTI>(H, A, t, col, xp, X.i0, (float)X.frac, Iz, Ix)) continue;
         const float wx = (float)xa.w(j);
         cz.add(wx * Iz);
         cx.add(wx * Ix);
@@ -506,8 +550,9 @@ __device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int
 }
 
 // Workgroup l of a task's far (kNear = false) or near launch -> (mesh
-// point, phase); writes the phase's partial sums.
-template <bool kMixed, bool kNear>
+// point, phase); writes the phase's partial sums.  kTI (mixed only): the
+// phases read the time-innermost FP32 ring, else the chunked one.
+template <bool kMixed, bool kNear, bool kTI>
 __device__ inline void run_phase_block(const WakeTask &T, int l, double *__restrict__ partials,
                                        double *red) {
   const int xb = T.P.mesh_xbins, zb = T.P.mesh_zbins;
@@ -521,7 +566,7 @@ __device__ inline void run_phase_block(const WakeTask &T, int l, double *__restr
   i = (rem - q * xb) * zb + zl;
   ph = kNear ? T.nA + q : q;
   double sz, sx;
-  wake_phase<kMixed>(T, partials + T.obs + (int64_t)i * kObsStride, ph, red, sz, sx);
+  wake_phase<kMixed, kTI>(T, partials + T.obs + (int64_t)i * kObsStride, ph, red, sz, sx);
   if (threadIdx.x == 0) {
     double *p = partials + T.part + ((int64_t)i * (T.nA + T.nB) + ph) * 2;
     p[0] = sz;
@@ -529,13 +574,20 @@ __device__ inline void run_phase_block(const WakeTask &T, int l, double *__restr
   }
 }
 
-template <bool kMixed, bool kNear>
+// The FP32 ring a mixed launch reads: region 0 (many retarded planes per
+// wavefront) the time-innermost one, the later regions (one plane,
+// consecutive z') the chunked one, each when present.
+__host__ __device__ inline bool mixed_reads_ti(const DfcsrHistory &H, bool near) {
+  return near ? (H.data32 == nullptr) : (H.data32t != nullptr);
+}
+
+template <bool kMixed, bool kNear, bool kTI>
 __global__ __launch_bounds__(kThreads, DFCSR_WAKE_MINBLOCKS) void wake_phase_kernel(
     WakeTask T, double *__restrict__ partials) {
   __shared__ double red[kThreads / 64];
   const int b = blockIdx.x, nb = gridDim.x;
-  run_phase_block<kMixed, kNear>(T, kNear ? xcd_group(b, nb, kNearGroup) : xcd_remap(b, nb),
-                                 partials, red);
+  run_phase_block<kMixed, kNear, kTI>(
+      T, kNear ? xcd_group(b, nb, kNearGroup) : xcd_remap(b, nb), partials, red);
 }
 
 // Last task whose first index (workgroup or observer) is <= b.
@@ -559,7 +611,15 @@ __global__ __launch_bounds__(kThreads, DFCSR_WAKE_MINBLOCKS) void wake_phase_bat
   const int j = find_task(tasks, ntask, b,
                           [](const WakeTask &T) { return kNear ? T.blockB0 : T.blockA0; });
   const WakeTask &T = tasks[j];
-  run_phase_block<kMixed, kNear>(T, b - (kNear ? T.blockB0 : T.blockA0), partials, red);
+  const int l = b - (kNear ? T.blockB0 : T.blockA0);
+  if constexpr (kMixed) {   // per task: tasks may hold different FP32 rings
+    if (mixed_reads_ti(T.H, kNear))
+      run_phase_block<true, kNear, true>(T, l, partials, red);
+    else
+      run_phase_block<true, kNear, false>(T, l, partials, red);
+  } else {
+    run_phase_block<false, kNear, false>(T, l, partials, red);
+  }
 }
 
 __global__ __launch_bounds__(kThreads) void wake_observer_kernel(WakeTask T,
@@ -620,6 +680,16 @@ int check_params(const DfcsrWakeParams *p, const char *who) {
   if ((int64_t)p->mesh_xbins * p->mesh_zbins > (int64_t)1 << 24)
     return set_error(DFCSR_ERANGE, "%s: mesh too large", who);
 
+  return DFCSR_OK;
+}
+
+// The rings a wake of this precision reads: FP64 the float64 ring, mixed
+// one of the float32 rings.
+int check_rings(const DfcsrHistory &H, int precision, const char *who) {
+  if (precision == DFCSR_PREC_FP64 && !H.data)
+    return set_error(DFCSR_EINVAL, "%s: FP64 precision needs the FP64 history ring", who);
+  if (precision == DFCSR_PREC_MIXED && !H.data32 && !H.data32t)
+    return set_error(DFCSR_EINVAL, "%s: mixed precision needs an FP32 history ring", who);
   return DFCSR_OK;
 }
 
@@ -687,8 +757,7 @@ extern "C" int dfcsr_wake_mesh(const DfcsrHistory *hist, const DfcsrLattice *lat
     return set_error(DFCSR_EINVAL, "wake_mesh: xmesh and zmesh must both be given or both NULL");
   if (precision != DFCSR_PREC_FP64 && precision != DFCSR_PREC_MIXED)
     return set_error(DFCSR_EINVAL, "wake_mesh: unknown precision %d", precision);
-  if (precision == DFCSR_PREC_MIXED && !hist->data32)
-    return set_error(DFCSR_EINVAL, "wake_mesh: mixed precision needs the FP32 history");
+  if ((rc = check_rings(*hist, precision, "wake_mesh"))) return rc;
   if (!ws || ws_bytes < dfcsr_wake_workspace_bytes(p))
     return set_error(DFCSR_ENOWS, "wake_mesh: workspace of %lld bytes, %lld needed",
                      (long long)ws_bytes, (long long)dfcsr_wake_workspace_bytes(p));
@@ -699,11 +768,17 @@ extern "C" int dfcsr_wake_mesh(const DfcsrHistory *hist, const DfcsrLattice *lat
   const dim3 ga(T.M * T.nA), gb(T.M * T.nB), gf((T.M + kThreads - 1) / kThreads);
   hipLaunchKernelGGL(wake_observer_kernel, gf, dim3(kThreads), 0, s, T, part);
   if (precision == DFCSR_PREC_MIXED) {
-    hipLaunchKernelGGL((wake_phase_kernel<true, false>), ga, dim3(kThreads), 0, s, T, part);
-    hipLaunchKernelGGL((wake_phase_kernel<true, true>), gb, dim3(kThreads), 0, s, T, part);
+    if (mixed_reads_ti(*hist, false))
+      hipLaunchKernelGGL((wake_phase_kernel<true, false, true>), ga, dim3(kThreads), 0, s, T, part);
+    else
+      hipLaunchKernelGGL((wake_phase_kernel<true, false, false>), ga, dim3(kThreads), 0, s, T, part);
+    if (mixed_reads_ti(*hist, true))
+      hipLaunchKernelGGL((wake_phase_kernel<true, true, true>), gb, dim3(kThreads), 0, s, T, part);
+    else
+      hipLaunchKernelGGL((wake_phase_kernel<true, true, false>), gb, dim3(kThreads), 0, s, T, part);
   } else {
-    hipLaunchKernelGGL((wake_phase_kernel<false, false>), ga, dim3(kThreads), 0, s, T, part);
-    hipLaunchKernelGGL((wake_phase_kernel<false, true>), gb, dim3(kThreads), 0, s, T, part);
+    hipLaunchKernelGGL((wake_phase_kernel<false, false, false>), ga, dim3(kThreads), 0, s, T, part);
+    hipLaunchKernelGGL((wake_phase_kernel<false, true, false>), gb, dim3(kThreads), 0, s, T, part);
   }
   hipLaunchKernelGGL(wake_finalize_kernel, gf, dim3(kThreads), 0, s, T, (const double *)part);
   DFCSR_LAUNCH_CHECK("wake_mesh");
@@ -739,8 +814,7 @@ extern "C" int dfcsr_wake_tasks_pack(int k, const DfcsrHistory *hist, const Dfcs
     if ((rc = check_lattice(&lat[j], "wake_tasks_pack"))) return rc;
     if ((rc = check_params(&p[j], "wake_tasks_pack"))) return rc;
     if (!dE_dct[j] || !x_kick[j]) return set_error(DFCSR_EINVAL, "wake_tasks_pack: NULL output");
-    if (precision == DFCSR_PREC_MIXED && !hist[j].data32)
-      return set_error(DFCSR_EINVAL, "wake_tasks_pack: task %d lacks the FP32 history", j);
+    if ((rc = check_rings(hist[j], precision, "wake_tasks_pack"))) return rc;
     WakeTask &T = th[j];
     if ((rc = make_task(hist[j], lat[j], p[j], dE_dct[j], x_kick[j], nullptr, nullptr, T,
                         "wake_tasks_pack")))
@@ -762,6 +836,7 @@ extern "C" int dfcsr_wake_tasks_pack(int k, const DfcsrHistory *hist, const Dfcs
   batch->blocks_near = (int32_t)nb;
   batch->observers = (int32_t)nobs;
   batch->partial_bytes = part * (int64_t)sizeof(double);
+  batch->precision = precision;
   return DFCSR_OK;
 }
 
@@ -773,6 +848,9 @@ extern "C" int dfcsr_wake_batched_launch(const void *dev_tasks, const DfcsrWakeB
     return set_error(DFCSR_EINVAL, "wake_batched_launch: bad arguments");
   if (!partials || partials_bytes < batch->partial_bytes)
     return set_error(DFCSR_ENOWS, "wake_batched_launch: partial-sum buffer too small");
+  if (precision != batch->precision)
+    return set_error(DFCSR_EINVAL, "wake_batched_launch: tasks packed for precision %d, not %d",
+                     batch->precision, precision);
   hipStream_t s = as_stream(stream);
   const WakeTask *dt = static_cast<const WakeTask *>(dev_tasks);
   double *part = static_cast<double *>(partials);

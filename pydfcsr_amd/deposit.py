@@ -51,7 +51,10 @@ class SliceMeta:
 
 
 class DF_tracker:
-    def __init__(self, input_dic=None, device="cuda", keep_fp32=True, capacity=16):
+    def __init__(self, input_dic=None, device="cuda", keep_fp32=True, capacity=16,
+                 keep_fp64=True):
+        """keep_fp64: hold the float64 history ring (FP64 wakes, lookups);
+        keep_fp32: hold the two float32 rings (mixed-precision wakes)."""
         self.params = Deposition_params(input_dic)
         p = self.params
         self.xbins, self.zbins, self.xlim, self.zlim = p.xbins, p.zbins, p.xlim, p.zlim
@@ -59,6 +62,9 @@ class DF_tracker:
         self.velocity_threhold, self.upper_limit = p.velocity_threhold, p.upper_limit
         self.device = torch.device(device)
         self.keep_fp32 = keep_fp32
+        self.keep_fp64 = keep_fp64
+        if not (keep_fp32 or keep_fp64):
+            raise DfcsrError("DF_tracker needs the float64 or the float32 history rings")
         self.ws = _lib.Workspace(self.device)
         self.nbmax = max(int(self.xbins), int(self.zbins), 100)
         self.cap = 0
@@ -71,8 +77,9 @@ class DF_tracker:
         self.Xi = self.Zi = 0
         self.xi = self.zi = None          # (start, stop) of the interp grid
         self.raw = None
-        self.hist = None
-        self.hist32 = None
+        self.hist = None      # float64 (Xi, Zi, cap, 20)
+        self.hist32 = None    # float32 chunked (Xi, cap, 5, Zi, 4)
+        self.hist32t = None   # float32 time-innermost (Xi, Zi, cap, 20)
         self.cur = None
         self.n_regrid = 0
         self.info = torch.zeros(4, dtype=torch.float64, device=self.device)
@@ -94,9 +101,10 @@ class DF_tracker:
             return
         new_cap = max(need, 2 * self.cap, 16)
         raw = torch.empty((new_cap, self._raw_stride()), dtype=torch.float64, device=self.device)
-        hist = hist32 = None
+        rings = (None, None, None)
         if self.Xi:
-            hist, hist32 = self._rings(self.Xi, self.Zi, new_cap)
+            rings = self._rings(self.Xi, self.Zi, new_cap)
+        hist, hist32, hist32t = rings
         if self.cap:
             # keep the ring's logical order, pending slot included (index nt)
             for i in range(min(self.nt + 1, self.cap)):
@@ -104,26 +112,30 @@ class DF_tracker:
                 raw[i].copy_(self.raw[s])
                 if hist is not None and self.hist is not None:
                     hist[:, :, i].copy_(self.hist[:, :, s])
-                    if hist32 is not None and self.hist32 is not None:
-                        hist32[:, i].copy_(self.hist32[:, s])
-        self.raw, self.hist, self.hist32 = raw, hist, hist32
+                if hist32t is not None and self.hist32t is not None:
+                    hist32t[:, :, i].copy_(self.hist32t[:, :, s])
+                if hist32 is not None and self.hist32 is not None:
+                    hist32[:, i].copy_(self.hist32[:, s])
+        self.raw, self.hist, self.hist32, self.hist32t = raw, hist, hist32, hist32t
         self.cap = new_cap
         self.head = 0
 
     def _rings(self, Xi, Zi, cap):
-        hist = torch.empty((Xi, Zi, cap, _lib.HIST_REC64), dtype=torch.float64,
-                           device=self.device)
-        hist32 = None
+        hist = hist32 = hist32t = None
+        if self.keep_fp64:
+            hist = torch.empty((Xi, Zi, cap, _lib.HIST_REC64), dtype=torch.float64,
+                               device=self.device)
         if self.keep_fp32:
             hist32 = torch.empty((Xi, cap, _lib.HIST_CHUNKS32, Zi, 4), dtype=torch.float32,
                                  device=self.device)
-        return hist, hist32
+            hist32t = torch.empty((Xi, Zi, cap, _lib.HIST_REC32), dtype=torch.float32,
+                                  device=self.device)
+        return hist, hist32, hist32t
 
     def _alloc_hist(self, Xi, Zi):
-        if self.hist is None or self.Xi != Xi or self.Zi != Zi:
-            self.hist = None
-            self.hist32 = None
-            self.hist, self.hist32 = self._rings(Xi, Zi, self.cap)
+        if self.Xi != Xi or self.Zi != Zi or (self.hist is None and self.hist32 is None):
+            self.hist = self.hist32 = self.hist32t = None   # free before allocating
+            self.hist, self.hist32, self.hist32t = self._rings(Xi, Zi, self.cap)
         self.Xi, self.Zi = Xi, Zi
 
     def raw_fields(self, i=None):
@@ -222,7 +234,7 @@ class DF_tracker:
         for k in range(0, len(jobs), 4096):
             kernels.regrid(self.raw, self._raw_stride(), jobs[k:k + 4096], self.hist, self.hist32,
                            self.Xi, self.Zi, self.xi[0], self.xi[1], self.zi[0], self.zi[1],
-                           self.ws)
+                           self.ws, hist32t=self.hist32t)
 
     def build_interpolant(self):   # deposit.py:395-426
         nt = self.nt
@@ -236,23 +248,26 @@ class DF_tracker:
 
     def descriptor(self):
         """ctypes history descriptor for the kernels (after build_interpolant)."""
-        return _lib.History(self.hist.data_ptr(),
-                            self.hist32.data_ptr() if self.hist32 is not None else None,
-                            self.cap, self.head, self.nt, self.Xi, self.Zi, float(self.min_x),
-                            float(self.min_y), float(self.min_z), float(self.delta_x),
-                            float(self.delta_y), float(self.delta_z))
+        def p(t):
+            return t.data_ptr() if t is not None else None
+        return _lib.History(p(self.hist), p(self.hist32), p(self.hist32t), self.cap, self.head,
+                            self.nt, self.Xi, self.Zi, float(self.min_x), float(self.min_y),
+                            float(self.min_z), float(self.delta_x), float(self.delta_y),
+                            float(self.delta_z))
 
     def history_fields(self):
         """Stacked (nt, Xi, Zi) copies of the five fields in logical order
         (the reference's data_*_interp arrays; for inspection and tests)."""
         idx = torch.tensor([self._slot(i) for i in range(self.nt)], device=self.device)
-        h = self.hist[..., 0::4].index_select(2, idx)   # corner (x, z) of each field
+        ring = self.hist if self.hist is not None else self.hist32t
+        h = ring[..., 0::4].index_select(2, idx).double()   # corner (x, z) of each field
         return {f: h[..., k].permute(2, 0, 1).contiguous() for k, f in enumerate(_lib.FIELDS)}
 
     def footprint_bytes(self):
         b = self.raw.numel() * 8 if self.raw is not None else 0
         if self.hist is not None:
             b += self.hist.numel() * 8
-        if self.hist32 is not None:
-            b += self.hist32.numel() * 4
+        for r in (self.hist32, self.hist32t):
+            if r is not None:
+                b += r.numel() * 4
         return b

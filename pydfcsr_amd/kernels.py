@@ -121,22 +121,36 @@ def savgol_operator(window, order):
 
 
 def regrid(src: torch.Tensor, slot_stride: int, jobs, hist64, hist32, Xi, Zi, xi0, xi1, zi0, zi1,
-           ws: _lib.Workspace):
-    """jobs: list of (src_slot, dst_slot, nbx, nbz, x0, x1, z0, z1); hist64 is the
-    (Xi, Zi, cap, 20) float64 ring (hist32 the (Xi, cap, 5, Zi, 4) float32 one or None)."""
+           ws: _lib.Workspace, hist32t=None):
+    """jobs: list of (src_slot, dst_slot, nbx, nbz, x0, x1, z0, z1).  Rings (any
+    may be None, not all): hist64 float64 (Xi, Zi, cap, 20), hist32 float32
+    chunked (Xi, cap, 5, Zi, 4), hist32t float32 time-innermost (Xi, Zi, cap, 20)."""
     lib = _lib.load()
     n = len(jobs)
     if n == 0:
         return
     arr = (_lib.RegridJob * n)(*[_lib.RegridJob(*j) for j in jobs])
     b = ws.get("regrid", lib.dfcsr_regrid_workspace_bytes(n))
-    cap = int(hist64.shape[2])
-    if (tuple(hist64.shape) != (Xi, Zi, cap, _lib.HIST_REC64) or
-            (hist32 is not None and tuple(hist32.shape) != (Xi, cap, _lib.HIST_CHUNKS32, Zi, 4))):
-        raise DfcsrError(f"regrid: history rings {tuple(hist64.shape)} do not match {Xi} x {Zi}")
-    check(lib.dfcsr_regrid(ptr(src), int(slot_stride), arr, n, ptr(hist64), ptr(hist32), int(Xi),
-                           int(Zi), cap, float(xi0), float(xi1), float(zi0), float(zi1), ptr(b),
-                           b.numel(), stream_handle(src.device)), "regrid")
+    rings = [r for r in (hist64, hist32t) if r is not None]
+    if rings:
+        cap = int(rings[0].shape[2])
+    elif hist32 is not None:
+        cap = int(hist32.shape[1])
+    else:
+        raise DfcsrError("regrid: no history ring")
+    want = {id(hist64): ((Xi, Zi, cap, _lib.HIST_REC64), torch.float64),
+            id(hist32t): ((Xi, Zi, cap, _lib.HIST_REC32), torch.float32),
+            id(hist32): ((Xi, cap, _lib.HIST_CHUNKS32, Zi, 4), torch.float32)}
+    for r in (hist64, hist32, hist32t):
+        if r is None:
+            continue
+        shape, dtype = want[id(r)]
+        if tuple(r.shape) != shape or r.dtype != dtype or not r.is_contiguous():
+            raise DfcsrError(f"regrid: history ring {tuple(r.shape)} {r.dtype} does not match "
+                             f"{shape} {dtype}")
+    check(lib.dfcsr_regrid(ptr(src), int(slot_stride), arr, n, ptr(hist64), ptr(hist32),
+                           ptr(hist32t), int(Xi), int(Zi), cap, float(xi0), float(xi1), float(zi0),
+                           float(zi1), ptr(b), b.numel(), stream_handle(src.device)), "regrid")
 
 
 def interpolate3D(hist: _lib.History, field: int, q: torch.Tensor, device):

@@ -65,7 +65,9 @@ def request_footprint(config: dict, steps: int, precision: str = "mixed") -> int
     ul = dep.get("upper_limit")
     grid = int(ul) if isinstance(ul, int) else 2000
     slots = steps + 2
-    rec = 160 + (80 if precision == "mixed" else 0)   # corner-packed records
+    # corner-packed records: 160 B in the float64 ring, or 80 B in each of the
+    # two float32 rings a mixed-precision simulation keeps instead
+    rec = 160
     hist = slots * grid * grid * rec
     raw = slots * 5 * nb * nb * 8
     particles = n * 8 * (6 * 2 + 4)           # coordinates, tracking temporary, scratch

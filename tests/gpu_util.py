@@ -27,8 +27,9 @@ def corner_records(data, rec_len=20):
 
 
 def ring64(records, cap, head):
-    """Corner-packed float64 records (nt, Xi, Zi, 20) -> the device ring
-    (Xi, Zi, cap, 20), logical slice i in slot (head + i) % cap."""
+    """Corner-packed records (nt, Xi, Zi, 20) -> a time-innermost device ring
+    (Xi, Zi, cap, 20) (the float64 ring, or the float32 time-innermost one),
+    logical slice i in slot (head + i) % cap."""
     nt, Xi, Zi, n = records.shape
     ring = np.zeros((Xi, Zi, cap, n), dtype=records.dtype)
     for i in range(nt):
@@ -51,25 +52,31 @@ def device_ring(records, cap, head, width=4):
 class DeviceHistory:
     """A device ring holding an oracle DFTracker's stacked history.
 
-    wrap=k places logical slice 0 at slot cap-k so the ring wraps around."""
+    wrap=k places logical slice 0 at slot cap-k so the ring wraps around.
+    rings: which of the float64, float32 chunked and float32 time-innermost
+    rings to build."""
 
-    def __init__(self, tracker, device="cuda", wrap=0, fp32=True):
+    def __init__(self, tracker, device="cuda", wrap=0, rings=("fp64", "fp32", "fp32t")):
         data = np.stack([tracker.data[f] for f in FIELDS], axis=-1)   # (nt, Xi, Zi, 5)
         nt, Xi, Zi, _ = data.shape
         cap = nt + (wrap + 1 if wrap else 0)
         head = (cap - wrap) % cap if wrap else 0
         packed = corner_records(data)
-        self.h64 = torch.from_numpy(ring64(packed, cap, head)).to(device)
-        self.h32 = None
-        if fp32:
+        self.h64 = self.h32 = self.h32t = None
+        if "fp64" in rings:
+            self.h64 = torch.from_numpy(ring64(packed, cap, head)).to(device)
+        if "fp32" in rings:
             self.h32 = torch.from_numpy(device_ring(packed.astype(np.float32), cap, head,
                                                     4)).to(device)
+        if "fp32t" in rings:
+            self.h32t = torch.from_numpy(ring64(packed.astype(np.float32), cap, head)).to(device)
         a = tracker.axes()
-        self.desc = _lib.History(self.h64.data_ptr(),
-                                 self.h32.data_ptr() if self.h32 is not None else None,
-                                 cap, head, nt, Xi, Zi, float(a["min_t"]), float(a["min_x"]),
-                                 float(a["min_z"]), float(a["dt"]), float(a["dx"]),
-                                 float(a["dz"]))
+
+        def p(t):
+            return t.data_ptr() if t is not None else None
+        self.desc = _lib.History(p(self.h64), p(self.h32), p(self.h32t), cap, head, nt, Xi, Zi,
+                                 float(a["min_t"]), float(a["min_x"]), float(a["min_z"]),
+                                 float(a["dt"]), float(a["dx"]), float(a["dz"]))
 
 
 class DeviceLattice:

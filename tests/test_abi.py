@@ -27,7 +27,7 @@ def test_library_exports_header_symbols():
     for s in syms:
         assert hasattr(lib, s), s
     assert set(syms) <= set(_lib.EXPORTED) | {"dfcsr_savgol_operator"}
-    assert lib.dfcsr_abi_version() == _lib.ABI_VERSION == 2
+    assert lib.dfcsr_abi_version() == _lib.ABI_VERSION == 3
 
 
 def test_argument_errors_without_device():
@@ -40,9 +40,16 @@ def test_argument_errors_without_device():
     with pytest.raises(_lib.DfcsrError, match="workspace"):
         _lib.check(lib.dfcsr_df_process(1, 1, 1, 1, 1, 8, 0.0, 1.0, 8, 0.0, 1.0, 5.0, 5, 2, None,
                                         1, 8, None), "df_process")
-    H = _lib.History(1, None, 4, 0, 1, 8, 8, 0, 0, 0, 1, 1, 1)
+    H = _lib.History(1, None, None, 4, 0, 1, 8, 8, 0, 0, 0, 1, 1, 1)
     rc = lib.dfcsr_csr_integrand(H, _lib.Lattice(), 0.0, 0.0, 0.0, 1, 1, 1, 1, 1, 1, None)
     assert rc == -1 and b"slice" in lib.dfcsr_last_error()
+    # the integrand reads the float64 ring; a history needs at least one ring
+    H = _lib.History(None, 2, 3, 4, 0, 2, 8, 8, 0, 0, 0, 1, 1, 1)
+    rc = lib.dfcsr_csr_integrand(H, _lib.Lattice(), 0.0, 0.0, 0.0, 1, 1, 1, 1, 1, 1, None)
+    assert rc == -1 and b"FP64 ring" in lib.dfcsr_last_error()
+    H = _lib.History(None, None, None, 4, 0, 2, 8, 8, 0, 0, 0, 1, 1, 1)
+    rc = lib.dfcsr_csr_integrand(H, _lib.Lattice(), 0.0, 0.0, 0.0, 1, 1, 1, 1, 1, 1, None)
+    assert rc == -1 and b"without a ring" in lib.dfcsr_last_error()
 
 
 @pytest.mark.parametrize("window,order", [(5, 2), (7, 2), (9, 3), (5, 0), (9, 1), (21, 4)])
@@ -97,7 +104,7 @@ def test_wake_tasks_pack_host_logic():
     import ctypes
     lib = _lib.load()
     k = 3
-    hist = (_lib.History * k)(*[_lib.History(1, 2, 8, 0, 5, 16, 16, 0, 0, 0, 1, 1, 1)
+    hist = (_lib.History * k)(*[_lib.History(1, 2, 3, 8, 0, 5, 16, 16, 0, 0, 0, 1, 1, 1)
                                  for _ in range(k)])
     lat = (_lib.Lattice * k)(*[_lib.Lattice(3, 100, 1, 0.0, 0.01, 4, 5) for _ in range(k)])
     P = (_lib.WakeParams * k)(_params(-1.25, 8), _params(0.5, 16), _params(-1.25, 4))
@@ -115,11 +122,29 @@ def test_wake_tasks_pack_host_logic():
     assert batch.blocks_near == sum(m * b for m, (_, b) in zip(M, ph))
     assert batch.partial_bytes == sum(m * (2 * (a + b) + 48) * 8 for m, (a, b) in zip(M, ph))
     assert lib.dfcsr_wake_batched_workspace_bytes(k, P) == nb + batch.partial_bytes
-    # mixed precision needs every task's FP32 history
+    assert batch.precision == _lib.DFCSR_PREC_MIXED
+    # a launch must use the precision the tasks were checked for
+    rc = lib.dfcsr_wake_batched_launch(tbl, ctypes.byref(batch), _lib.DFCSR_PREC_FP64, 1,
+                                       batch.partial_bytes, None)
+    assert rc == -1 and b"packed for precision" in lib.dfcsr_last_error()
+    # mixed precision needs one FP32 ring per task (either layout), FP64 the FP64 ring
     hist[1].data32 = None
     rc = lib.dfcsr_wake_tasks_pack(k, hist, lat, P, _lib.DFCSR_PREC_MIXED, out, out, tbl, nb,
                                    ctypes.byref(batch))
+    assert rc == 0, lib.dfcsr_last_error()
+    hist[1].data32t = None
+    rc = lib.dfcsr_wake_tasks_pack(k, hist, lat, P, _lib.DFCSR_PREC_MIXED, out, out, tbl, nb,
+                                   ctypes.byref(batch))
     assert rc == -1 and b"FP32" in lib.dfcsr_last_error()
+    hist[1].data32 = 2
+    hist[2].data = None
+    rc = lib.dfcsr_wake_tasks_pack(k, hist, lat, P, _lib.DFCSR_PREC_MIXED, out, out, tbl, nb,
+                                   ctypes.byref(batch))
+    assert rc == 0, lib.dfcsr_last_error()
+    rc = lib.dfcsr_wake_tasks_pack(k, hist, lat, P, _lib.DFCSR_PREC_FP64, out, out, tbl, nb,
+                                   ctypes.byref(batch))
+    assert rc == -1 and b"FP64 precision" in lib.dfcsr_last_error()
+    hist[2].data = 1
     rc = lib.dfcsr_wake_tasks_pack(k, hist, lat, P, _lib.DFCSR_PREC_FP64, out, out, tbl, 16,
                                    ctypes.byref(batch))
     assert rc != 0 and b"too small" in lib.dfcsr_last_error()

@@ -129,7 +129,7 @@ def test_interpolate_golden_bitexact(K, golden_dir):
     stack = np.zeros((nt, nx, nz, 5))
     stack[..., 0] = data
     h = _t(ring64(corner_records(stack), nt, 0))
-    H = _lib.History(h.data_ptr(), None, nt, 0, nt, nx, nz, *[float(v) for v in d["mins"]],
+    H = _lib.History(h.data_ptr(), None, None, nt, 0, nt, nx, nz, *[float(v) for v in d["mins"]],
                      *[float(v) for v in d["dels"]])
     r = K.interpolate3D(H, 0, _t(d["q3"]), DEV).cpu().numpy()
     assert np.array_equal(r, d["r3"])
@@ -275,3 +275,31 @@ def test_wake_mixed_precision_golden(K, ws, golden_dir, golden_history):
         o = K.wake_mesh(DH.desc, DL.desc, Pc, 1, DEV, ws)
         assert rel_max(o["dE_dct"].cpu().numpy(), w[key][:, 0]) <= 1e-4
         assert rel_max(o["x_kick"].cpu().numpy(), w[key][:, 1]) <= 1e-4
+
+
+def test_wake_mixed_precision_ring_layouts(K, ws, golden_dir, golden_history):
+    """The mixed form reads either float32 ring layout (chunked, time-innermost,
+    or both: region 0 the time-innermost one, the later regions the chunked
+    one) with the same arithmetic, so the wakes are bit-identical across
+    layouts; without the float64 ring, vx at the observer comes from a float32
+    ring, within 1e-4 of max|W| of the reference's float64 wakes."""
+    w, beam, _, DL, P = _golden_wake_inputs(golden_dir, golden_history)
+    _, T = golden_history
+    res = {}
+    for rings in (("fp64", "fp32", "fp32t"), ("fp64", "fp32"), ("fp64", "fp32t"),
+                  ("fp32", "fp32t"), ("fp32",), ("fp32t",)):
+        DH = DeviceHistory(T, DEV, wrap=2, rings=rings)
+        out = K.wake_mesh(DH.desc, DL.desc, P, 1, DEV, ws)
+        res[rings] = np.stack([out["dE_dct"].cpu().numpy(), out["x_kick"].cpu().numpy()])
+        e_z = rel_max(res[rings][0].reshape(3, 4), w["dE_dct"])
+        e_x = rel_max(res[rings][1].reshape(3, 4), w["x_kick"])
+        assert e_z <= 1e-4 and e_x <= 1e-4, (rings, e_z, e_x)
+    with64 = [r for r in res if "fp64" in r]
+    without = [r for r in res if "fp64" not in r]
+    for group in (with64, without):
+        for r in group[1:]:
+            assert np.array_equal(res[r], res[group[0]]), (r, group[0])
+    # FP64 precision refuses a history without the float64 ring
+    DH = DeviceHistory(T, DEV, rings=("fp32", "fp32t"))
+    with pytest.raises(Exception, match="FP64 history ring"):
+        K.wake_mesh(DH.desc, DL.desc, P, 0, DEV, ws)

@@ -6,16 +6,19 @@ older than the retarded-time window and re-grids either the new slice or,
 when the beam size left the [1/2, 2] band, every logged slice onto a new
 interp grid; build_interpolant fixes the axes the wake kernel reads.
 
-Storage is two rings with one slot per logged slice, on the device for the
+Storage is rings with one slot per logged slice, on the device for the
 whole run:
-  raw   [cap][5][nbmax*nbmax] float64  fields on the slice's own grid
-  hist  [Xi][Zi][cap][20]     float64  re-gridded corner-packed records
-  hist32[Xi][cap][5][Zi][4]   float32  shadow read by the mixed-precision wake
+  raw    [cap][5][nbmax*nbmax] float64  fields on the slice's own grid
+  hist   [Xi][Zi][cap][20]     float64  re-gridded corner-packed records
+  hist32t[Xi][Zi][cap][20]     float32  the same records (mixed precision)
+  hist32 [Xi][cap][5][Zi][4]   float32  the same, chunked (mixed precision)
 (record (x, z) holds every field at (x, z), (x, z+1), (x+1, z), (x+1, z+1),
-clamped to the last plane: one record per time plane per lookup.  The
-float64 ring keeps the slot innermost (the two planes of a lookup are
-neighbours); the float32 ring stores each field's four corners as one
-16-byte chunk, z fastest, so neighbouring z of one plane are neighbours)
+clamped to the last plane: one record per time plane per lookup.  hist and
+hist32t keep the slot innermost (the two planes of a lookup are neighbours:
+what the far region's lookups want); hist32 stores each field's four corners
+as one 16-byte chunk, z fastest, so neighbouring z of one plane are
+neighbours: what the near regions' lookups want.  keep_fp64 / keep_fp32
+choose the float64 ring and the two float32 rings.)
 Appending writes slot (head + nt) % cap; popping advances head.  Nothing is
 copied per step: the reference's np.array(deque) stacking of the whole
 history (deposit.py:422-426) becomes a descriptor update.

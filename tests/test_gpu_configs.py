@@ -162,3 +162,59 @@ def test_large_run_sampled_mesh_points_vs_oracle():
     for k in ("dE_dct", "x_kick"):
         assert report[("fp64", k)][0] <= 1e-9, report
         assert report[("mixed", k)][0] <= 1e-4, report
+
+
+def test_chicane_run_sampled_mesh_points_vs_oracle():
+    """The bench's chicane configuration (1e6 particles, 4 bends at 0.05 rad,
+    deposition 128 x 128 SG 7/2, wake mesh 64 x 64, 200 x 200 integration, chirp
+    branch) after 200 steps (192 slices on the full interp grid): FP64 and
+    mixed wakes at 64 seeded mesh points against the oracle's C/OpenMP
+    integral on the same history, within 1e-9 / 1e-4 of max|W|; the mixed
+    wake without the float64 ring (vx at the observer from float32) within
+    1e-4 as well."""
+    import torch
+    from bench import chicane_config
+    from oracle import csr_ref_c as C
+    from pydfcsr_amd import _lib, kernels
+    from pydfcsr_amd.csr import CSR2D
+
+    cfg = chicane_config()
+    sim = CSR2D(config=cfg, device=DEV, precision="fp64", keep_fp32=True)
+    sim.run(max_steps=200)
+    torch.cuda.synchronize()
+    tr = sim.DF_tracker
+    sim.get_CSR_mesh()
+    H = tr.descriptor()
+    L = sim.lattice.device_tables(sim.device)
+    P = sim.wake_params()
+    assert abs(P.slope0) > 1   # the chirp branch (four regions)
+    fp64 = kernels.wake_mesh(H, L, P, _lib.DFCSR_PREC_FP64, sim.device, sim.ws)
+    mixed = kernels.wake_mesh(H, L, P, _lib.DFCSR_PREC_MIXED, sim.device, sim.ws,
+                              want_mesh=False)
+    H32 = _lib.History.from_buffer_copy(H)
+    H32.data = None
+    mixed32 = kernels.wake_mesh(H32, L, P, _lib.DFCSR_PREC_MIXED, sim.device, sim.ws,
+                                want_mesh=False)
+    torch.cuda.synchronize()
+    M = P.mesh_xbins * P.mesh_zbins
+    rows = np.sort(np.random.default_rng(7).choice(M, 64, replace=False))
+    keep = C._Keep()
+    hist = C.hist_from_records(tr.hist.cpu().numpy(), tr.head, tr.nt,
+                               (float(tr.min_x), float(tr.min_y), float(tr.min_z),
+                                float(tr.delta_x), float(tr.delta_y), float(tr.delta_z)), keep)
+    latc = C.lat_from_tables(sim.lattice, keep)
+    b = sim.beam
+    beam = dict(sigma_x=b._sigma_x, sigma_z=b._sigma_z, slope=b._slope, mean_x=b._mean_x)
+    obs = np.stack([_np(fp64["zmesh"])[rows], _np(fp64["xmesh"])[rows]], axis=1)
+    ip = cfg["CSR_integration"]
+    dE, xk = C.wake(hist, latc, P.t, beam, P.formation_length, P.n_formation_length, ip["xbins"],
+                    ip["zbins"], P.csr_scaling, obs, C.threads())
+    report = {}
+    for name, res in (("fp64", fp64), ("mixed", mixed), ("mixed_fp32_rings", mixed32)):
+        for k, ref in (("dE_dct", dE), ("x_kick", xk)):
+            report[(name, k)] = _max_rms(_np(res[k])[rows], ref)
+    print(f"chicane run, s = {P.t:.3f} m, {tr.nt} slices on {tr.Xi}x{tr.Zi}:", report, flush=True)
+    for k in ("dE_dct", "x_kick"):
+        assert report[("fp64", k)][0] <= 1e-9, report
+        assert report[("mixed", k)][0] <= 1e-4, report
+        assert report[("mixed_fp32_rings", k)][0] <= 1e-4, report

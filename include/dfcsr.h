@@ -78,6 +78,30 @@ int64_t dfcsr_moments_workspace_bytes(int64_t n);
 int dfcsr_beam_moments(const double *x, const double *z, int64_t n, void *ws, int64_t ws_bytes,
                        double *out, void *stream);
 
+/* Per-step beam statistics on the device: replaces CSR2D.update_statistics'
+ * reads of Beam.twiss / sigma_energy / mean_energy (CSR.py:837-859,
+ * twiss.py:2-70, beams.py:166-181).  out[DFCSR_STATS_COUNT] (device):
+ *   [DFCSR_STAT_MEAN + k]  mean of coordinate k (x, px, y, py, z, pz)
+ *   [DFCSR_STAT_COV + t]   sample covariance (np.cov, ddof = 1), row-major
+ *                          upper triangle: t = 0 (x,x), 1 (x,px), ... 20 (pz,pz)
+ *   [DFCSR_STAT_TWISS_X + k], [DFCSR_STAT_TWISS_Y + k]  twiss_dispersion_calc
+ *                          of (x, px, pz) / (y, py, pz): alpha, beta, gamma,
+ *                          emit, eta, etap, norm_emit (= emit * p0c / mc2)
+ *   [DFCSR_STAT_MEAN_ENERGY], [DFCSR_STAT_SIGMA_ENERGY]  of (pz + 1) * p0c
+ *                          (population std).
+ * Deterministic (fixed reduction tree); no host synchronisation. */
+#define DFCSR_STAT_MEAN 0
+#define DFCSR_STAT_COV 6
+#define DFCSR_STAT_TWISS_X 27
+#define DFCSR_STAT_TWISS_Y 34
+#define DFCSR_STAT_MEAN_ENERGY 41
+#define DFCSR_STAT_SIGMA_ENERGY 42
+#define DFCSR_STATS_COUNT 43
+int64_t dfcsr_beam_statistics_workspace_bytes(int64_t n);
+int dfcsr_beam_statistics(const double *x, const double *px, const double *y, const double *py,
+                          const double *z, const double *pz, int64_t n, double p0c, double mc2,
+                          void *ws, int64_t ws_bytes, double *out, void *stream);
+
 /* ---- a3: cloud-in-cell deposition ----------------------------------------
  * Replaces histogram_cic_2d (deposit.py:42-87): hist[i1][i2] += w*S1*S2 over
  * the four CIC corners with cell = (q - start) * (1/((end - start)/nbins)).

@@ -25,6 +25,10 @@ Pinning (what checks each function, see tests/test_oracle_*.py):
   csr_integrand/wake    zero-density and linearity invariants, scalar loop
                         form, golden vectors from the reference module.
   apply_wakes           RegularGridInterpolator closed form on bilinear fields.
+  beam_statistics       twiss: a beam built with known Twiss/dispersion (exact
+                        sample moments), beta*gamma - alpha^2 = 1, emittance =
+                        sqrt(det) of the dispersion-free matrix; energy
+                        statistics against the definitions.
 """
 from __future__ import annotations
 
@@ -147,6 +151,40 @@ def beam_moments(x, z):
     return dict(mean_x=np.mean(x), mean_z=np.mean(z), sigma_x=sigma_x, sigma_z=sigma_z,
                 slope=p, mean_xt=np.mean(xt), sigma_xt=np.std(xt), slice_sigma_x=slice_sig,
                 n_slice=int(slice_ind.shape[0]))
+
+
+# ---------------------------------------------------------------------------
+# per-step statistics   (CSR.py:837-859; twiss.py:2-70; beams.py:166-181)
+# ---------------------------------------------------------------------------
+def twiss_dispersion_calc(sigma3):
+    """Twiss and dispersion of one plane from the 3 x 3 covariance of
+    (u, pu, delta), twiss.py:30-70: the dispersion-free 2 x 2 matrix
+    [[eb, -ea], [-ea, eg]] = Sigma_uu - eta eta^T delta2 and its emittance."""
+    delta2 = sigma3[2, 2]
+    xd = sigma3[0, 2]
+    pd = sigma3[1, 2]
+    eb = sigma3[0, 0] - xd ** 2 / delta2
+    eg = sigma3[1, 1] - pd ** 2 / delta2
+    ea = -sigma3[0, 1] + xd * pd / delta2
+    emit = np.sqrt(eb * eg - ea ** 2)
+    return dict(alpha=ea / emit, beta=eb / emit, gamma=eg / emit, emit=emit, eta=xd / delta2,
+                etap=pd / delta2)
+
+
+def beam_statistics(x, px, y, py, z, pz, p0c, mc2):
+    """The per-step record of CSR2D.update_statistics (CSR.py:837-859):
+    twiss_from_bmadx_particles (twiss.py:2-27: np.cov of (x, px, pz) and
+    (y, py, pz), norm_emit = emit p0c / mc2) and the energy (pz + 1) p0c's
+    mean and np.std (beams.py:166-181)."""
+    out = {}
+    for plane, (u, pu) in (("x", (x, px)), ("y", (y, py))):
+        tw = twiss_dispersion_calc(np.cov([u, pu, pz]))
+        tw["norm_emit"] = tw["emit"] * p0c / mc2
+        for k, v in tw.items():
+            out[f"{k}_{plane}"] = v
+    energy = (np.asarray(pz) + 1) * p0c
+    return dict(twiss=out, mean_energy=np.mean(energy), sigma_energy=np.std(energy),
+                cov=np.cov([x, px, y, py, z, pz]))
 
 
 # ---------------------------------------------------------------------------

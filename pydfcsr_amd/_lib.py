@@ -24,6 +24,10 @@ ABI_VERSION = 3   # DFCSR_ABI_VERSION
 DFCSR_PREC_FP64 = 0
 DFCSR_PREC_MIXED = 1
 MOMENTS_COUNT = 10
+# beam statistics (DFCSR_STAT_*): means, covariance triangle, Twiss x / y, energy
+STAT_MEAN, STAT_COV, STAT_TWISS_X, STAT_TWISS_Y = 0, 6, 27, 34
+STAT_MEAN_ENERGY, STAT_SIGMA_ENERGY, STATS_COUNT = 41, 42, 43
+TWISS_KEYS = ("alpha", "beta", "gamma", "emit", "eta", "etap", "norm_emit")
 HIST_REC64 = 20   # corner-packed records: field f, corner c at [4 f + c]
 HIST_REC32 = 20
 HIST_CHUNKS32 = 5
@@ -72,6 +76,9 @@ _SIGNATURES = [
     ("dfcsr_last_error", ctypes.c_char_p, []),
     ("dfcsr_moments_workspace_bytes", c_i64, [c_i64]),
     ("dfcsr_beam_moments", c_i32, [c_dp, c_dp, c_i64, c_dp, c_i64, c_dp, c_dp]),
+    ("dfcsr_beam_statistics_workspace_bytes", c_i64, [c_i64]),
+    ("dfcsr_beam_statistics", c_i32, [c_dp, c_dp, c_dp, c_dp, c_dp, c_dp, c_i64, c_f64, c_f64,
+                                      c_dp, c_i64, c_dp, c_dp]),
     ("dfcsr_histogram_workspace_bytes", c_i64, [c_i32, c_i32]),
     ("dfcsr_histogram_cic_2d", c_i32, [c_dp, c_dp, c_dp, c_i64, c_i32, c_f64, c_f64, c_i32,
                                        c_f64, c_f64, c_dp, c_dp, c_dp, c_dp, c_i64, c_dp]),

@@ -7,7 +7,9 @@ apply_wakes and tracking through lattice elements.
 
 Statistics (sigma, mean, slope = polyfit(z, x, 1), x_transform moments and
 the |z| < 0.1 sigma_z slice std) come from one deterministic device
-reduction; the kick is the gfx950 mesh->particle kernel.
+reduction; the per-step record (Twiss of x and y, energy mean and spread)
+from another (dfcsr_beam_statistics); the kick is the gfx950 mesh->particle
+kernel.
 
 Tracking replaces bmadx's track_element, which this build cannot use: each
 step applies the linear transfer map of the element piece (r_gen6.py:4-122
@@ -25,7 +27,9 @@ import torch
 
 from . import _lib, kernels
 
-MC2 = 510998.95   # electron rest energy, eV
+# electron rest energy [eV]: scipy.constants' 'electron mass energy equivalent
+# in MeV' (CODATA 2018) x 1e6, as physical_constants.py:11
+MC2 = 510998.95069
 
 
 def transfer_matrix(L, angle=0.0, k1=0.0, E1=0.0, E2=0.0, hgap=0.0):
@@ -220,13 +224,24 @@ class Beam:
     def energy(self):
         return (self.pz + 1) * self.init_energy
 
-    @property
-    def mean_energy(self):
-        return float(self.energy.mean())
+    def statistics(self, out=None):
+        """Device vector of the per-step statistics (dfcsr_beam_statistics):
+        means, covariance, Twiss x / y, energy mean and spread; the particle
+        momentum reference p0c is init_energy (beams.py:33)."""
+        self.coords = self.coords.contiguous()
+        return kernels.beam_statistics(self.coords, self.init_energy, MC2, self.ws, out)
 
     @property
-    def sigma_energy(self):
-        return float(self.energy.std(unbiased=False))
+    def twiss(self):   # beams.py:222-224 -> twiss.py:2-27
+        return kernels.statistics_dict(self.statistics())["twiss"]
+
+    @property
+    def mean_energy(self):   # beams.py:171-173
+        return kernels.statistics_dict(self.statistics())["mean_energy"]
+
+    @property
+    def sigma_energy(self):   # beams.py:179-181 (np.std: population)
+        return kernels.statistics_dict(self.statistics())["sigma_energy"]
 
 
 def _axis(a):

@@ -96,25 +96,53 @@ class CSR2D:
         self.CSR_scaling = CSR_SCALING_PER_COULOMB * self.beam.charge
         self.init_statistics()
 
-    def init_statistics(self):
+    def init_statistics(self):   # CSR.py:82-108
         n = self.lattice.total_steps
-        self.statistics = {k: np.zeros(n) for k in ("sigma_x", "sigma_z", "mean_x", "mean_z")}
-        self.statistics["slope"] = np.zeros((n, 2))
+        st = {"twiss": {f"{key}_{plane}": np.zeros(n) for plane in ("x", "y")
+                        for key in _lib.TWISS_KEYS}}
+        st["slope"] = np.zeros((n, 2))
+        for k in ("sigma_x", "sigma_z", "sigma_energy", "mean_x", "mean_z", "mean_energy"):
+            st[k] = np.zeros(n)
+        self._statistics = st
+        # Twiss and energy rows stay on the device until the dict is read
+        self._stats_dev = torch.zeros((n, _lib.STATS_COUNT), dtype=torch.float64,
+                                      device=self.device)
+        self._stats_dirty = False
         self.update_statistics(step=0)
         self.inbend = False
         self.afterbend = False
         self.R_rec = None
         self.phi_rec = None
 
-    def update_statistics(self, step):   # CSR.py:837-859 (beam moments subset)
-        if step >= len(self.statistics["sigma_x"]):
+    def update_statistics(self, step):   # CSR.py:837-859
+        if step >= len(self._statistics["sigma_x"]):
             return
         b = self.beam
-        self.statistics["slope"][step, :] = b._slope
-        self.statistics["sigma_x"][step] = b._sigma_x
-        self.statistics["sigma_z"][step] = b._sigma_z
-        self.statistics["mean_x"][step] = b._mean_x
-        self.statistics["mean_z"][step] = b._mean_z
+        st = self._statistics
+        st["slope"][step, :] = b._slope
+        st["sigma_x"][step] = b._sigma_x
+        st["sigma_z"][step] = b._sigma_z
+        st["mean_x"][step] = b._mean_x
+        st["mean_z"][step] = b._mean_z
+        b.statistics(out=self._stats_dev[step])   # Twiss, energy: device row, no sync
+        self._stats_dirty = True
+
+    @property
+    def statistics(self):
+        """The reference's statistics dict (CSR.py:82-106): per step twiss
+        (alpha/beta/gamma/emit/eta/etap/norm_emit of x and y), slope, sigma_x,
+        sigma_z, sigma_energy, mean_x, mean_z, mean_energy.  Reading it copies
+        the device-side rows once."""
+        if self._stats_dirty:
+            v = self._stats_dev.cpu().numpy()
+            tw = self._statistics["twiss"]
+            for plane, off in (("x", _lib.STAT_TWISS_X), ("y", _lib.STAT_TWISS_Y)):
+                for k, key in enumerate(_lib.TWISS_KEYS):
+                    tw[f"{key}_{plane}"][:] = v[:, off + k]
+            self._statistics["mean_energy"][:] = v[:, _lib.STAT_MEAN_ENERGY]
+            self._statistics["sigma_energy"][:] = v[:, _lib.STAT_SIGMA_ENERGY]
+            self._stats_dirty = False
+        return self._statistics
 
     def get_formation_length(self, R, sigma_z, phi=0.0, inbend=True):   # CSR.py:140-144
         if inbend:

@@ -102,10 +102,141 @@ __global__ __launch_bounds__(kThreads) void moments_final(int64_t n, int pass, i
   }
 }
 
+// ---- per-step beam statistics (CSR.py:837-859, twiss.py:2-70) -------------
+//
+// Means and the sample covariance (np.cov, ddof = 1) of the six coordinates
+// (x, px, y, py, z, pz), two passes like np.cov (mean, then centred products),
+// each a fixed reduction tree; the final single-workgroup pass derives the
+// Twiss and dispersion parameters of (x, px, pz) and (y, py, pz) and the
+// energy statistics from them.
+constexpr int kNCov = 27;          // 6 sums, then the 21 centred products (upper triangle)
+constexpr int kStatOff = 32;       // block partials start here (doubles)
+
+__device__ inline int tri_index(int i, int j) {   // i <= j < 6, row-major upper triangle
+  return i * 6 - i * (i - 1) / 2 + (j - i);
+}
+
+__global__ __launch_bounds__(kThreads) void stats_partial(
+    const double *__restrict__ x, const double *__restrict__ px, const double *__restrict__ y,
+    const double *__restrict__ py, const double *__restrict__ z, const double *__restrict__ pz,
+    int64_t n, int pass, double *__restrict__ st) {
+  __shared__ double red[kThreads / 64];
+  const double *c[6] = {x, px, y, py, z, pz};
+  double a[21];
+#pragma unroll
+  for (int k = 0; k < 21; ++k) a[k] = 0.0;
+  double m[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) m[k] = st[k];
+  const int64_t stride = (int64_t)gridDim.x * kThreads;
+  for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) {
+    double v[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v[k] = c[k][i];
+    if (pass == 0) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) a[k] += v[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) v[k] = v[k] - m[k];
+#pragma unroll
+      for (int p = 0; p < 6; ++p)
+#pragma unroll
+        for (int q = p; q < 6; ++q) a[tri_index(p, q)] += v[p] * v[q];
+    }
+  }
+  const int nv = pass == 0 ? 6 : 21;
+  for (int k = 0; k < nv; ++k) {
+    const double v = block_sum(a[k], red);
+    if (threadIdx.x == 0) st[kStatOff + blockIdx.x * kNCov + k] = v;
+  }
+}
+
+// Twiss and dispersion of one plane from its 3 x 3 covariance of (u, pu,
+// delta) (twiss.py:30-70): out = alpha, beta, gamma, emit, eta, etap, norm_emit.
+__device__ inline void twiss_plane(double s00, double s01, double s02, double s11, double s12,
+                                   double s22, double p0c, double mc2, double *out) {
+  const double delta2 = s22;
+  const double eb = s00 - s02 * s02 / delta2;
+  const double eg = s11 - s12 * s12 / delta2;
+  const double ea = -s01 + s02 * s12 / delta2;
+  const double emit = sqrt(eb * eg - ea * ea);
+  out[0] = ea / emit;
+  out[1] = eb / emit;
+  out[2] = eg / emit;
+  out[3] = emit;
+  out[4] = s02 / delta2;
+  out[5] = s12 / delta2;
+  out[6] = emit * p0c / mc2;
+}
+
+__global__ __launch_bounds__(kThreads) void stats_final(int64_t n, int pass, int nblk,
+                                                        double p0c, double mc2,
+                                                        double *__restrict__ st,
+                                                        double *__restrict__ out) {
+  __shared__ double red[kThreads / 64];
+  const int nv = pass == 0 ? 6 : 21;
+  double tot[21];
+  for (int k = 0; k < nv; ++k) {
+    double v = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += kThreads) v += st[kStatOff + b * kNCov + k];
+    tot[k] = block_sum(v, red);
+  }
+  if (threadIdx.x != 0) return;
+  const double dn = (double)n;
+  if (pass == 0) {
+    for (int k = 0; k < 6; ++k) st[k] = tot[k] / dn;
+    return;
+  }
+  double *cov = out + DFCSR_STAT_COV;
+  for (int k = 0; k < 6; ++k) out[DFCSR_STAT_MEAN + k] = st[k];
+  for (int k = 0; k < 21; ++k) cov[k] = tot[k] / (dn - 1.0);
+  // planes (x, px, pz) and (y, py, pz): coordinates 0, 1, 5 and 2, 3, 5
+  twiss_plane(cov[tri_index(0, 0)], cov[tri_index(0, 1)], cov[tri_index(0, 5)],
+              cov[tri_index(1, 1)], cov[tri_index(1, 5)], cov[tri_index(5, 5)], p0c, mc2,
+              out + DFCSR_STAT_TWISS_X);
+  twiss_plane(cov[tri_index(2, 2)], cov[tri_index(2, 3)], cov[tri_index(2, 5)],
+              cov[tri_index(3, 3)], cov[tri_index(3, 5)], cov[tri_index(5, 5)], p0c, mc2,
+              out + DFCSR_STAT_TWISS_Y);
+  // energy = (pz + 1) p0c (beams.py:166-181): its mean and population std
+  out[DFCSR_STAT_MEAN_ENERGY] = (st[5] + 1.0) * p0c;
+  out[DFCSR_STAT_SIGMA_ENERGY] = p0c * sqrt(tot[tri_index(5, 5)] / dn);
+}
+
 }  // namespace
 }  // namespace dfcsr
 
 using namespace dfcsr;
+
+extern "C" int64_t dfcsr_beam_statistics_workspace_bytes(int64_t n) {
+  (void)n;
+  return (int64_t)(kStatOff + kMaxBlocks * kNCov) * (int64_t)sizeof(double);
+}
+
+extern "C" int dfcsr_beam_statistics(const double *x, const double *px, const double *y,
+                                     const double *py, const double *z, const double *pz,
+                                     int64_t n, double p0c, double mc2, void *ws,
+                                     int64_t ws_bytes, double *out, void *stream) {
+  if (!x || !px || !y || !py || !z || !pz || !out || !ws)
+    return set_error(DFCSR_EINVAL, "beam_statistics: NULL pointer");
+  if (n < 2)
+    return set_error(DFCSR_EINVAL, "beam_statistics: need n >= 2 particles (n=%lld)",
+                     (long long)n);
+  if (ws_bytes < dfcsr_beam_statistics_workspace_bytes(n))
+    return set_error(DFCSR_ENOWS, "beam_statistics: workspace %lld < %lld bytes",
+                     (long long)ws_bytes, (long long)dfcsr_beam_statistics_workspace_bytes(n));
+  hipStream_t s = as_stream(stream);
+  double *st = static_cast<double *>(ws);
+  const int nb = nblocks_for(n);
+  for (int pass = 0; pass < 2; ++pass) {
+    hipLaunchKernelGGL(stats_partial, dim3(nb), dim3(kThreads), 0, s, x, px, y, py, z, pz, n,
+                       pass, st);
+    hipLaunchKernelGGL(stats_final, dim3(1), dim3(kThreads), 0, s, n, pass, nb, p0c, mc2, st,
+                       out);
+  }
+  DFCSR_LAUNCH_CHECK("beam_statistics");
+  return DFCSR_OK;
+}
 
 extern "C" int64_t dfcsr_moments_workspace_bytes(int64_t n) {
   (void)n;

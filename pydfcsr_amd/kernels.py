@@ -53,6 +53,50 @@ def beam_moments(x, z, ws: _lib.Workspace, out: torch.Tensor | None = None, host
     return d
 
 
+def beam_statistics(coords: torch.Tensor, p0c, mc2, ws: _lib.Workspace,
+                    out: torch.Tensor | None = None):
+    """Device statistics of the (6, N) coordinates (x, px, y, py, z, pz):
+    means, np.cov triangle, Twiss x / y and energy statistics, written into
+    ``out`` (a device float64 view of DFCSR_STATS_COUNT values) without a
+    host synchronisation."""
+    lib = _lib.load()
+    _f64(coords, "coords")
+    if coords.dim() != 2 or coords.shape[0] != 6:
+        raise DfcsrError(f"beam_statistics: coords must be (6, N), got {tuple(coords.shape)}")
+    n = coords.shape[1]
+    if out is None:
+        out = torch.empty(_lib.STATS_COUNT, dtype=torch.float64, device=coords.device)
+    if (out.dtype != torch.float64 or out.numel() != _lib.STATS_COUNT or not out.is_contiguous()
+            or out.device != coords.device):
+        raise DfcsrError("beam_statistics: out must be a contiguous float64 device vector of "
+                         f"{_lib.STATS_COUNT}")
+    b = ws.get("statistics", lib.dfcsr_beam_statistics_workspace_bytes(n))
+    rows = [coords[k] for k in range(6)]
+    check(lib.dfcsr_beam_statistics(*[ptr(r) for r in rows], n, float(p0c), float(mc2), ptr(b),
+                                    b.numel(), ptr(out), stream_handle(coords.device)),
+          "beam_statistics")
+    return out
+
+
+def statistics_dict(v) -> dict:
+    """Host dict of one statistics vector: twiss (the reference's keys
+    alpha_x ... norm_emit_y), mean_energy, sigma_energy, mean, cov (6 x 6)."""
+    v = np.asarray(v.cpu() if isinstance(v, torch.Tensor) else v, dtype=np.float64)
+    tw = {}
+    for plane, off in (("x", _lib.STAT_TWISS_X), ("y", _lib.STAT_TWISS_Y)):
+        for k, key in enumerate(_lib.TWISS_KEYS):
+            tw[f"{key}_{plane}"] = float(v[off + k])
+    cov = np.zeros((6, 6))
+    t = _lib.STAT_COV
+    for i in range(6):
+        for j in range(i, 6):
+            cov[i, j] = cov[j, i] = v[t]
+            t += 1
+    return dict(twiss=tw, mean_energy=float(v[_lib.STAT_MEAN_ENERGY]),
+                sigma_energy=float(v[_lib.STAT_SIGMA_ENERGY]),
+                mean=v[_lib.STAT_MEAN:_lib.STAT_MEAN + 6].copy(), cov=cov)
+
+
 def histogram_cic_2d(q1, q2, w, nbins_1, bins_start_1, bins_end_1, nbins_2, bins_start_2,
                      bins_end_2, ws: _lib.Workspace, want_cells=False):
     """Device histogram_cic_2d; w=None means unit weights.  With want_cells,

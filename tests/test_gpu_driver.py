@@ -123,6 +123,20 @@ def test_run_loop_matches_oracle_step_by_step():
             pending = (x, z, px, pz, dE, xk, mesh)
         else:
             steps = step
+            # the step's statistics record (CSR.py:837-859) on the particles it ends with
+            X = _np(b.coords)
+            ref = O.beam_statistics(*X, float(b.init_energy), 510998.95069)
+            st = sim.statistics
+            # non-finite values must coincide: a degenerate plane (y = py = 0) gives
+            # 0/0 and x/0 as numpy does, and NaN kicks (mesh points past the lattice
+            # end) make the last step's particles NaN
+            pairs = [(k, st["twiss"][k][step], r, 1e-10) for k, r in ref["twiss"].items()]
+            pairs += [(k, st[k][step], ref[k], 1e-12) for k in ("mean_energy", "sigma_energy")]
+            for k, v, r, tol in pairs:
+                if not np.isfinite(r):
+                    assert (np.isnan(r) and np.isnan(v)) or v == r, (step, k, v, r)
+                else:
+                    assert abs(v - r) <= tol * abs(r), (step, k, v, r)
             if pending is None:
                 continue
             x, z, px, pz, dE, xk, mesh = pending

@@ -78,6 +78,30 @@ def test_beam_moments(K, ws):
     assert all(m2[k] == m[k] for k in m if k != "slope")   # deterministic
 
 
+def test_beam_statistics(K, ws):
+    """Per-step record (CSR.py:837-859): Twiss of x and y from np.cov of
+    (x, px, pz) / (y, py, pz), energy mean and spread, the 6 x 6 covariance;
+    a correlated beam with dispersion, 300 007 particles, against the oracle
+    within 1e-11 relative; deterministic."""
+    rng = np.random.default_rng(12)
+    n = 300007
+    A = rng.standard_normal((6, 6)) * np.array([1e-4, 1e-5, 2e-4, 3e-5, 5e-5, 1e-3])[:, None]
+    X = A @ rng.standard_normal((6, n)) + np.array([1e-5, 2e-6, -2e-5, 0, 3e-6, 2e-3])[:, None]
+    p0c, mc2 = 1.0e9, 510998.95069
+    coords = _t(X)
+    v = K.beam_statistics(coords, p0c, mc2, ws)
+    d = K.statistics_dict(v)
+    ref = O.beam_statistics(*X, p0c, mc2)
+    for k, r in ref["twiss"].items():
+        assert abs(d["twiss"][k] - r) <= 1e-11 * abs(r), (k, d["twiss"][k], r)
+    for k in ("mean_energy", "sigma_energy"):
+        assert abs(d[k] - ref[k]) <= 1e-12 * abs(ref[k]), k
+    assert np.allclose(d["cov"], ref["cov"], rtol=1e-11, atol=0)
+    assert np.allclose(d["mean"], X.mean(axis=1), rtol=1e-12, atol=1e-20)
+    v2 = K.beam_statistics(coords, p0c, mc2, ws)
+    assert torch.equal(v, v2)
+
+
 # -- a3 ------------------------------------------------------------------------------
 def test_histogram_golden_and_cells(K, ws, golden_dir):
     d = _g(golden_dir, "ref_histogram_cic_2d.npz")

@@ -203,3 +203,62 @@ def test_rgi_restatement_matches_scipy():
     oob = (q[:, 0] < xg[0]) | (q[:, 0] > xg[-1]) | (q[:, 1] < zg[0]) | (q[:, 1] > zg[-1])
     v[oob] = 0.0
     assert np.array_equal(v, ref)
+
+
+# -- per-step statistics (twiss.py:2-70) -------------------------------------------
+def _beam_with_covariance(cov, n=4000, seed=0):
+    """Particles whose sample covariance (ddof = 1) is exactly cov (to rounding):
+    centre random samples, whiten with their own Cholesky factor, colour with
+    cov's."""
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((cov.shape[0], n))
+    X -= X.mean(axis=1, keepdims=True)
+    Lc = np.linalg.cholesky(np.cov(X))
+    Lt = np.linalg.cholesky(cov)
+    return Lt @ np.linalg.solve(Lc, X)
+
+
+@pytest.mark.parametrize("alpha,beta,emit,eta,etap,sd", [
+    (-1.3, 7.5, 2e-9, 0.12, -0.03, 1e-3), (0.4, 0.8, 5e-7, 0.0, 0.0, 2e-4),
+    (2.0, 25.0, 1e-10, -0.5, 0.2, 5e-5)])
+def test_twiss_recovers_constructed_beam(alpha, beta, emit, eta, etap, sd):
+    gamma = (1 + alpha ** 2) / beta
+    d2 = sd ** 2
+    cov = np.zeros((3, 3))
+    cov[:2, :2] = emit * np.array([[beta, -alpha], [-alpha, gamma]]) + \
+        d2 * np.outer([eta, etap], [eta, etap])
+    cov[:2, 2] = cov[2, :2] = d2 * np.array([eta, etap])
+    cov[2, 2] = d2
+    u, pu, d = _beam_with_covariance(cov)
+    tw = O.twiss_dispersion_calc(np.cov([u, pu, d]))
+    for k, v in dict(alpha=alpha, beta=beta, gamma=gamma, emit=emit, eta=eta, etap=etap).items():
+        assert abs(tw[k] - v) <= 1e-9 * max(abs(v), 1.0) * (1 if k != "emit" else emit), (k, tw[k], v)
+    # invariant of any beam
+    assert abs(tw["beta"] * tw["gamma"] - tw["alpha"] ** 2 - 1) < 1e-12
+
+
+def test_beam_statistics_planes_and_energy():
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((6, 6)) * np.array([1e-4, 1e-5, 2e-4, 3e-5, 5e-5, 1e-3])[:, None]
+    C = A @ A.T
+    x, px, y, py, z, pz = _beam_with_covariance(C, n=5000, seed=4) + \
+        np.array([1e-5, 0, -2e-5, 0, 3e-6, 2e-3])[:, None]
+    p0c, mc2 = 1e9, 510998.95069
+    st = O.beam_statistics(x, px, y, py, z, pz, p0c, mc2)
+    Xc = np.stack([x, px, y, py, z, pz])
+    Xc = Xc - Xc.mean(axis=1, keepdims=True)
+    assert np.allclose(st["cov"], Xc @ Xc.T / (Xc.shape[1] - 1), rtol=1e-12, atol=0)
+    # each plane uses (u, pu, pz): the x plane must not see y and vice versa
+    for plane, idx in (("x", [0, 1, 5]), ("y", [2, 3, 5])):
+        s3 = C[np.ix_(idx, idx)]
+        d2 = s3[2, 2]
+        m = s3[:2, :2] - np.outer(s3[:2, 2], s3[:2, 2]) / d2
+        emit = np.sqrt(np.linalg.det(m))
+        assert abs(st["twiss"][f"emit_{plane}"] / emit - 1) < 1e-9
+        assert abs(st["twiss"][f"beta_{plane}"] - m[0, 0] / emit) < 1e-9 * abs(m[0, 0] / emit)
+        assert abs(st["twiss"][f"alpha_{plane}"] + m[0, 1] / emit) < 1e-9 * max(abs(m[0, 1] / emit), 1)
+        assert abs(st["twiss"][f"eta_{plane}"] - s3[0, 2] / d2) < 1e-9 * abs(s3[0, 2] / d2)
+        assert abs(st["twiss"][f"norm_emit_{plane}"] - emit * p0c / mc2) < 1e-9 * emit * p0c / mc2
+    assert abs(st["mean_energy"] - p0c * (1 + pz.mean())) < 1e-15 * p0c * 4
+    assert abs(st["sigma_energy"] - p0c * np.sqrt(np.mean((pz - pz.mean()) ** 2))) < \
+        1e-12 * st["sigma_energy"]

@@ -270,6 +270,35 @@ int dfcsr_wake_tasks_pack(int k, const DfcsrHistory *hist, const DfcsrLattice *l
 int dfcsr_wake_batched_launch(const void *dev_tasks, const DfcsrWakeBatch *batch, int precision,
                               void *partials, int64_t partials_bytes, void *stream);
 
+/* ---- tracking: one lattice-element piece -----------------------------------
+ * Replaces bmadx.track_element on the element CSR2D.get_bmadx_element builds
+ * for a step piece (CSR.py:146-199, 227-294).  Bmad's canonical coordinates
+ * (x, px, y, py, z, pz) relative to the reference momentum p0c [eV]; mc2 the
+ * particle rest energy [eV].  Drift: exact.  SBend: exact motion in its
+ * uniform field of curvature g, hard-edge pole faces e1 (entrance) and e2
+ * (exit) with linear edge kicks (px += x g tan e, py -= y g tan(e - psi),
+ * psi = FINT 2 HGAP g (1 + sin^2 e) / cos e); the field's reference momentum is p0c (the
+ * reference always passes P0C = beam.init_energy).  Quadrupole: energy-
+ * scaled linear map per step with second-order z terms and the low-energy z
+ * correction.  Sextupole: drift - thin kick - drift per step.  Offsets and
+ * tilt (quadrupole, sextupole) as X_OFFSET / Y_OFFSET / TILT.  In place. */
+enum { DFCSR_ELE_DRIFT = 0, DFCSR_ELE_SBEND = 1, DFCSR_ELE_QUAD = 2, DFCSR_ELE_SEXTUPOLE = 3 };
+typedef struct {
+  int32_t type;        /* DFCSR_ELE_* */
+  int32_t num_steps;   /* quadrupole / sextupole steps (>= 1) */
+  double L;            /* length of this piece [m] */
+  double g;            /* sbend: G = angle / L of the element [1/m] */
+  double e1, e2;       /* sbend: face angles at this piece (0 where it has no face) */
+  double fint, hgap;   /* sbend entrance fringe: FINT, HGAP (vertical edge focusing) */
+  double fintx, hgapx; /* sbend exit fringe: FINTX, HGAPX */
+  double k1;           /* quadrupole K1 [1/m^2] */
+  double k2;           /* sextupole K2 [1/m^3] */
+  double x_offset, y_offset, tilt;
+} DfcsrElement;
+int dfcsr_track_element(double *x, double *px, double *y, double *py, double *z, double *pz,
+                        int64_t n, const DfcsrElement *ele, double p0c, double mc2,
+                        void *stream);
+
 /* ---- a16: mesh -> particle kick --------------------------------------------
  * Replaces Beam.apply_wakes (beams.py:108-131): bilinear
  * RegularGridInterpolator((xrange, zrange), step*W*1e6/E0, fill 0) at

@@ -25,6 +25,11 @@ Pinning (what checks each function, see tests/test_oracle_*.py):
   csr_integrand/wake    zero-density and linearity invariants, scalar loop
                         form, golden vectors from the reference module.
   apply_wakes           RegularGridInterpolator closed form on bilinear fields.
+  track_*               bmadx's Drift / SBend / Quadrupole single-particle results
+                        held by the reference's test notebook (1e-12); the
+                        symplectic condition of the maps' Jacobians; the linear
+                        limit against the r_gen6 transfer matrices.  Sextupole:
+                        symplectic only (parity unpinned).
   beam_statistics       twiss: a beam built with known Twiss/dispersion (exact
                         sample moments), beta*gamma - alpha^2 = 1, emittance =
                         sqrt(det) of the dispersion-free matrix; energy
@@ -185,6 +190,151 @@ def beam_statistics(x, px, y, py, z, pz, p0c, mc2):
     energy = (np.asarray(pz) + 1) * p0c
     return dict(twiss=out, mean_energy=np.mean(energy), sigma_energy=np.std(energy),
                 cov=np.cov([x, px, y, py, z, pz]))
+
+
+# ---------------------------------------------------------------------------
+# tracking: bmadx.track_element as CSR2D calls it (CSR.py:146-199, 227-294)
+# ---------------------------------------------------------------------------
+# bmadx (Bmad-X, git+https://github.com/bmad-sim/Bmad-X/, environment.yml:24,
+# no version pinned) is absent.  These are the published Bmad maps for the
+# element types the reference builds, in Bmad's canonical coordinates
+# (x, px = Px/P0, y, py, z = -beta c (t - t_ref), pz = dP/P0), restated from the
+# physics, and pinned on the Drift / SBend / Quadrupole single-particle results
+# the reference's test notebook holds (test/test_BmadX_tracking.ipynb cells
+# 25, 28, 31; tests/golden/ref_bmadx_tracking.json).  Sextupole: no
+# reference result exists -- parity unpinned.
+def sqrt_one(v):
+    """sqrt(1 + v) - 1 without cancellation."""
+    return v / (np.sqrt(1 + v) + 1)
+
+
+def low_energy_z_correction(pz, p0c, mc2, ds):
+    """ds (beta/beta0 - 1): the z slip of a particle slower than the
+    reference; its pz series where mc2 (beta0 pz)^2 < 3e-7 E_tot."""
+    pz = np.asarray(pz, dtype=np.float64)
+    beta = (1 + pz) * p0c / np.sqrt(((1 + pz) * p0c) ** 2 + mc2 ** 2)
+    beta0 = p0c / np.sqrt(p0c ** 2 + mc2 ** 2)
+    e_tot = np.sqrt(p0c ** 2 + mc2 ** 2)
+    series = ds * pz * (1 - 3 * (pz * beta0 ** 2) / 2 + pz ** 2 * beta0 ** 2 *
+                        (2 * beta0 ** 2 - (mc2 / e_tot) ** 2 / 2)) * (mc2 / e_tot) ** 2
+    exact = ds * (beta - beta0) / beta0
+    return np.where(mc2 * (beta0 * pz) ** 2 < 3e-7 * e_tot, series, exact)
+
+
+def track_drift(c, L, p0c, mc2):
+    """Exact field-free drift of length L; c = (x, px, y, py, z, pz)."""
+    x, px, y, py, z, pz = (np.asarray(v, dtype=np.float64) for v in c)
+    P = 1 + pz
+    Px, Py = px / P, py / P
+    pxy2 = Px ** 2 + Py ** 2
+    Pl = np.sqrt(1 - pxy2)
+    x = x + L * Px / Pl
+    y = y + L * Py / Pl
+    # z + L (beta/beta0 - 1/Pl), both differences without cancellation
+    dz = L * (sqrt_one((mc2 ** 2 * (2 * pz + pz ** 2)) / ((p0c * P) ** 2 + mc2 ** 2)) +
+              sqrt_one(-pxy2) / Pl)
+    return np.stack([x, px, y, py, z + dz, pz])
+
+
+def _quad_plane(k, L, P):
+    """One plane of a quadrupole, u'' = -k u (k > 0 focusing), u' = pu/P:
+    the 2 x 2 map and the path-length coefficients of -1/2 int u'^2 ds in
+    (u^2, u pu, pu^2)."""
+    sk = np.sqrt(np.abs(k))
+    th = sk * L
+    with np.errstate(invalid="ignore", divide="ignore"):
+        cx = np.where(k > 0, np.cos(th), np.where(k < 0, np.cosh(th), 1.0))
+        sx = np.where(k > 0, np.sin(th) / sk, np.where(k < 0, np.sinh(th) / sk, L))
+    m = (cx, sx / P, -k * sx * P, cx)
+    d = (k * (cx * sx - L) / 4, k * sx ** 2 / (2 * P), -(L + cx * sx) / (4 * P ** 2))
+    return m, d
+
+
+def _offset_set(c, x_off, y_off, tilt):
+    x, px, y, py, z, pz = c
+    x, y = x - x_off, y - y_off
+    ct, st = np.cos(tilt), np.sin(tilt)
+    return (x * ct + y * st, px * ct + py * st, -x * st + y * ct, -px * st + py * ct, z, pz)
+
+
+def _offset_unset(c, x_off, y_off, tilt):
+    x, px, y, py, z, pz = c
+    ct, st = np.cos(tilt), np.sin(tilt)
+    x, px, y, py = x * ct - y * st, px * ct - py * st, x * st + y * ct, px * st + py * ct
+    return (x + x_off, px, y + y_off, py, z, pz)
+
+
+def track_quadrupole(c, L, K1, p0c, mc2, num_steps=1, x_off=0.0, y_off=0.0, tilt=0.0):
+    """Quadrupole: energy-scaled linear map per step, second-order z terms
+    of the path length, low-energy z correction."""
+    x, px, y, py, z, pz = _offset_set(tuple(np.asarray(v, dtype=np.float64) for v in c),
+                                      x_off, y_off, tilt)
+    ds = L / num_steps
+    for _ in range(num_steps):
+        P = 1 + pz
+        k = K1 / P
+        tx, dx = _quad_plane(k, ds, P)
+        ty, dy = _quad_plane(-k, ds, P)
+        z = (z + dx[0] * x ** 2 + dx[1] * x * px + dx[2] * px ** 2 +
+             dy[0] * y ** 2 + dy[1] * y * py + dy[2] * py ** 2)
+        x, px = tx[0] * x + tx[1] * px, tx[2] * x + tx[3] * px
+        y, py = ty[0] * y + ty[1] * py, ty[2] * y + ty[3] * py
+        z = z + low_energy_z_correction(pz, p0c, mc2, ds)
+    return np.stack(_offset_unset((x, px, y, py, z, pz), x_off, y_off, tilt))
+
+
+def _bend_edge(c, g, e, fint=0.0, hgap=0.0):
+    """Hard-edge pole face rotated by e (linear edge kicks); the fringe
+    integral softens the vertical focusing by psi = fint (2 hgap) g
+    (1 + sin^2 e) / cos e (FINT > 0: parity unpinned)."""
+    x, px, y, py, z, pz = c
+    psi = fint * (2 * hgap) * g * (1 + np.sin(e) ** 2) / np.cos(e)
+    if e == 0 and psi == 0:
+        return (x, px, y, py, z, pz)
+    return (x, px + x * g * np.tan(e), y, py - y * g * np.tan(e - psi), z, pz)
+
+
+def track_sbend(c, L, g, e1, e2, p0c, mc2, fint=0.0, hgap=0.0, fintx=0.0, hgapx=0.0):
+    """Sector bend of curvature g (= angle / L) in its uniform field, exact:
+    the particle's circle of radius P_h / g meets the exit face at angle
+    g L; entrance and exit faces rotated by e1, e2.  Written without the
+    cancellations of r - 1/g: px1 = px cos + (p_s - 1 - g x) sin,
+    x1 = x cos + ((p_s1 - 1) - cos (p_s - 1) + px sin) / g,
+    phi = g L + asin(px / P_h) - asin(px1 / P_h), path = phi P / g."""
+    c = tuple(np.asarray(v, dtype=np.float64) for v in c)
+    if g == 0:
+        return np.stack(_bend_edge(tuple(track_drift(_bend_edge(c, g, e1, fint, hgap), L, p0c,
+                                                     mc2)), g, e2, fintx, hgapx))
+    x, px, y, py, z, pz = _bend_edge(c, g, e1, fint, hgap)
+    P = 1 + pz
+    th = g * L
+    ct, st = np.cos(th), np.sin(th)
+    pxy2 = px ** 2 + py ** 2
+    ps_1 = sqrt_one(2 * pz + pz ** 2 - pxy2)
+    Ph = np.sqrt(P ** 2 - py ** 2)
+    px1 = px * ct + (ps_1 - g * x) * st
+    ps1_1 = sqrt_one(2 * pz + pz ** 2 - py ** 2 - px1 ** 2)
+    x1 = x * ct + (ps1_1 - ct * ps_1 + px * st) / g
+    dphi = np.arcsin(px / Ph) - np.arcsin(px1 / Ph)
+    y1 = y + py * (L + dphi / g)
+    beta = P * p0c / np.sqrt((P * p0c) ** 2 + mc2 ** 2)
+    beta0 = p0c / np.sqrt(p0c ** 2 + mc2 ** 2)
+    z1 = z + L * beta / beta0 - P * (L + dphi / g)
+    out = _bend_edge((x1, px1, y1, py, z1, P - 1), g, e2, fintx, hgapx)
+    return np.stack(out)
+
+
+def track_sextupole(c, L, K2, p0c, mc2, num_steps=1, x_off=0.0, y_off=0.0, tilt=0.0):
+    """Sextupole as drift - thin kick - drift per step (the kick of K2 ds:
+    dpx = -K2 ds (x^2 - y^2) / 2, dpy = K2 ds x y).  Parity unpinned."""
+    cc = _offset_set(tuple(np.asarray(v, dtype=np.float64) for v in c), x_off, y_off, tilt)
+    ds = L / num_steps
+    for _ in range(num_steps):
+        cc = tuple(track_drift(cc, ds / 2, p0c, mc2))
+        x, px, y, py, z, pz = cc
+        cc = (x, px - K2 * ds * (x * x - y * y) / 2, y, py + K2 * ds * x * y, z, pz)
+        cc = tuple(track_drift(cc, ds / 2, p0c, mc2))
+    return np.stack(_offset_unset(cc, x_off, y_off, tilt))
 
 
 # ---------------------------------------------------------------------------

@@ -57,6 +57,16 @@ class Lattice(ctypes.Structure):
                 ("ds", c_f64), ("distance", c_dp), ("rho", c_dp)]
 
 
+DFCSR_ELE_DRIFT, DFCSR_ELE_SBEND, DFCSR_ELE_QUAD, DFCSR_ELE_SEXTUPOLE = 0, 1, 2, 3
+
+
+class Element(ctypes.Structure):
+    _fields_ = [("type", c_i32), ("num_steps", c_i32), ("L", c_f64), ("g", c_f64),
+                ("e1", c_f64), ("e2", c_f64), ("fint", c_f64), ("hgap", c_f64),
+                ("fintx", c_f64), ("hgapx", c_f64), ("k1", c_f64), ("k2", c_f64),
+                ("x_offset", c_f64), ("y_offset", c_f64), ("tilt", c_f64)]
+
+
 class WakeBatch(ctypes.Structure):
     _fields_ = [("tasks", c_i32), ("blocks_far", c_i32), ("blocks_near", c_i32),
                 ("observers", c_i32), ("partial_bytes", c_i64), ("precision", c_i32)]
@@ -113,6 +123,8 @@ _SIGNATURES = [
                                       ctypes.POINTER(WakeBatch)]),
     ("dfcsr_wake_batched_launch", c_i32, [c_dp, ctypes.POINTER(WakeBatch), c_i32, c_dp, c_i64,
                                           c_dp]),
+    ("dfcsr_track_element", c_i32, [c_dp, c_dp, c_dp, c_dp, c_dp, c_dp, c_i64,
+                                    ctypes.POINTER(Element), c_f64, c_f64, c_dp]),
     ("dfcsr_apply_wakes", c_i32, [c_dp, c_dp, c_dp, c_dp, c_i64, c_dp, c_dp, c_i32, c_i32, c_f64,
                                   c_f64, c_f64, c_f64, c_f64, c_f64, c_f64, c_f64, c_i32, c_dp]),
 ]

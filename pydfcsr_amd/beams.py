@@ -11,12 +11,12 @@ reduction; the per-step record (Twiss of x and y, energy mean and spread)
 from another (dfcsr_beam_statistics); the kick is the gfx950 mesh->particle
 kernel.
 
-Tracking replaces bmadx's track_element, which this build cannot use: each
-step applies the linear transfer map of the element piece (r_gen6.py:4-122
-conventions: sector-bend body, hard-edge pole-face focusing at the piece
-that holds the entrance/exit face, quadrupoles; thin sextupole kick).  It is
-host-side plumbing outside the CSR step, and its numbers are not pinned
-against Bmad-X.
+Tracking replaces bmadx's track_element (absent here) with the gfx950
+element kernel (dfcsr_track_element): exact drift, exact sector bend with
+hard-edge faces, quadrupole with second-order path length and low-energy z
+terms, sextupole drift-kick-drift; the drift, bend and quadrupole maps
+reproduce the reference notebook's bmadx results.  transfer_matrix is the
+linear (r_gen6.py) map of a piece, kept for linear optics checks.
 """
 from __future__ import annotations
 
@@ -75,17 +75,42 @@ def transfer_matrix(L, angle=0.0, k1=0.0, E1=0.0, E2=0.0, hgap=0.0):
     return R
 
 
-def element_map(ele: dict, DL: float, entrance=False, exit=False):
-    """Map of a piece of length DL of `ele` (CSR.py:146-199 element pieces)."""
+def bmadx_element(ele: dict, DL: float, entrance=False, exit=False) -> _lib.Element:
+    """The element of a step piece of length DL, as CSR2D.get_bmadx_element
+    builds it for bmadx (CSR.py:146-199): a dipole's G (G key, else angle/L),
+    its faces E1 / E2 and their fringe (FINT, HGAP / FINTX, HGAPX) only at the
+    piece that holds them; a quad's K1, a sextupole's K2, with X_OFFSET,
+    Y_OFFSET, TILT and NUM_STEPS when given."""
     typ = ele["type"]
+    e = _lib.Element()
+    e.L = float(DL)
+    e.num_steps = int(ele.get("NUM_STEPS", 1))
     if typ == "dipole":
-        G = ele["G"] if "G" in ele else ele["angle"] / ele["L"]
-        E1 = ele.get("E1", 0.0) if entrance else 0.0
-        E2 = ele.get("E2", 0.0) if exit else 0.0
-        return transfer_matrix(DL, angle=G * DL, E1=E1, E2=E2, hgap=ele.get("HGAP", 0.0) or 0.0)
-    if typ == "quad":
-        return transfer_matrix(DL, k1=ele["K1"])
-    return transfer_matrix(DL)   # drift, sextupole body (kick applied separately)
+        e.type = _lib.DFCSR_ELE_SBEND
+        e.g = float(ele["G"]) if "G" in ele else float(ele["angle"]) / float(ele["L"])
+        if entrance:
+            e.e1 = float(ele.get("E1", 0.0))
+            e.fint = float(ele.get("FINT", 0.0) or 0.0)
+            e.hgap = float(ele.get("HGAP", 0.0) or 0.0)
+        if exit:
+            e.e2 = float(ele.get("E2", 0.0))
+            e.fintx = float(ele.get("FINTX", ele.get("FINT", 0.0)) or 0.0)
+            e.hgapx = float(ele.get("HGAPX", ele.get("HGAP", 0.0)) or 0.0)
+    elif typ == "drift":
+        e.type = _lib.DFCSR_ELE_DRIFT
+    elif typ in ("quad", "sextupole"):
+        e.type = _lib.DFCSR_ELE_QUAD if typ == "quad" else _lib.DFCSR_ELE_SEXTUPOLE
+        if typ == "quad":
+            e.k1 = float(ele["K1"])
+        else:
+            e.k2 = float(ele["K2"])
+        e.x_offset = float(ele.get("X_OFFSET", 0.0))
+        e.y_offset = float(ele.get("Y_OFFSET", 0.0))
+        e.tilt = float(ele.get("TILT", 0.0))
+    else:
+        raise ValueError(f"element type {typ!r}: the reference builds dipole, drift, quad and "
+                         "sextupole pieces only (CSR.py:153-197)")
+    return e
 
 
 class Beam:
@@ -174,13 +199,11 @@ class Beam:
         self._mean_x = m["mean_x"]
         self._mean_z = m["mean_z"]
 
-    def track(self, element_map_6x6, step_size, update_step=True, sextupole_k2l=0.0):
-        M = torch.from_numpy(element_map_6x6).to(self.device)
-        self.coords = M @ self.coords
-        if sextupole_k2l:
-            x, y = self.coords[0], self.coords[2]
-            self.coords[1] -= 0.5 * sextupole_k2l * (x * x - y * y)
-            self.coords[3] += sextupole_k2l * x * y
+    def track(self, element, step_size, update_step=True):   # beams.py:101-106
+        """Tracks through one element piece (an Element from bmadx_element)
+        on the device, in place."""
+        self.coords = self.coords.contiguous()
+        kernels.track_element(self.coords, element, self.init_energy, MC2)
         self.position += step_size
         if update_step:
             self.step += 1

@@ -20,7 +20,7 @@ import numpy as np
 import torch
 
 from . import _lib, kernels
-from .beams import Beam, element_map
+from .beams import Beam, bmadx_element
 from .deposit import DF_tracker
 from .lattice import Lattice
 from .params import CSR_params, Integration_params
@@ -150,11 +150,12 @@ class CSR2D:
         else:
             self.formation_length = (3 * R ** 2 * phi ** 4) / (4 * (-6 * sigma_z + R * phi ** 3))
 
+    def get_bmadx_element(self, ele, DL, entrance=False, exit=False):   # CSR.py:146-199
+        return bmadx_element(self.lattice.lattice_config[ele], DL, entrance=entrance, exit=exit)
+
     def _track(self, ele_name, DL, entrance=False, exit=False, update_step=True):
-        ele = self.lattice.lattice_config[ele_name]
-        M = element_map(ele, DL, entrance=entrance, exit=exit)
-        k2l = ele["K2"] * DL if ele["type"] == "sextupole" else 0.0
-        self.beam.track(M, DL, update_step=update_step, sextupole_k2l=k2l)
+        element = self.get_bmadx_element(ele_name, DL, entrance=entrance, exit=exit)
+        self.beam.track(element, DL, update_step=update_step)
 
     # -- CSR.py:202-358 ---------------------------------------------------------------
     def run(self, stop_time=None, debug=False, max_steps=None, on_step=None):

@@ -97,6 +97,19 @@ def statistics_dict(v) -> dict:
                 mean=v[_lib.STAT_MEAN:_lib.STAT_MEAN + 6].copy(), cov=cov)
 
 
+def track_element(coords: torch.Tensor, ele: _lib.Element, p0c, mc2):
+    """Tracks the (6, N) device coordinates through one element piece, in
+    place (dfcsr_track_element)."""
+    lib = _lib.load()
+    _f64(coords, "coords")
+    if coords.dim() != 2 or coords.shape[0] != 6:
+        raise DfcsrError(f"track_element: coords must be (6, N), got {tuple(coords.shape)}")
+    rows = [ptr(coords[k]) for k in range(6)]
+    check(lib.dfcsr_track_element(*rows, coords.shape[1], ctypes.byref(ele), float(p0c),
+                                  float(mc2), stream_handle(coords.device)), "track_element")
+    return coords
+
+
 def histogram_cic_2d(q1, q2, w, nbins_1, bins_start_1, bins_end_1, nbins_2, bins_start_2,
                      bins_end_2, ws: _lib.Workspace, want_cells=False):
     """Device histogram_cic_2d; w=None means unit weights.  With want_cells,

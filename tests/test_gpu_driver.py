@@ -1,8 +1,9 @@
 """Driver-level parity: CSR2D's run loop (CSR.py:202-358) against the oracle,
 step by step, on a short dipole + drift lattice.
 
-Tracking is outside the accelerated path (its transfer maps are not pinned),
-so at every step the oracle is fed the particles the driver tracked: after
+Tracking is pinned separately (tests of dfcsr_track_element against the
+reference notebook's bmadx results), so at every step the oracle is fed the
+particles the driver tracked: after
 the driver's history update (external_wake hook) the oracle repeats
 get_DF -> append_DF -> append_interpolant -> build_interpolant
 (deposit.py:145-426) on the same particles, the mesh (CSR.py:361-394), the

@@ -262,3 +262,90 @@ def test_beam_statistics_planes_and_energy():
     assert abs(st["mean_energy"] - p0c * (1 + pz.mean())) < 1e-15 * p0c * 4
     assert abs(st["sigma_energy"] - p0c * np.sqrt(np.mean((pz - pz.mean()) ** 2))) < \
         1e-12 * st["sigma_energy"]
+
+
+# -- tracking (bmadx.track_element as CSR.py:146-199 builds it) -------------------
+def _gold_tracking():
+    import json
+    import os
+    d = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                    "ref_bmadx_tracking.json")))
+    p = d["particle_in"]
+    return d, [p[k] for k in ("x", "px", "y", "py", "z", "pz")], p["p0c"], p["mc2"]
+
+
+_KEYS = ("x", "px", "y", "py", "z", "pz")
+
+
+def _track_case(c, case, p0c, mc2):
+    e = case["element"]
+    if e == "Drift":
+        return O.track_drift(c, case["L"], p0c, mc2)
+    if e == "SBend":
+        return O.track_sbend(c, case["L"], case["G"], case["E1"], case["E2"], p0c, mc2)
+    return O.track_quadrupole(c, case["L"], case["K1"], p0c, mc2)
+
+
+def test_tracking_matches_bmadx_single_particles():
+    """The reference notebook's bmadx results (test_BmadX_tracking.ipynb
+    cells 25, 28, 31) within 1e-12 relative per coordinate."""
+    d, c, p0c, mc2 = _gold_tracking()
+    for case in d["cases"]:
+        r = _track_case(c, case, p0c, mc2)
+        g = np.array([case["out"][k] for k in _KEYS])
+        assert np.all(np.abs(r - g) <= 1e-12 * np.abs(g)), (case["element"], r - g)
+
+
+def _jacobian(f, c0, h=1e-7):
+    c0 = np.asarray(c0, dtype=np.float64)
+    J = np.zeros((6, 6))
+    for j in range(6):
+        e = np.zeros(6)
+        e[j] = h
+        J[:, j] = (np.asarray(f(c0 + e)) - np.asarray(f(c0 - e))) / (2 * h)
+    return J
+
+
+@pytest.mark.parametrize("name", ["drift", "quad_f", "quad_d", "bend", "bend_edges", "sext",
+                                  "quad_tilt"])
+def test_tracking_maps_are_symplectic(name):
+    """J^T S J = S for the Jacobian at a generic point: ties the z update to
+    the energy dependence of the transverse map, so a wrong sign or a dropped
+    path-length term fails."""
+    p0c, mc2 = 1e9, 510998.95069
+    maps = {
+        "drift": lambda c: O.track_drift(c, 0.7, p0c, mc2),
+        "quad_f": lambda c: O.track_quadrupole(c, 0.3, 4.0, p0c, mc2),
+        "quad_d": lambda c: O.track_quadrupole(c, 0.3, -6.0, p0c, mc2, num_steps=3),
+        "bend": lambda c: O.track_sbend(c, 0.3, 0.05 / 0.3, 0.0, 0.0, p0c, mc2),
+        "bend_edges": lambda c: O.track_sbend(c, 0.3, -0.08 / 0.3, 0.02, -0.08, p0c, mc2,
+                                              fint=0.6, hgap=0.017, fintx=0.5, hgapx=0.02),
+        "sext": lambda c: O.track_sextupole(c, 0.2, 30.0, p0c, mc2, num_steps=2),
+        "quad_tilt": lambda c: O.track_quadrupole(c, 0.3, 4.0, p0c, mc2, x_off=1e-4,
+                                                  y_off=-2e-4, tilt=0.3),
+    }
+    c0 = np.array([1.2e-3, -0.7e-3, 0.8e-3, 0.5e-3, 0.3e-3, 2e-3])
+    J = _jacobian(maps[name], c0)
+    S = np.kron(np.eye(3), np.array([[0.0, 1.0], [-1.0, 0.0]]))
+    assert np.abs(J.T @ S @ J - S).max() < 1e-7, np.abs(J.T @ S @ J - S).max()
+
+
+@pytest.mark.parametrize("L,angle,k1,e1,e2", [(0.3, 0.0, 0.0, 0.0, 0.0), (0.3, 0.0, 5.0, 0.0, 0.0),
+                                            (0.3, 0.0, -5.0, 0.0, 0.0),
+                                            (0.3, 0.05, 0.0, 0.05, 0.0),
+                                            (0.3, -0.05, 0.0, 0.0, -0.05)])
+def test_tracking_linear_limit_is_r_gen6(L, angle, k1, e1, e2):
+    """At small amplitude the maps' transverse rows (x, px, y, py over x, px,
+    y, py, pz) equal the r_gen6 transfer matrix (r_gen6.py:4-122)."""
+    from pydfcsr_amd.beams import transfer_matrix
+    p0c, mc2 = 1e10, 510998.95069
+    if angle:
+        f = lambda c: O.track_sbend(c, L, angle / L, e1, e2, p0c, mc2)
+    elif k1:
+        f = lambda c: O.track_quadrupole(c, L, k1, p0c, mc2)
+    else:
+        f = lambda c: O.track_drift(c, L, p0c, mc2)
+    J = _jacobian(f, np.zeros(6), h=1e-9)
+    R = transfer_matrix(L, angle=angle, k1=k1, E1=e1, E2=e2)
+    rows, cols = [0, 1, 2, 3], [0, 1, 2, 3, 5]
+    assert np.allclose(J[np.ix_(rows, cols)], R[np.ix_(rows, cols)], rtol=1e-6, atol=1e-8)

@@ -178,9 +178,19 @@ typedef struct {
   const double *data;    /* [Xi][Zi][cap][DFCSR_HIST_REC64] or NULL */
   const float *data32;   /* [Xi][cap][DFCSR_HIST_CHUNKS32][Zi][4] or NULL */
   const float *data32t;  /* [Xi][Zi][cap][DFCSR_HIST_REC32] or NULL */
+  /* Optional scratch for FP64 wakes, [recent_planes][Xi][Zi][DFCSR_RECENT_CELL]
+   * doubles (or NULL): each FP64 wake first copies the latest
+   * K = min(recent_planes, nt) planes of the FP64 ring into it, cell-major
+   * (the five fields of cell (x, z), then a pad), and its near-region
+   * lookups that fall on those planes read the copy -- a third of the
+   * bytes of the corner-packed records, neighbouring z in neighbouring
+   * cells.  Results are identical with or without it. */
+  double *recent;
   int32_t cap, head, nt, Xi, Zi;
+  int32_t recent_planes;
   double min_t, min_x, min_z, dt, dx, dz;
 } DfcsrHistory;
+#define DFCSR_RECENT_CELL 6
 
 /* Lattice tables (lattice.py:4-110, 136-150): table[k][0..5] =
  * coords_x, coords_y, n_x, n_y, tau_x, tau_y at s = min_s + k*ds, k < ns;

@@ -31,6 +31,7 @@ TWISS_KEYS = ("alpha", "beta", "gamma", "emit", "eta", "etap", "norm_emit")
 HIST_REC64 = 20   # corner-packed records: field f, corner c at [4 f + c]
 HIST_REC32 = 20
 HIST_CHUNKS32 = 5
+RECENT_CELL = 6   # doubles per cell of the FP64 wake's recent-plane copy
 # rings: FP64 (Xi, Zi, cap, 20); FP32 chunked (Xi, cap, 5, Zi, 4); FP32
 # time-innermost (Xi, Zi, cap, 20)
 FIELDS = ("density", "density_x", "density_z", "vx", "vx_x")   # DFCSR_F_* order
@@ -46,10 +47,10 @@ class RegridJob(ctypes.Structure):
 
 
 class History(ctypes.Structure):
-    _fields_ = [("data", c_dp), ("data32", c_dp), ("data32t", c_dp), ("cap", c_i32),
-                ("head", c_i32), ("nt", c_i32), ("Xi", c_i32), ("Zi", c_i32), ("min_t", c_f64),
-                ("min_x", c_f64), ("min_z", c_f64), ("dt", c_f64), ("dx", c_f64),
-                ("dz", c_f64)]
+    _fields_ = [("data", c_dp), ("data32", c_dp), ("data32t", c_dp), ("recent", c_dp),
+                ("cap", c_i32), ("head", c_i32), ("nt", c_i32), ("Xi", c_i32), ("Zi", c_i32),
+                ("recent_planes", c_i32), ("min_t", c_f64), ("min_x", c_f64), ("min_z", c_f64),
+                ("dt", c_f64), ("dx", c_f64), ("dz", c_f64)]
 
 
 class Lattice(ctypes.Structure):

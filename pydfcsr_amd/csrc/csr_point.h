@@ -82,7 +82,9 @@ __device__ inline const double *hist_rec(const DfcsrHistory &H, int ti, int xi, 
 
 // Trilinear lookup of all five fields at (t, x, z) with interpolate3D's
 // truncation and validity rules.  Returns false (and zeros) when invalid.
-template <bool kRecip>
+// kRecent: lookups whose planes lie in the cell-major copy of the latest
+// planes (H.recent) read it; the values, and so the result, are the same.
+template <bool kRecip, bool kRecent = false>
 __device__ inline bool hist_lookup5(const DfcsrHistory &H, const HistAxes &A, double tq,
                                     double xq, double zq, double out[5]) {
   double ut = kRecip ? (tq - A.min_t) * A.it : (tq - A.min_t) / A.dt;
@@ -94,6 +96,46 @@ __device__ inline bool hist_lookup5(const DfcsrHistory &H, const HistAxes &A, do
     return false;
   }
   const double td = T.frac, xd = X.frac, zd = Z.frac;
+  if (kRecent) {
+    const int c0 = H.nt - min(H.recent_planes, H.nt);
+    if (T.i0 >= c0) {
+      // cell (x, z) of copied plane k at ((k Xi + x) Zi + z) * 6: fields 0..3 as
+      // two 16-byte loads and field 4, per corner (x0,z0), (x0,z1), (x1,z0), (x1,z1)
+      const int64_t Zi = H.Zi;
+      const double2 *b0 = reinterpret_cast<const double2 *>(
+          H.recent + (((int64_t)(T.i0 - c0) * H.Xi + X.i0) * Zi + Z.i0) * DFCSR_RECENT_CELL);
+      const double2 *b1 = reinterpret_cast<const double2 *>(
+          H.recent + (((int64_t)(T.i1 - c0) * H.Xi + X.i0) * Zi + Z.i0) * DFCSR_RECENT_CELL);
+      const int64_t dz = (int64_t)(Z.i1 - Z.i0) * 3, dx = (int64_t)(X.i1 - X.i0) * Zi * 3;
+      const int64_t off[4] = {0, dz, dx, dx + dz};
+      double v0[4][5], v1[4][5];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const double2 a = b0[off[c]], b = b0[off[c] + 1], e = b1[off[c]], g = b1[off[c] + 1];
+        v0[c][0] = a.x;
+        v0[c][1] = a.y;
+        v0[c][2] = b.x;
+        v0[c][3] = b.y;
+        v0[c][4] = reinterpret_cast<const double *>(b0 + off[c] + 2)[0];
+        v1[c][0] = e.x;
+        v1[c][1] = e.y;
+        v1[c][2] = g.x;
+        v1[c][3] = g.y;
+        v1[c][4] = reinterpret_cast<const double *>(b1 + off[c] + 2)[0];
+      }
+#pragma unroll
+      for (int f = 0; f < 5; ++f) {
+        double c00 = v0[0][f] * (1 - td) + v1[0][f] * td;
+        double c01 = v0[1][f] * (1 - td) + v1[1][f] * td;
+        double c10 = v0[2][f] * (1 - td) + v1[2][f] * td;
+        double c11 = v0[3][f] * (1 - td) + v1[3][f] * td;
+        double c0v = c00 * (1 - xd) + c10 * xd;
+        double c1v = c01 * (1 - xd) + c11 * xd;
+        out[f] = c0v * (1 - zd) + c1v * zd;
+      }
+      return true;
+    }
+  }
   // corner-packed records: p0 at (t0, x0, z0), p1 at (t1, x0, z0); element
   // f*4 + {0,1,2,3} = (x0,z0), (x0,z1), (x1,z0), (x1,z1)
   const double *p0 = hist_rec(H, T.i0, X.i0, Z.i0), *p1 = hist_rec(H, T.i1, X.i0, Z.i0);
@@ -156,7 +198,7 @@ __device__ inline Column make_column(const DfcsrLattice &L, const Observer &o, d
 // lookup falls outside the history, where every gathered field is zero and
 // the reference's formula gives exactly 0 -- except at R == 0, where its
 // divisions give 0/0: that point is evaluated (with zero fields) as it is.
-template <bool kRecip>
+template <bool kRecip, bool kRecent = false>
 __device__ inline bool csr_point(const DfcsrHistory &H, const HistAxes &A, const Observer &o,
                                  const Column &c, double xp, double &Iz, double &Ix) {
   const double rx = c.ax - xp * c.L.nx;
@@ -164,7 +206,7 @@ __device__ inline bool csr_point(const DfcsrHistory &H, const HistAxes &A, const
   const double R = sqrt(rx * rx + ry * ry);
   const double t_ret = o.t - R;
   double f[5];
-  if (!hist_lookup5<kRecip>(H, A, t_ret, xp, c.sp - t_ret, f) && R != 0.0) {
+  if (!hist_lookup5<kRecip, kRecent>(H, A, t_ret, xp, c.sp - t_ret, f) && R != 0.0) {
     Iz = 0.0;
     Ix = 0.0;
     return false;

@@ -445,8 +445,9 @@ __device__ inline void observer_point(const WakeTask &T, int i, double *__restri
 
 // Integrates phase ph of the mesh point whose observer data is od; every
 // thread returns the block sums of wz_k wx_j (I_z, I_x) over the phase.
-// kTI (mixed only): read the time-innermost FP32 ring.
-template <bool kMixed, bool kTI>
+// kTI (mixed only): read the time-innermost FP32 ring.  kRecent (FP64
+// only): lookups on the latest planes read the cell-major copy H.recent.
+template <bool kMixed, bool kTI, bool kRecent = false>
 __device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int ph, double *red,
                            double &out_z, double &out_x) {
   const DfcsrHistory &H = T.H;
@@ -536,7 +537,7 @@ __device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int
         const double ux = (xp - A.min_x) * A.ix;
         if (!(ux > -1.0 && ux < (double)H.Xi)) continue;
         double Iz, Ix;
-        if (!csr_point<true>(H, A, o, col, xp, Iz, Ix)) continue;
+        if (!csr_point<true, kRecent>(H, A, o, col, xp, Iz, Ix)) continue;
         const double wx = xa.w(j);
         cz += wx * Iz;
         cx += wx * Ix;
@@ -551,8 +552,9 @@ __device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int
 
 // Workgroup l of a task's far (kNear = false) or near launch -> (mesh
 // point, phase); writes the phase's partial sums.  kTI (mixed only): the
-// phases read the time-innermost FP32 ring, else the chunked one.
-template <bool kMixed, bool kNear, bool kTI>
+// phases read the time-innermost FP32 ring, else the chunked one; kRecent
+// (FP64 only): the latest planes come from the cell-major copy.
+template <bool kMixed, bool kNear, bool kTI, bool kRecent = false>
 __device__ inline void run_phase_block(const WakeTask &T, int l, double *__restrict__ partials,
                                        double *red) {
   const int xb = T.P.mesh_xbins, zb = T.P.mesh_zbins;
@@ -566,7 +568,8 @@ __device__ inline void run_phase_block(const WakeTask &T, int l, double *__restr
   i = (rem - q * xb) * zb + zl;
   ph = kNear ? T.nA + q : q;
   double sz, sx;
-  wake_phase<kMixed, kTI>(T, partials + T.obs + (int64_t)i * kObsStride, ph, red, sz, sx);
+  wake_phase<kMixed, kTI, kRecent>(T, partials + T.obs + (int64_t)i * kObsStride, ph, red, sz,
+                                   sx);
   if (threadIdx.x == 0) {
     double *p = partials + T.part + ((int64_t)i * (T.nA + T.nB) + ph) * 2;
     p[0] = sz;
@@ -581,13 +584,45 @@ __host__ __device__ inline bool mixed_reads_ti(const DfcsrHistory &H, bool near)
   return near ? (H.data32 == nullptr) : (H.data32t != nullptr);
 }
 
-template <bool kMixed, bool kNear, bool kTI>
-__global__ __launch_bounds__(kThreads, DFCSR_WAKE_MINBLOCKS) void wake_phase_kernel(
+// The FP64 near phases with the recent-plane copy hold both lookups' loads:
+// capped at 3 workgroups per CU (166 registers), the others at 2.
+template <bool kMixed, bool kNear, bool kTI, bool kRecent = false>
+__global__ __launch_bounds__(kThreads, kRecent ? 3 : DFCSR_WAKE_MINBLOCKS) void wake_phase_kernel(
     WakeTask T, double *__restrict__ partials) {
   __shared__ double red[kThreads / 64];
   const int b = blockIdx.x, nb = gridDim.x;
-  run_phase_block<kMixed, kNear, kTI>(
+  run_phase_block<kMixed, kNear, kTI, kRecent>(
       T, kNear ? xcd_group(b, nb, kNearGroup) : xcd_remap(b, nb), partials, red);
+}
+
+// Copies the latest K = min(recent_planes, nt) planes of the FP64 ring into
+// H.recent, cell-major: cell (x, z) of plane k at ((k Xi + x) Zi + z) * 6 =
+// the five fields at corner (x, z) of its record, then a zero pad.
+__device__ inline void recent_copy(const DfcsrHistory &H, int64_t c, int64_t stride) {
+  const int K = min(H.recent_planes, H.nt), c0 = H.nt - K;
+  const int64_t cells = (int64_t)H.Xi * H.Zi;
+  for (; c < cells; c += stride) {
+    const int xi = (int)(c / H.Zi), zi = (int)(c - (int64_t)xi * H.Zi);
+    for (int k = 0; k < K; ++k) {
+      const double *rec = hist_rec(H, c0 + k, xi, zi);
+      double2 *o = reinterpret_cast<double2 *>(H.recent + ((int64_t)k * cells + c) * DFCSR_RECENT_CELL);
+      o[0] = make_double2(rec[0], rec[4]);
+      o[1] = make_double2(rec[8], rec[12]);
+      o[2] = make_double2(rec[16], 0.0);
+    }
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void recent_copy_kernel(DfcsrHistory H) {
+  recent_copy(H, (int64_t)blockIdx.x * kThreads + threadIdx.x, (int64_t)gridDim.x * kThreads);
+}
+
+// blockIdx.y = task; tasks without a copy buffer return at once.
+__global__ __launch_bounds__(kThreads) void recent_copy_batched_kernel(
+    const WakeTask *__restrict__ tasks) {
+  const DfcsrHistory &H = tasks[blockIdx.y].H;
+  if (!H.recent) return;
+  recent_copy(H, (int64_t)blockIdx.x * kThreads + threadIdx.x, (int64_t)gridDim.x * kThreads);
 }
 
 // Last task whose first index (workgroup or observer) is <= b.
@@ -617,6 +652,8 @@ __global__ __launch_bounds__(kThreads, DFCSR_WAKE_MINBLOCKS) void wake_phase_bat
       run_phase_block<true, kNear, true>(T, l, partials, red);
     else
       run_phase_block<true, kNear, false>(T, l, partials, red);
+  } else if (kNear && T.H.recent) {   // per task: a recent-plane copy or not
+    run_phase_block<false, kNear, false, true>(T, l, partials, red);
   } else {
     run_phase_block<false, kNear, false>(T, l, partials, red);
   }
@@ -688,6 +725,9 @@ int check_params(const DfcsrWakeParams *p, const char *who) {
 int check_rings(const DfcsrHistory &H, int precision, const char *who) {
   if (precision == DFCSR_PREC_FP64 && !H.data)
     return set_error(DFCSR_EINVAL, "%s: FP64 precision needs the FP64 history ring", who);
+  if (H.recent && H.recent_planes < 1)
+    return set_error(DFCSR_EINVAL, "%s: recent-plane buffer with %d planes", who,
+                     H.recent_planes);
   if (precision == DFCSR_PREC_MIXED && !H.data32 && !H.data32t)
     return set_error(DFCSR_EINVAL, "%s: mixed precision needs an FP32 history ring", who);
   return DFCSR_OK;
@@ -778,7 +818,15 @@ extern "C" int dfcsr_wake_mesh(const DfcsrHistory *hist, const DfcsrLattice *lat
       hipLaunchKernelGGL((wake_phase_kernel<true, true, false>), gb, dim3(kThreads), 0, s, T, part);
   } else {
     hipLaunchKernelGGL((wake_phase_kernel<false, false, false>), ga, dim3(kThreads), 0, s, T, part);
-    hipLaunchKernelGGL((wake_phase_kernel<false, true, false>), gb, dim3(kThreads), 0, s, T, part);
+    if (hist->recent) {
+      const int64_t cells = (int64_t)hist->Xi * hist->Zi;
+      const int gc = (int)std::min<int64_t>((cells + kThreads - 1) / kThreads, 4096);
+      hipLaunchKernelGGL(recent_copy_kernel, dim3(gc), dim3(kThreads), 0, s, *hist);
+      hipLaunchKernelGGL((wake_phase_kernel<false, true, false, true>), gb, dim3(kThreads), 0, s, T,
+                         part);
+    } else {
+      hipLaunchKernelGGL((wake_phase_kernel<false, true, false>), gb, dim3(kThreads), 0, s, T, part);
+    }
   }
   hipLaunchKernelGGL(wake_finalize_kernel, gf, dim3(kThreads), 0, s, T, (const double *)part);
   DFCSR_LAUNCH_CHECK("wake_mesh");
@@ -864,6 +912,7 @@ extern "C" int dfcsr_wake_batched_launch(const void *dev_tasks, const DfcsrWakeB
     hipLaunchKernelGGL((wake_phase_batched_kernel<true, true>), gb, dim3(kThreads), 0, s, dt, k, part);
   } else if (precision == DFCSR_PREC_FP64) {
     hipLaunchKernelGGL((wake_phase_batched_kernel<false, false>), ga, dim3(kThreads), 0, s, dt, k, part);
+    hipLaunchKernelGGL(recent_copy_batched_kernel, dim3(512, k), dim3(kThreads), 0, s, dt);
     hipLaunchKernelGGL((wake_phase_batched_kernel<false, true>), gb, dim3(kThreads), 0, s, dt, k, part);
   } else {
     return set_error(DFCSR_EINVAL, "wake_batched_launch: unknown precision %d", precision);

@@ -18,7 +18,9 @@ hist32t keep the slot innermost (the two planes of a lookup are neighbours:
 what the far region's lookups want); hist32 stores each field's four corners
 as one 16-byte chunk, z fastest, so neighbouring z of one plane are
 neighbours: what the near regions' lookups want.  keep_fp64 / keep_fp32
-choose the float64 ring and the two float32 rings.)
+choose the float64 ring and the two float32 rings.)  FP64 wakes also get a
+scratch buffer, recent[K][Xi][Zi][6], that each wake fills with a cell-major
+copy of the latest K planes for its near-region lookups.
 Appending writes slot (head + nt) % cap; popping advances head.  Nothing is
 copied per step: the reference's np.array(deque) stacking of the whole
 history (deposit.py:422-426) becomes a descriptor update.
@@ -55,9 +57,11 @@ class SliceMeta:
 
 class DF_tracker:
     def __init__(self, input_dic=None, device="cuda", keep_fp32=True, capacity=16,
-                 keep_fp64=True):
+                 keep_fp64=True, recent_planes=4):
         """keep_fp64: hold the float64 history ring (FP64 wakes, lookups);
-        keep_fp32: hold the two float32 rings (mixed-precision wakes)."""
+        keep_fp32: hold the two float32 rings (mixed-precision wakes);
+        recent_planes: planes of the FP64 wake's cell-major copy of the latest
+        slices (0: none)."""
         self.params = Deposition_params(input_dic)
         p = self.params
         self.xbins, self.zbins, self.xlim, self.zlim = p.xbins, p.zbins, p.xlim, p.zlim
@@ -66,6 +70,8 @@ class DF_tracker:
         self.device = torch.device(device)
         self.keep_fp32 = keep_fp32
         self.keep_fp64 = keep_fp64
+        self.recent_planes = int(recent_planes) if keep_fp64 else 0
+        self.recent = None    # float64 (K, Xi, Zi, 6) scratch of FP64 wakes
         if not (keep_fp32 or keep_fp64):
             raise DfcsrError("DF_tracker needs the float64 or the float32 history rings")
         self.ws = _lib.Workspace(self.device)
@@ -137,8 +143,11 @@ class DF_tracker:
 
     def _alloc_hist(self, Xi, Zi):
         if self.Xi != Xi or self.Zi != Zi or (self.hist is None and self.hist32 is None):
-            self.hist = self.hist32 = self.hist32t = None   # free before allocating
+            self.hist = self.hist32 = self.hist32t = self.recent = None   # free first
             self.hist, self.hist32, self.hist32t = self._rings(Xi, Zi, self.cap)
+            if self.recent_planes > 0:
+                self.recent = torch.empty((self.recent_planes, Xi, Zi, _lib.RECENT_CELL),
+                                          dtype=torch.float64, device=self.device)
         self.Xi, self.Zi = Xi, Zi
 
     def raw_fields(self, i=None):
@@ -253,10 +262,11 @@ class DF_tracker:
         """ctypes history descriptor for the kernels (after build_interpolant)."""
         def p(t):
             return t.data_ptr() if t is not None else None
-        return _lib.History(p(self.hist), p(self.hist32), p(self.hist32t), self.cap, self.head,
-                            self.nt, self.Xi, self.Zi, float(self.min_x), float(self.min_y),
-                            float(self.min_z), float(self.delta_x), float(self.delta_y),
-                            float(self.delta_z))
+        return _lib.History(p(self.hist), p(self.hist32), p(self.hist32t), p(self.recent),
+                            self.cap, self.head, self.nt, self.Xi, self.Zi,
+                            self.recent_planes if self.recent is not None else 0,
+                            float(self.min_x), float(self.min_y), float(self.min_z),
+                            float(self.delta_x), float(self.delta_y), float(self.delta_z))
 
     def history_fields(self):
         """Stacked (nt, Xi, Zi) copies of the five fields in logical order
@@ -273,4 +283,6 @@ class DF_tracker:
         for r in (self.hist32, self.hist32t):
             if r is not None:
                 b += r.numel() * 4
+        if self.recent is not None:
+            b += self.recent.numel() * 8
         return b

@@ -56,7 +56,8 @@ class DeviceHistory:
     rings: which of the float64, float32 chunked and float32 time-innermost
     rings to build."""
 
-    def __init__(self, tracker, device="cuda", wrap=0, rings=("fp64", "fp32", "fp32t")):
+    def __init__(self, tracker, device="cuda", wrap=0, rings=("fp64", "fp32", "fp32t"),
+                 recent_planes=0):
         data = np.stack([tracker.data[f] for f in FIELDS], axis=-1)   # (nt, Xi, Zi, 5)
         nt, Xi, Zi, _ = data.shape
         cap = nt + (wrap + 1 if wrap else 0)
@@ -70,13 +71,20 @@ class DeviceHistory:
                                                     4)).to(device)
         if "fp32t" in rings:
             self.h32t = torch.from_numpy(ring64(packed.astype(np.float32), cap, head)).to(device)
+        # the FP64 wake's recent-plane scratch, filled with NaN: the wake must
+        # write every cell it reads
+        self.recent = None
+        if recent_planes:
+            self.recent = torch.full((recent_planes, Xi, Zi, _lib.RECENT_CELL), float("nan"),
+                                     dtype=torch.float64, device=device)
         a = tracker.axes()
 
         def p(t):
             return t.data_ptr() if t is not None else None
-        self.desc = _lib.History(p(self.h64), p(self.h32), p(self.h32t), cap, head, nt, Xi, Zi,
-                                 float(a["min_t"]), float(a["min_x"]), float(a["min_z"]),
-                                 float(a["dt"]), float(a["dx"]), float(a["dz"]))
+        self.desc = _lib.History(p(self.h64), p(self.h32), p(self.h32t), p(self.recent), cap, head,
+                                 nt, Xi, Zi, recent_planes, float(a["min_t"]), float(a["min_x"]),
+                                 float(a["min_z"]), float(a["dt"]), float(a["dx"]),
+                                 float(a["dz"]))
 
 
 class DeviceLattice:

@@ -153,7 +153,8 @@ def test_interpolate_golden_bitexact(K, golden_dir):
     stack = np.zeros((nt, nx, nz, 5))
     stack[..., 0] = data
     h = _t(ring64(corner_records(stack), nt, 0))
-    H = _lib.History(h.data_ptr(), None, None, nt, 0, nt, nx, nz, *[float(v) for v in d["mins"]],
+    H = _lib.History(h.data_ptr(), None, None, None, nt, 0, nt, nx, nz, 0,
+                     *[float(v) for v in d["mins"]],
                      *[float(v) for v in d["dels"]])
     r = K.interpolate3D(H, 0, _t(d["q3"]), DEV).cpu().numpy()
     assert np.array_equal(r, d["r3"])
@@ -248,6 +249,15 @@ def test_wake_golden(K, ws, golden_dir, golden_history):
     _, _, DHw, _, _ = _golden_wake_inputs(golden_dir, golden_history, wrap=3)
     out3 = K.wake_mesh(DHw.desc, DL.desc, P, 0, DEV, ws)
     assert torch.equal(out["dE_dct"], out3["dE_dct"]) and torch.equal(out["x_kick"], out3["x_kick"])
+    # with the recent-plane copy (NaN-filled scratch: every cell read must be
+    # written) of 1, 2, all and more than all planes, wrapped or not: identical
+    _, T = golden_history
+    for wrap in (0, 3):
+        for kp in (1, 2, T.data["density"].shape[0], 40):
+            DHr = DeviceHistory(T, DEV, wrap=wrap, recent_planes=kp)
+            o = K.wake_mesh(DHr.desc, DL.desc, P, 0, DEV, ws)
+            assert torch.equal(out["dE_dct"], o["dE_dct"]), (wrap, kp)
+            assert torch.equal(out["x_kick"], o["x_kick"]), (wrap, kp)
 
 
 @pytest.mark.parametrize("slope0,key", [(2.5, "chirp_pos"), (-2.5, "chirp_neg")])

@@ -178,13 +178,13 @@ typedef struct {
   const double *data;    /* [Xi][Zi][cap][DFCSR_HIST_REC64] or NULL */
   const float *data32;   /* [Xi][cap][DFCSR_HIST_CHUNKS32][Zi][4] or NULL */
   const float *data32t;  /* [Xi][Zi][cap][DFCSR_HIST_REC32] or NULL */
-  /* Optional scratch for FP64 wakes, [recent_planes][Xi][Zi][DFCSR_RECENT_CELL]
-   * doubles (or NULL): each FP64 wake first copies the latest
-   * K = min(recent_planes, nt) planes of the FP64 ring into it, cell-major
-   * (the five fields of cell (x, z), then a pad), and its near-region
-   * lookups that fall on those planes read the copy -- a third of the
-   * bytes of the corner-packed records, neighbouring z in neighbouring
-   * cells.  Results are identical with or without it. */
+  /* Optional wake scratch, [recent_planes][Xi][Zi][DFCSR_RECENT_CELL]
+   * doubles for FP64 wakes, floats for mixed ones (or NULL): each wake first
+   * copies the latest K = min(recent_planes, nt) planes of the ring it reads
+   * into it, cell-major (the five fields of cell (x, z), then a pad), and its
+   * near-region lookups that fall on those planes read the copy -- less
+   * than a third of the bytes of the corner-packed records, neighbouring z
+   * in neighbouring cells.  Results are identical with or without it. */
   double *recent;
   int32_t cap, head, nt, Xi, Zi;
   int32_t recent_planes;

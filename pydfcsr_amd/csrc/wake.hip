@@ -274,11 +274,13 @@ struct Neumaier {
 };
 
 // One source point in mixed precision; false when outside the history.
-// kTI: read the time-innermost FP32 ring, else the chunked one.
-template <bool kTI>
+// kTI: read the time-innermost FP32 ring, else the chunked one.  kRecent:
+// lookups on the latest planes read the float32 cell-major copy in H.recent
+// (same values, same arithmetic).
+template <bool kTI, bool kRecent = false>
 __device__ inline bool csr_point_mixed(const DfcsrHistory &H, const HistAxes &A, double t,
-                                       const ColumnMixed &c, double xp, int xi0, float xd,
-                                       float &Iz, float &Ix) {
+                                       const ColumnMixed &c, double xp, int xi0, int xi1,
+                                       float xd, float &Iz, float &Ix) {
   const double rx = c.ax - xp * c.npx;
   const double ry = c.ay - xp * c.npy;
   const double R2 = rx * rx + ry * ry;
@@ -295,21 +297,57 @@ __device__ inline bool csr_point_mixed(const DfcsrHistory &H, const HistAxes &A,
   const TruncIdx T = trunc_index(ut, H.nt), Z = trunc_index(uz, H.Zi);
   if (!(T.ok && Z.ok)) return false;
   const float td = (float)T.frac, zd = (float)Z.frac;
-  // one corner-packed record per time plane, one 16-byte chunk per field
-  const float4 *q0 = kTI ? hist_rec32t(H, T.i0, xi0, Z.i0) : hist_rec32(H, T.i0, xi0, Z.i0);
-  const float4 *q1 = kTI ? hist_rec32t(H, T.i1, xi0, Z.i0) : hist_rec32(H, T.i1, xi0, Z.i0);
-  const int64_t cs = kTI ? 1 : H.Zi;
   float f[5];
+  const int cr = H.nt - min(H.recent_planes, H.nt);
+  if (kRecent && T.i0 >= cr) {
+    // float32 cell (x, z) of copied plane k at ((k Xi + x) Zi + z) * 6: three 8-byte
+    // loads per corner (x0,z0), (x0,z1), (x1,z0), (x1,z1) of each plane
+    const float *R32 = reinterpret_cast<const float *>(H.recent);
+    const int64_t Zi = H.Zi;
+    const float2 *b0 = reinterpret_cast<const float2 *>(
+        R32 + (((int64_t)(T.i0 - cr) * H.Xi + xi0) * Zi + Z.i0) * DFCSR_RECENT_CELL);
+    const float2 *b1 = reinterpret_cast<const float2 *>(
+        R32 + (((int64_t)(T.i1 - cr) * H.Xi + xi0) * Zi + Z.i0) * DFCSR_RECENT_CELL);
+    const int64_t dz = (int64_t)(Z.i1 - Z.i0) * 3, dx = (int64_t)(xi1 - xi0) * Zi * 3;
+    const int64_t off[4] = {0, dz, dx, dx + dz};
+    float v0[4][6], v1[4][6];
 #pragma unroll
-  for (int k = 0; k < 5; ++k) {
-    const float4 a = q0[k * cs], b = q1[k * cs];   // (x0,z0), (x0,z1), (x1,z0), (x1,z1)
-    const float c00 = fmaf(td, b.x - a.x, a.x);
-    const float c01 = fmaf(td, b.y - a.y, a.y);
-    const float c10 = fmaf(td, b.z - a.z, a.z);
-    const float c11 = fmaf(td, b.w - a.w, a.w);
-    const float c0 = fmaf(xd, c10 - c00, c00);
-    const float c1 = fmaf(xd, c11 - c01, c01);
-    f[k] = fmaf(zd, c1 - c0, c0);
+    for (int q = 0; q < 4; ++q) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const float2 a = b0[off[q] + k], b = b1[off[q] + k];
+        v0[q][2 * k] = a.x;
+        v0[q][2 * k + 1] = a.y;
+        v1[q][2 * k] = b.x;
+        v1[q][2 * k + 1] = b.y;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      const float c00 = fmaf(td, v1[0][k] - v0[0][k], v0[0][k]);
+      const float c01 = fmaf(td, v1[1][k] - v0[1][k], v0[1][k]);
+      const float c10 = fmaf(td, v1[2][k] - v0[2][k], v0[2][k]);
+      const float c11 = fmaf(td, v1[3][k] - v0[3][k], v0[3][k]);
+      const float c0 = fmaf(xd, c10 - c00, c00);
+      const float c1 = fmaf(xd, c11 - c01, c01);
+      f[k] = fmaf(zd, c1 - c0, c0);
+    }
+  } else {
+    // one corner-packed record per time plane, one 16-byte chunk per field
+    const float4 *q0 = kTI ? hist_rec32t(H, T.i0, xi0, Z.i0) : hist_rec32(H, T.i0, xi0, Z.i0);
+    const float4 *q1 = kTI ? hist_rec32t(H, T.i1, xi0, Z.i0) : hist_rec32(H, T.i1, xi0, Z.i0);
+    const int64_t cs = kTI ? 1 : H.Zi;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      const float4 a = q0[k * cs], b = q1[k * cs];   // (x0,z0), (x0,z1), (x1,z0), (x1,z1)
+      const float c00 = fmaf(td, b.x - a.x, a.x);
+      const float c01 = fmaf(td, b.y - a.y, a.y);
+      const float c10 = fmaf(td, b.z - a.z, a.z);
+      const float c11 = fmaf(td, b.w - a.w, a.w);
+      const float c0 = fmaf(xd, c10 - c00, c00);
+      const float c1 = fmaf(xd, c11 - c01, c01);
+      f[k] = fmaf(zd, c1 - c0, c0);
+    }
   }
   const float dens = f[DFCSR_F_DENSITY], dx = f[DFCSR_F_DENSITY_X], dz = f[DFCSR_F_DENSITY_Z];
   const float vr = f[DFCSR_F_VX], dv = f[DFCSR_F_VX_X];
@@ -445,8 +483,9 @@ __device__ inline void observer_point(const WakeTask &T, int i, double *__restri
 
 // Integrates phase ph of the mesh point whose observer data is od; every
 // thread returns the block sums of wz_k wx_j (I_z, I_x) over the phase.
-// kTI (mixed only): read the time-innermost FP32 ring.  kRecent (FP64
-// only): lookups on the latest planes read the cell-major copy H.recent.
+// kTI (mixed only): read the time-innermost FP32 ring.  kRecent: lookups
+// on the latest planes read the cell-major copy H.recent (float64 cells for
+// FP64, float32 cells for mixed).
 template <bool kMixed, bool kTI, bool kRecent = false>
 __device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int ph, double *red,
                            double &out_z, double &out_x) {
@@ -522,7 +561,8 @@ __device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int
         const TruncIdx X = trunc_index((xp - A.min_x) * A.ix, H.Xi);
         if (!X.ok) continue;
         float Iz, Ix;
-        if (!csr_point_mixed<kTI>(H, A, t, col, xp, X.i0, (float)X.frac, Iz, Ix)) continue;
+        if (!csr_point_mixed<kTI, kRecent>(H, A, t, col, xp, X.i0, X.i1, (float)X.frac, Iz, Ix))
+          continue;
         const float wx = (float)xa.w(j);
         cz.add(wx * Iz);
         cx.add(wx * Ix);
@@ -552,8 +592,8 @@ __device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int
 
 // Workgroup l of a task's far (kNear = false) or near launch -> (mesh
 // point, phase); writes the phase's partial sums.  kTI (mixed only): the
-// phases read the time-innermost FP32 ring, else the chunked one; kRecent
-// (FP64 only): the latest planes come from the cell-major copy.
+// phases read the time-innermost FP32 ring, else the chunked one; kRecent:
+// the latest planes come from the cell-major copy.
 template <bool kMixed, bool kNear, bool kTI, bool kRecent = false>
 __device__ inline void run_phase_block(const WakeTask &T, int l, double *__restrict__ partials,
                                        double *red) {
@@ -584,8 +624,8 @@ __host__ __device__ inline bool mixed_reads_ti(const DfcsrHistory &H, bool near)
   return near ? (H.data32 == nullptr) : (H.data32t != nullptr);
 }
 
-// The FP64 near phases with the recent-plane copy hold both lookups' loads:
-// capped at 3 workgroups per CU (166 registers), the others at 2.
+// The near phases with the recent-plane copy hold the copy's loads as well
+// as the ring's: capped at 3 workgroups per CU, the others at 2.
 template <bool kMixed, bool kNear, bool kTI, bool kRecent = false>
 __global__ __launch_bounds__(kThreads, kRecent ? 3 : DFCSR_WAKE_MINBLOCKS) void wake_phase_kernel(
     WakeTask T, double *__restrict__ partials) {
@@ -613,16 +653,53 @@ __device__ inline void recent_copy(const DfcsrHistory &H, int64_t c, int64_t str
   }
 }
 
+// The same for mixed-precision wakes: float32 cells (five fields, a pad)
+// from the time-innermost float32 ring, else from the chunked one.
+__device__ inline void recent_copy32(const DfcsrHistory &H, int64_t c, int64_t stride) {
+  const int K = min(H.recent_planes, H.nt), c0 = H.nt - K;
+  const int64_t cells = (int64_t)H.Xi * H.Zi;
+  float *R32 = reinterpret_cast<float *>(H.recent);
+  for (; c < cells; c += stride) {
+    const int xi = (int)(c / H.Zi), zi = (int)(c - (int64_t)xi * H.Zi);
+    for (int k = 0; k < K; ++k) {
+      float v[5];
+      if (H.data32t) {
+        const float4 *r = hist_rec32t(H, c0 + k, xi, zi);
+#pragma unroll
+        for (int f = 0; f < 5; ++f) v[f] = r[f].x;
+      } else {
+        const float4 *r = hist_rec32(H, c0 + k, xi, zi);
+#pragma unroll
+        for (int f = 0; f < 5; ++f) v[f] = r[(int64_t)f * H.Zi].x;
+      }
+      float2 *o = reinterpret_cast<float2 *>(R32 + ((int64_t)k * cells + c) * DFCSR_RECENT_CELL);
+      o[0] = make_float2(v[0], v[1]);
+      o[1] = make_float2(v[2], v[3]);
+      o[2] = make_float2(v[4], 0.0f);
+    }
+  }
+}
+
+template <bool kMixed>
 __global__ __launch_bounds__(kThreads) void recent_copy_kernel(DfcsrHistory H) {
-  recent_copy(H, (int64_t)blockIdx.x * kThreads + threadIdx.x, (int64_t)gridDim.x * kThreads);
+  const int64_t c = (int64_t)blockIdx.x * kThreads + threadIdx.x, st = (int64_t)gridDim.x * kThreads;
+  if (kMixed)
+    recent_copy32(H, c, st);
+  else
+    recent_copy(H, c, st);
 }
 
 // blockIdx.y = task; tasks without a copy buffer return at once.
+template <bool kMixed>
 __global__ __launch_bounds__(kThreads) void recent_copy_batched_kernel(
     const WakeTask *__restrict__ tasks) {
   const DfcsrHistory &H = tasks[blockIdx.y].H;
   if (!H.recent) return;
-  recent_copy(H, (int64_t)blockIdx.x * kThreads + threadIdx.x, (int64_t)gridDim.x * kThreads);
+  const int64_t c = (int64_t)blockIdx.x * kThreads + threadIdx.x, st = (int64_t)gridDim.x * kThreads;
+  if (kMixed)
+    recent_copy32(H, c, st);
+  else
+    recent_copy(H, c, st);
 }
 
 // Last task whose first index (workgroup or observer) is <= b.
@@ -647,11 +724,18 @@ __global__ __launch_bounds__(kThreads, DFCSR_WAKE_MINBLOCKS) void wake_phase_bat
                           [](const WakeTask &T) { return kNear ? T.blockB0 : T.blockA0; });
   const WakeTask &T = tasks[j];
   const int l = b - (kNear ? T.blockB0 : T.blockA0);
-  if constexpr (kMixed) {   // per task: tasks may hold different FP32 rings
-    if (mixed_reads_ti(T.H, kNear))
+  if constexpr (kMixed) {   // per task: its FP32 rings, a recent-plane copy or not
+    const bool ti = mixed_reads_ti(T.H, kNear);
+    if (kNear && T.H.recent) {
+      if (ti)
+        run_phase_block<true, kNear, true, true>(T, l, partials, red);
+      else
+        run_phase_block<true, kNear, false, true>(T, l, partials, red);
+    } else if (ti) {
       run_phase_block<true, kNear, true>(T, l, partials, red);
-    else
+    } else {
       run_phase_block<true, kNear, false>(T, l, partials, red);
+    }
   } else if (kNear && T.H.recent) {   // per task: a recent-plane copy or not
     run_phase_block<false, kNear, false, true>(T, l, partials, red);
   } else {
@@ -812,16 +896,28 @@ extern "C" int dfcsr_wake_mesh(const DfcsrHistory *hist, const DfcsrLattice *lat
       hipLaunchKernelGGL((wake_phase_kernel<true, false, true>), ga, dim3(kThreads), 0, s, T, part);
     else
       hipLaunchKernelGGL((wake_phase_kernel<true, false, false>), ga, dim3(kThreads), 0, s, T, part);
-    if (mixed_reads_ti(*hist, true))
+    const int64_t cells = (int64_t)hist->Xi * hist->Zi;
+    const int gc = (int)std::min<int64_t>((cells + kThreads - 1) / kThreads, 4096);
+    const bool ti = mixed_reads_ti(*hist, true);
+    if (hist->recent) {
+      hipLaunchKernelGGL(recent_copy_kernel<true>, dim3(gc), dim3(kThreads), 0, s, *hist);
+      if (ti)
+        hipLaunchKernelGGL((wake_phase_kernel<true, true, true, true>), gb, dim3(kThreads), 0, s,
+                           T, part);
+      else
+        hipLaunchKernelGGL((wake_phase_kernel<true, true, false, true>), gb, dim3(kThreads), 0, s,
+                           T, part);
+    } else if (ti) {
       hipLaunchKernelGGL((wake_phase_kernel<true, true, true>), gb, dim3(kThreads), 0, s, T, part);
-    else
+    } else {
       hipLaunchKernelGGL((wake_phase_kernel<true, true, false>), gb, dim3(kThreads), 0, s, T, part);
+    }
   } else {
     hipLaunchKernelGGL((wake_phase_kernel<false, false, false>), ga, dim3(kThreads), 0, s, T, part);
     if (hist->recent) {
       const int64_t cells = (int64_t)hist->Xi * hist->Zi;
       const int gc = (int)std::min<int64_t>((cells + kThreads - 1) / kThreads, 4096);
-      hipLaunchKernelGGL(recent_copy_kernel, dim3(gc), dim3(kThreads), 0, s, *hist);
+      hipLaunchKernelGGL(recent_copy_kernel<false>, dim3(gc), dim3(kThreads), 0, s, *hist);
       hipLaunchKernelGGL((wake_phase_kernel<false, true, false, true>), gb, dim3(kThreads), 0, s, T,
                          part);
     } else {
@@ -909,10 +1005,11 @@ extern "C" int dfcsr_wake_batched_launch(const void *dev_tasks, const DfcsrWakeB
                      batch->observers, part);
   if (precision == DFCSR_PREC_MIXED) {
     hipLaunchKernelGGL((wake_phase_batched_kernel<true, false>), ga, dim3(kThreads), 0, s, dt, k, part);
+    hipLaunchKernelGGL(recent_copy_batched_kernel<true>, dim3(512, k), dim3(kThreads), 0, s, dt);
     hipLaunchKernelGGL((wake_phase_batched_kernel<true, true>), gb, dim3(kThreads), 0, s, dt, k, part);
   } else if (precision == DFCSR_PREC_FP64) {
     hipLaunchKernelGGL((wake_phase_batched_kernel<false, false>), ga, dim3(kThreads), 0, s, dt, k, part);
-    hipLaunchKernelGGL(recent_copy_batched_kernel, dim3(512, k), dim3(kThreads), 0, s, dt);
+    hipLaunchKernelGGL(recent_copy_batched_kernel<false>, dim3(512, k), dim3(kThreads), 0, s, dt);
     hipLaunchKernelGGL((wake_phase_batched_kernel<false, true>), gb, dim3(kThreads), 0, s, dt, k, part);
   } else {
     return set_error(DFCSR_EINVAL, "wake_batched_launch: unknown precision %d", precision);

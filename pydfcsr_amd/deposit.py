@@ -18,9 +18,10 @@ hist32t keep the slot innermost (the two planes of a lookup are neighbours:
 what the far region's lookups want); hist32 stores each field's four corners
 as one 16-byte chunk, z fastest, so neighbouring z of one plane are
 neighbours: what the near regions' lookups want.  keep_fp64 / keep_fp32
-choose the float64 ring and the two float32 rings.)  FP64 wakes also get a
-scratch buffer, recent[K][Xi][Zi][6], that each wake fills with a cell-major
-copy of the latest K planes for its near-region lookups.
+choose the float64 ring and the two float32 rings.)  The wakes also get a
+scratch buffer, recent[K][Xi][Zi][6] (float64, or float32 without the float64
+ring), that each wake fills with a cell-major copy of the latest K planes for
+its near-region lookups.
 Appending writes slot (head + nt) % cap; popping advances head.  Nothing is
 copied per step: the reference's np.array(deque) stacking of the whole
 history (deposit.py:422-426) becomes a descriptor update.
@@ -60,7 +61,7 @@ class DF_tracker:
                  keep_fp64=True, recent_planes=4):
         """keep_fp64: hold the float64 history ring (FP64 wakes, lookups);
         keep_fp32: hold the two float32 rings (mixed-precision wakes);
-        recent_planes: planes of the FP64 wake's cell-major copy of the latest
+        recent_planes: planes of the wakes' cell-major copy of the latest
         slices (0: none)."""
         self.params = Deposition_params(input_dic)
         p = self.params
@@ -70,8 +71,8 @@ class DF_tracker:
         self.device = torch.device(device)
         self.keep_fp32 = keep_fp32
         self.keep_fp64 = keep_fp64
-        self.recent_planes = int(recent_planes) if keep_fp64 else 0
-        self.recent = None    # float64 (K, Xi, Zi, 6) scratch of FP64 wakes
+        self.recent_planes = int(recent_planes)
+        self.recent = None    # (K, Xi, Zi, 6) scratch of the wakes
         if not (keep_fp32 or keep_fp64):
             raise DfcsrError("DF_tracker needs the float64 or the float32 history rings")
         self.ws = _lib.Workspace(self.device)
@@ -146,8 +147,10 @@ class DF_tracker:
             self.hist = self.hist32 = self.hist32t = self.recent = None   # free first
             self.hist, self.hist32, self.hist32t = self._rings(Xi, Zi, self.cap)
             if self.recent_planes > 0:
+                # float64 cells when FP64 wakes may run, else float32 cells
+                dt = torch.float64 if self.keep_fp64 else torch.float32
                 self.recent = torch.empty((self.recent_planes, Xi, Zi, _lib.RECENT_CELL),
-                                          dtype=torch.float64, device=self.device)
+                                          dtype=dt, device=self.device)
         self.Xi, self.Zi = Xi, Zi
 
     def raw_fields(self, i=None):
@@ -284,5 +287,5 @@ class DF_tracker:
             if r is not None:
                 b += r.numel() * 4
         if self.recent is not None:
-            b += self.recent.numel() * 8
+            b += self.recent.numel() * self.recent.element_size()
         return b

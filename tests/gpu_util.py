@@ -71,12 +71,14 @@ class DeviceHistory:
                                                     4)).to(device)
         if "fp32t" in rings:
             self.h32t = torch.from_numpy(ring64(packed.astype(np.float32), cap, head)).to(device)
-        # the FP64 wake's recent-plane scratch, filled with NaN: the wake must
-        # write every cell it reads
+        # the wakes' recent-plane scratch (float64-sized), every byte 0xff: NaN
+        # read as float64 and as float32, so a cell the wake reads without
+        # writing it first shows up in the result
         self.recent = None
         if recent_planes:
-            self.recent = torch.full((recent_planes, Xi, Zi, _lib.RECENT_CELL), float("nan"),
-                                     dtype=torch.float64, device=device)
+            n = recent_planes * Xi * Zi * _lib.RECENT_CELL
+            self.recent = torch.full((n * 8,), 255, dtype=torch.uint8, device=device) \
+                .view(torch.float64).view(recent_planes, Xi, Zi, _lib.RECENT_CELL)
         a = tracker.axes()
 
         def p(t):

@@ -314,20 +314,24 @@ def test_wake_mixed_precision_golden(K, ws, golden_dir, golden_history):
 def test_wake_mixed_precision_ring_layouts(K, ws, golden_dir, golden_history):
     """The mixed form reads either float32 ring layout (chunked, time-innermost,
     or both: region 0 the time-innermost one, the later regions the chunked
-    one) with the same arithmetic, so the wakes are bit-identical across
-    layouts; without the float64 ring, vx at the observer comes from a float32
-    ring, within 1e-4 of max|W| of the reference's float64 wakes."""
+    one), with or without the cell-major copy of the latest planes (NaN-filled
+    scratch: every cell read must be written), with the same arithmetic, so the
+    wakes are bit-identical across layouts; without the float64 ring, vx at the
+    observer comes from a float32 ring, within 1e-4 of max|W| of the
+    reference's float64 wakes."""
     w, beam, _, DL, P = _golden_wake_inputs(golden_dir, golden_history)
     _, T = golden_history
     res = {}
     for rings in (("fp64", "fp32", "fp32t"), ("fp64", "fp32"), ("fp64", "fp32t"),
                   ("fp32", "fp32t"), ("fp32",), ("fp32t",)):
-        DH = DeviceHistory(T, DEV, wrap=2, rings=rings)
-        out = K.wake_mesh(DH.desc, DL.desc, P, 1, DEV, ws)
-        res[rings] = np.stack([out["dE_dct"].cpu().numpy(), out["x_kick"].cpu().numpy()])
-        e_z = rel_max(res[rings][0].reshape(3, 4), w["dE_dct"])
-        e_x = rel_max(res[rings][1].reshape(3, 4), w["x_kick"])
-        assert e_z <= 1e-4 and e_x <= 1e-4, (rings, e_z, e_x)
+        for kp in (0, 1, 3, 40):   # with the float32 recent-plane copy of kp planes
+            DH = DeviceHistory(T, DEV, wrap=2, rings=rings, recent_planes=kp)
+            out = K.wake_mesh(DH.desc, DL.desc, P, 1, DEV, ws)
+            res[rings + (kp,)] = np.stack([out["dE_dct"].cpu().numpy(),
+                                           out["x_kick"].cpu().numpy()])
+            e_z = rel_max(res[rings + (kp,)][0].reshape(3, 4), w["dE_dct"])
+            e_x = rel_max(res[rings + (kp,)][1].reshape(3, 4), w["x_kick"])
+            assert e_z <= 1e-4 and e_x <= 1e-4, (rings, kp, e_z, e_x)
     with64 = [r for r in res if "fp64" in r]
     without = [r for r in res if "fp64" not in r]
     for group in (with64, without):

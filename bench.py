@@ -362,7 +362,8 @@ def main():
         "csr_step_ms": round(float(np.mean(csr_ms)), 3),
         "wake_ms": round(wake_avg_ms, 3),
         "history": {"slices": tr.nt, "Xi": tr.Xi, "Zi": tr.Zi,
-                    "bytes": int(tr.footprint_bytes()), "regrids": tr.n_regrid},
+                    "bytes": int(tr.footprint_bytes()), "regrids": tr.n_regrid,
+                            "recent_planes": int(tr.recent_use)},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
                      # bytes per wake from the committed counter runs of this workload

@@ -58,11 +58,12 @@ class SliceMeta:
 
 class DF_tracker:
     def __init__(self, input_dic=None, device="cuda", keep_fp32=True, capacity=16,
-                 keep_fp64=True, recent_planes=4):
+                 keep_fp64=True, recent_planes=16):
         """keep_fp64: hold the float64 history ring (FP64 wakes, lookups);
         keep_fp32: hold the two float32 rings (mixed-precision wakes);
-        recent_planes: planes of the wakes' cell-major copy of the latest
-        slices (0: none)."""
+        recent_planes: planes allocated for the wakes' cell-major copy of the
+        latest slices (0: none); ``recent_use`` sets how many the next wake
+        copies (default: all)."""
         self.params = Deposition_params(input_dic)
         p = self.params
         self.xbins, self.zbins, self.xlim, self.zlim = p.xbins, p.zbins, p.xlim, p.zlim
@@ -72,6 +73,7 @@ class DF_tracker:
         self.keep_fp32 = keep_fp32
         self.keep_fp64 = keep_fp64
         self.recent_planes = int(recent_planes)
+        self.recent_use = self.recent_planes
         self.recent = None    # (K, Xi, Zi, 6) scratch of the wakes
         if not (keep_fp32 or keep_fp64):
             raise DfcsrError("DF_tracker needs the float64 or the float32 history rings")
@@ -267,7 +269,8 @@ class DF_tracker:
             return t.data_ptr() if t is not None else None
         return _lib.History(p(self.hist), p(self.hist32), p(self.hist32t), p(self.recent),
                             self.cap, self.head, self.nt, self.Xi, self.Zi,
-                            self.recent_planes if self.recent is not None else 0,
+                            max(1, min(int(self.recent_use), self.recent_planes))
+                            if self.recent is not None else 0,
                             float(self.min_x), float(self.min_y), float(self.min_z),
                             float(self.delta_x), float(self.delta_y), float(self.delta_z))
 

@@ -69,8 +69,8 @@ def request_footprint(config: dict, steps: int, precision: str = "mixed") -> int
     # two float32 rings a mixed-precision simulation keeps instead
     rec = 160
     hist = slots * grid * grid * rec
-    # the wake's cell-major copy of the latest 4 planes (float64 or float32 cells)
-    hist += 4 * grid * grid * (4 if precision == "mixed" else 8) * _lib.RECENT_CELL
+    # the wake's cell-major copy of the latest 16 planes (float64 or float32 cells)
+    hist += 16 * grid * grid * (4 if precision == "mixed" else 8) * _lib.RECENT_CELL
     raw = slots * 5 * nb * nb * 8
     particles = n * 8 * (6 * 2 + 4)           # coordinates, tracking temporary, scratch
     deposit = 4 * nb * nb * 8 * 2 + 3 * nb * nb * 8

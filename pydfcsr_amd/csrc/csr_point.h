@@ -84,13 +84,21 @@ __device__ inline const double *hist_rec(const DfcsrHistory &H, int ti, int xi, 
 // truncation and validity rules.  Returns false (and zeros) when invalid.
 // kRecent: lookups whose planes lie in the cell-major copy of the latest
 // planes (H.recent) read it; the values, and so the result, are the same.
+// Xpre: the x index of xq when the caller has it already (the same value).
 template <bool kRecip, bool kRecent = false>
 __device__ inline bool hist_lookup5(const DfcsrHistory &H, const HistAxes &A, double tq,
-                                    double xq, double zq, double out[5]) {
+                                    double xq, double zq, double out[5],
+                                    const TruncIdx *Xpre = nullptr) {
   double ut = kRecip ? (tq - A.min_t) * A.it : (tq - A.min_t) / A.dt;
-  double ux = kRecip ? (xq - A.min_x) * A.ix : (xq - A.min_x) / A.dx;
   double uz = kRecip ? (zq - A.min_z) * A.iz : (zq - A.min_z) / A.dz;
-  TruncIdx T = trunc_index(ut, H.nt), X = trunc_index(ux, H.Xi), Z = trunc_index(uz, H.Zi);
+  TruncIdx X;
+  if (Xpre) {
+    X = *Xpre;
+  } else {
+    const double ux = kRecip ? (xq - A.min_x) * A.ix : (xq - A.min_x) / A.dx;
+    X = trunc_index(ux, H.Xi);
+  }
+  TruncIdx T = trunc_index(ut, H.nt), Z = trunc_index(uz, H.Zi);
   if (!(T.ok && X.ok && Z.ok)) {
     for (int f = 0; f < 5; ++f) out[f] = 0.0;
     return false;
@@ -200,13 +208,14 @@ __device__ inline Column make_column(const DfcsrLattice &L, const Observer &o, d
 // divisions give 0/0: that point is evaluated (with zero fields) as it is.
 template <bool kRecip, bool kRecent = false>
 __device__ inline bool csr_point(const DfcsrHistory &H, const HistAxes &A, const Observer &o,
-                                 const Column &c, double xp, double &Iz, double &Ix) {
+                                 const Column &c, double xp, double &Iz, double &Ix,
+                                 const TruncIdx *Xpre = nullptr) {
   const double rx = c.ax - xp * c.L.nx;
   const double ry = c.ay - xp * c.L.ny;
   const double R = sqrt(rx * rx + ry * ry);
   const double t_ret = o.t - R;
   double f[5];
-  if (!hist_lookup5<kRecip, kRecent>(H, A, t_ret, xp, c.sp - t_ret, f) && R != 0.0) {
+  if (!hist_lookup5<kRecip, kRecent>(H, A, t_ret, xp, c.sp - t_ret, f, Xpre) && R != 0.0) {
     Iz = 0.0;
     Ix = 0.0;
     return false;

@@ -47,6 +47,19 @@ constexpr int kThreads = 256;
 // at most this many x' rows per phase, in region 0 / in the later regions
 constexpr int kChunkFar = DFCSR_WAKE_CHUNK_FAR;
 constexpr int kChunkNear = DFCSR_WAKE_CHUNK_NEAR;
+constexpr int kMaxRows = 64;   // a phase's row table holds at most this many rows
+static_assert(kChunkFar <= kMaxRows && kChunkNear <= kMaxRows, "phase rows exceed the row table");
+
+// A phase's x' rows, computed once per workgroup: x', its trapezoid weight
+// and its history index (the same expressions the points would evaluate).
+struct RowTable {
+  double xp[kMaxRows];
+  double w[kMaxRows];
+  double frac[kMaxRows];
+  int i0[kMaxRows];
+  int i1[kMaxRows];
+  int ok[kMaxRows];
+};
 constexpr int kNumXcd = 8;
 constexpr int kNearGroup = 16;   // consecutive near-phase workgroups per XCD
 
@@ -488,7 +501,7 @@ __device__ inline void observer_point(const WakeTask &T, int i, double *__restri
 // FP64, float32 cells for mixed).
 template <bool kMixed, bool kTI, bool kRecent = false>
 __device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int ph, double *red,
-                           double &out_z, double &out_x) {
+                           RowTable &rt, double &out_z, double &out_x) {
   const DfcsrHistory &H = T.H;
   const DfcsrLattice &L = T.L;
   const double t = T.P.t;
@@ -540,6 +553,17 @@ __device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int
   o.L.tx = od[6];
   o.L.ty = od[7];
   o.vx = od[8];
+  for (int r = threadIdx.x; r < v1 - v0; r += blockDim.x) {
+    const double xp = xa.at(v0 + r);
+    const TruncIdx X = trunc_index((xp - A.min_x) * A.ix, H.Xi);
+    rt.xp[r] = xp;
+    rt.w[r] = xa.w(v0 + r);
+    rt.frac[r] = X.frac;
+    rt.i0[r] = X.i0;
+    rt.i1[r] = X.i1;
+    rt.ok[r] = X.ok;
+  }
+  __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
   const int ngroup = (sa.n + 63) >> 6;
   double acc_z = 0.0, acc_x = 0.0;
@@ -556,14 +580,13 @@ __device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int
     if (kMixed) {
       const ColumnMixed col = make_column_mixed(L, o, sp, inv_ds);
       Neumaier cz, cx;
-      for (int j = v0 + q; j < v1; j += nq) {
-        const double xp = xa.at(j);
-        const TruncIdx X = trunc_index((xp - A.min_x) * A.ix, H.Xi);
-        if (!X.ok) continue;
+      for (int r = q; r < v1 - v0; r += nq) {
+        if (!rt.ok[r]) continue;
         float Iz, Ix;
-        if (!csr_point_mixed<kTI, kRecent>(H, A, t, col, xp, X.i0, X.i1, (float)X.frac, Iz, Ix))
+        if (!csr_point_mixed<kTI, kRecent>(H, A, t, col, rt.xp[r], rt.i0[r], rt.i1[r],
+                                           (float)rt.frac[r], Iz, Ix))
           continue;
-        const float wx = (float)xa.w(j);
+        const float wx = (float)rt.w[r];
         cz.add(wx * Iz);
         cx.add(wx * Ix);
       }
@@ -572,13 +595,12 @@ __device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int
     } else {
       const Column col = make_column<true>(L, o, sp, inv_ds);
       double cz = 0.0, cx = 0.0;
-      for (int j = v0 + q; j < v1; j += nq) {
-        const double xp = xa.at(j);
-        const double ux = (xp - A.min_x) * A.ix;
-        if (!(ux > -1.0 && ux < (double)H.Xi)) continue;
+      for (int r = q; r < v1 - v0; r += nq) {
+        if (!rt.ok[r]) continue;
+        const TruncIdx X = {rt.i0[r], rt.i1[r], rt.frac[r], true};
         double Iz, Ix;
-        if (!csr_point<true, kRecent>(H, A, o, col, xp, Iz, Ix)) continue;
-        const double wx = xa.w(j);
+        if (!csr_point<true, kRecent>(H, A, o, col, rt.xp[r], Iz, Ix, &X)) continue;
+        const double wx = rt.w[r];
         cz += wx * Iz;
         cx += wx * Ix;
       }
@@ -607,8 +629,9 @@ __device__ inline void run_phase_block(const WakeTask &T, int l, double *__restr
   const int q = rem / xb;
   i = (rem - q * xb) * zb + zl;
   ph = kNear ? T.nA + q : q;
+  __shared__ RowTable rt;
   double sz, sx;
-  wake_phase<kMixed, kTI, kRecent>(T, partials + T.obs + (int64_t)i * kObsStride, ph, red, sz,
+  wake_phase<kMixed, kTI, kRecent>(T, partials + T.obs + (int64_t)i * kObsStride, ph, red, rt, sz,
                                    sx);
   if (threadIdx.x == 0) {
     double *p = partials + T.part + ((int64_t)i * (T.nA + T.nB) + ph) * 2;

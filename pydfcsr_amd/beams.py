@@ -25,7 +25,7 @@ import math
 import numpy as np
 import torch
 
-from . import _lib, kernels
+from . import _lib, h5io, kernels
 
 # electron rest energy [eV]: scipy.constants' 'electron mass energy equivalent
 # in MeV' (CODATA 2018) x 1e6, as physical_constants.py:11
@@ -253,6 +253,13 @@ class Beam:
         momentum reference p0c is init_energy (beams.py:33)."""
         self.coords = self.coords.contiguous()
         return kernels.beam_statistics(self.coords, self.init_energy, MC2, self.ws, out)
+
+    @property
+    def particle_group(self):   # beams.py:226-230
+        """The beam as openPMD particle data (the dict of
+        bmadx_particles_to_openpmd, interfaces.py:65-121), copied to the host."""
+        c = self.coords.cpu().numpy()
+        return h5io.openpmd_particle_data(*c, self.position, self.init_energy, MC2, self.charge)
 
     @property
     def twiss(self):   # beams.py:222-224 -> twiss.py:2-27

@@ -20,7 +20,7 @@ import time
 import numpy as np
 import torch
 
-from . import _lib, kernels
+from . import _lib, h5io, kernels
 from .beams import Beam, bmadx_element
 from .deposit import DF_tracker
 from .lattice import Lattice
@@ -167,6 +167,10 @@ class CSR2D:
                 on_step(self, step_count)
             if max_steps is not None and step_count >= max_steps:
                 return
+        if self._stopped:
+            return
+        self.dump_beam(label="end")
+        self.write_statistics()
 
     def iter_steps(self, stop_time=None, debug=False):
         """Generator form of run(): yields the step count after each step."""
@@ -180,6 +184,7 @@ class CSR2D:
         into self.wake_out (e.g. batched with other simulations) and resumes,
         and the step then applies the kicks."""
         step_count = 1
+        self._stopped = False
         DL = self.lattice.step_size
         ele_count = 0
         skip_ele = False
@@ -245,6 +250,11 @@ class CSR2D:
                     if self.timing:
                         self.timing_events.append((ev, ev_w0, ev_w1, self._event(), pts,
                                                    self._mesh_M()))
+                    wb = self.CSR_params.write_beam
+                    if wb == "all" or (isinstance(wb, list) and step_count in wb):
+                        self.dump_beam(label=step_count)
+                    if self.CSR_params.write_wakes:
+                        self.write_wakes()
                 self.update_statistics(step=step_count)
                 if self.verbose:
                     print(f"Finish step {step_count}, s = {self.beam.position},  in "
@@ -252,6 +262,7 @@ class CSR2D:
                 yield ("done", step_count)
                 step_count += 1
                 if stop_time and self.beam.position > stop_time:
+                    self._stopped = True
                     return
             ele_prev = ele
             ele_count += 1
@@ -362,6 +373,40 @@ class CSR2D:
         p = self.CSR_params
         self.dE_dct = self.wake_out["dE_dct"].view(p.xbins, p.zbins)
         self.x_kick = self.wake_out["x_kick"].view(p.xbins, p.zbins)
+        self.CSR_xmesh, self.CSR_zmesh = self.host_mesh()
+
+    def host_mesh(self):
+        """(CSR_xmesh, CSR_zmesh) of the current mesh computed on the host
+        with the reference's expressions (CSR.py:380-389)."""
+        xt, zm = np.meshgrid(self.CSR_xrange_transformed, self.CSR_zrange, indexing="ij")
+        zm = zm.flatten()
+        return xt.flatten() + np.polyval(self._mesh_slope, zm), zm
+
+    # -- output files (CSR.py:784-879) ------------------------------------------------
+    def _output_file(self, what):
+        return os.path.join(self.CSR_params.workdir, f"{self.prefix}-{what}.h5")
+
+    def dump_beam(self, label):   # CSR.py:784-797
+        filename = self._output_file(f"particles-{label}")
+        if self.verbose:
+            print("Beam at position {} is written to {}".format(self.beam.position, filename))
+        h5io.write_particles_file(filename, self.beam.particle_group, verbose=self.verbose)
+
+    def write_wakes(self):   # CSR.py:799-835
+        b = self.beam
+        xm, zm = self.CSR_xmesh, self.CSR_zmesh
+        xm = xm.cpu().numpy() if isinstance(xm, torch.Tensor) else np.asarray(xm)
+        zm = zm.cpu().numpy() if isinstance(zm, torch.Tensor) else np.asarray(zm)
+        h5io.write_wakes_file(self._output_file("wakes"), b.step, b.position, b.init_gamma,
+                              b.init_energy, self.lattice.current_element, b.charge, xm, zm,
+                              self.dE_dct.cpu().numpy(), self.x_kick.cpu().numpy(),
+                              verbose=self.verbose)
+
+    def write_statistics(self):   # CSR.py:860-879
+        lat = self.lattice
+        h5io.write_statistics_file(self._output_file("statistics"), lat.steps_record,
+                                   lat.coords, lat.n_vec, lat.tau_vec, self.statistics,
+                                   verbose=self.verbose)
 
     def _event(self):
         e = torch.cuda.Event(enable_timing=True)

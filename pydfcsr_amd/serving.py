@@ -193,8 +193,15 @@ class Server:
         self.active_samples = []
 
     def _make_sim(self, req: Request):
+        """A request's simulation.  Per-step wake files are written only when
+        its CSR_computation asks for them explicitly (the reference's default
+        is on; each write waits for the step's wake)."""
         from .csr import CSR2D
-        return CSR2D(config=req.config, device=self.device, precision=self.precision,
+        cfg = dict(req.config)
+        comp = dict(cfg.get("CSR_computation") or {})
+        comp.setdefault("write_wakes", False)
+        cfg["CSR_computation"] = comp
+        return CSR2D(config=cfg, device=self.device, precision=self.precision,
                      keep_fp32=(self.precision == "mixed"), history_capacity=req.steps + 2)
 
     def submit(self, req: Request):

@@ -152,3 +152,54 @@ def test_run_loop_matches_oracle_step_by_step():
     # positions 0, 0.005, ..., 0.08: 16 steps (total_steps counts position 0 too)
     assert steps == int(sum(sim.lattice.steps_per_element)) == sim.lattice.total_steps - 1 == 16
     print("worst relative differences over", steps, "steps:", worst)
+
+
+def test_run_writes_the_reference_output_files(tmp_path):
+    """run() with write_wakes and write_beam (CSR.py:335-339, 357-358): a
+    wakes group per CSR step holding that step's wakes, particle files at
+    the listed steps and at the end, and the statistics file; read back
+    with h5py."""
+    import torch
+    from pydfcsr_amd.csr import CSR2D
+    from tests.h5_readback import PY, dump
+    if PY is None:
+        pytest.skip("no interpreter with h5py to read files back")
+    cfg = tiny_config(n=5000)
+    cfg["CSR_computation"].update(write_wakes=True, write_beam=[2, 5], workdir=str(tmp_path),
+                                  write_name="tiny")
+    sim = CSR2D(config=cfg, device=DEV, precision="fp64")
+    rec = {}
+
+    def on_step(s, step):
+        rec[step] = (_np(s.dE_dct), _np(s.x_kick), s.beam.position)
+    sim.run(on_step=on_step)
+    pre = str(tmp_path / sim.prefix)
+    w = dump(pre + "-wakes.h5")
+    steps = sorted(int(k[5:]) for k in w if k.startswith("step_") and "/" not in k)
+    assert steps == sorted(rec) and len(steps) >= 10
+    for k in steps:
+        g = f"step_{k}"
+        assert w[g]["attrs"]["step"]["value"]["num"] == k
+        assert w[g]["attrs"]["position"]["value"]["num"] == rec[k][2]
+        dE = np.array(w[g + "/longitudinal/dE_dct"]["data"], dtype=np.float64)
+        xk = np.array(w[g + "/transverse/xkicks"]["data"], dtype=np.float64)
+        assert np.array_equal(dE, rec[k][0], equal_nan=True), k
+        assert np.array_equal(xk, rec[k][1], equal_nan=True), k
+        assert w[g + "/longitudinal"]["attrs"]["unit"]["value"] == {"str": "MeV/m"}
+    assert w[f"step_{steps[-1]}"]["attrs"]["element"]["value"] == {"str": "D1"}
+    assert w[f"step_{steps[0]}"]["attrs"]["element"]["value"] == {"str": "B1"}
+    # the mesh of the last step, as get_CSR_mesh lays it out (x outer)
+    xg = np.array(w[f"step_{steps[-1]}/longitudinal/x_grids"]["data"])
+    assert np.allclose(xg.ravel(), _np(sim.CSR_xmesh), rtol=0, atol=1e-15)
+    st = dump(pre + "-statistics.h5")
+    ref = sim.statistics
+    assert np.array_equal(np.array(st["sigma_x"]["data"]), ref["sigma_x"])
+    assert np.array_equal(np.array(st["twiss/emit_x"]["data"]), ref["twiss"]["emit_x"],
+                          equal_nan=True)
+    assert np.array_equal(np.array(st["step_positions"]["data"]), sim.lattice.steps_record)
+    for label in ("2", "5", "end"):
+        p = dump(f"{pre}-particles-{label}.h5")
+        assert p["/"]["attrs"]["numParticles"]["value"]["num"] == 5000
+    p = dump(pre + "-particles-end.h5")
+    assert np.array_equal(np.array(p["position/x"]["data"]), _np(sim.beam.x))
+    torch.cuda.synchronize()

@@ -80,15 +80,24 @@ __device__ inline const double *hist_rec(const DfcsrHistory &H, int ti, int xi, 
   return H.data + (((int64_t)xi * H.Zi + zi) * H.cap + hist_slot(H, ti)) * DFCSR_HIST_REC64;
 }
 
+// Byte offsets of an x row in the recent-plane copy: row i0 of plane 0, and
+// from row i0 to row i1.  The copy spans at most 4 GiB (checked on the
+// host), so lookups address it with 32-bit offsets from its base.
+struct RecentRow {
+  uint32_t rb, dxb;
+};
+
 // Trilinear lookup of all five fields at (t, x, z) with interpolate3D's
 // truncation and validity rules.  Returns false (and zeros) when invalid.
 // kRecent: lookups whose planes lie in the cell-major copy of the latest
 // planes (H.recent) read it; the values, and so the result, are the same.
-// Xpre: the x index of xq when the caller has it already (the same value).
+// Xpre: the x index of xq when the caller has it already (the same value);
+// rr: that row's offsets in the copy (computed here when absent).
 template <bool kRecip, bool kRecent = false>
 __device__ inline bool hist_lookup5(const DfcsrHistory &H, const HistAxes &A, double tq,
                                     double xq, double zq, double out[5],
-                                    const TruncIdx *Xpre = nullptr) {
+                                    const TruncIdx *Xpre = nullptr,
+                                    const RecentRow *rr = nullptr) {
   double ut = kRecip ? (tq - A.min_t) * A.it : (tq - A.min_t) / A.dt;
   double uz = kRecip ? (zq - A.min_z) * A.iz : (zq - A.min_z) / A.dz;
   TruncIdx X;
@@ -107,29 +116,34 @@ __device__ inline bool hist_lookup5(const DfcsrHistory &H, const HistAxes &A, do
   if (kRecent) {
     const int c0 = H.nt - min(H.recent_planes, H.nt);
     if (T.i0 >= c0) {
-      // cell (x, z) of copied plane k at ((k Xi + x) Zi + z) * 6: fields 0..3 as
-      // two 16-byte loads and field 4, per corner (x0,z0), (x0,z1), (x1,z0), (x1,z1)
-      const int64_t Zi = H.Zi;
-      const double2 *b0 = reinterpret_cast<const double2 *>(
-          H.recent + (((int64_t)(T.i0 - c0) * H.Xi + X.i0) * Zi + Z.i0) * DFCSR_RECENT_CELL);
-      const double2 *b1 = reinterpret_cast<const double2 *>(
-          H.recent + (((int64_t)(T.i1 - c0) * H.Xi + X.i0) * Zi + Z.i0) * DFCSR_RECENT_CELL);
-      const int64_t dz = (int64_t)(Z.i1 - Z.i0) * 3, dx = (int64_t)(X.i1 - X.i0) * Zi * 3;
-      const int64_t off[4] = {0, dz, dx, dx + dz};
+      // cell (x, z) of copied plane k at byte ((k Xi + x) Zi + z) * 48 of the copy:
+      // fields 0..3 as two 16-byte loads and field 4, per corner (x0,z0), (x0,z1),
+      // (x1,z0), (x1,z1)
+      constexpr uint32_t cb = DFCSR_RECENT_CELL * sizeof(double);
+      const uint32_t rowB = (uint32_t)H.Zi * cb;
+      const RecentRow r = rr ? *rr : RecentRow{(uint32_t)X.i0 * rowB,
+                                               (uint32_t)(X.i1 - X.i0) * rowB};
+      const char *base = reinterpret_cast<const char *>(H.recent);
+      const uint32_t planeB = (uint32_t)H.Xi * rowB;
+      const uint32_t o0 = (uint32_t)(T.i0 - c0) * planeB + r.rb + (uint32_t)Z.i0 * cb;
+      const uint32_t ot = T.i1 != T.i0 ? planeB : 0u, oz = Z.i1 != Z.i0 ? cb : 0u;
+      const uint32_t off[4] = {o0, o0 + oz, o0 + r.dxb, o0 + r.dxb + oz};
       double v0[4][5], v1[4][5];
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        const double2 a = b0[off[c]], b = b0[off[c] + 1], e = b1[off[c]], g = b1[off[c] + 1];
+        const double2 *b0 = reinterpret_cast<const double2 *>(base + off[c]);
+        const double2 *b1 = reinterpret_cast<const double2 *>(base + (off[c] + ot));
+        const double2 a = b0[0], b = b0[1], e = b1[0], g = b1[1];
         v0[c][0] = a.x;
         v0[c][1] = a.y;
         v0[c][2] = b.x;
         v0[c][3] = b.y;
-        v0[c][4] = reinterpret_cast<const double *>(b0 + off[c] + 2)[0];
+        v0[c][4] = reinterpret_cast<const double *>(b0 + 2)[0];
         v1[c][0] = e.x;
         v1[c][1] = e.y;
         v1[c][2] = g.x;
         v1[c][3] = g.y;
-        v1[c][4] = reinterpret_cast<const double *>(b1 + off[c] + 2)[0];
+        v1[c][4] = reinterpret_cast<const double *>(b1 + 2)[0];
       }
 #pragma unroll
       for (int f = 0; f < 5; ++f) {
@@ -209,13 +223,14 @@ __device__ inline Column make_column(const DfcsrLattice &L, const Observer &o, d
 template <bool kRecip, bool kRecent = false>
 __device__ inline bool csr_point(const DfcsrHistory &H, const HistAxes &A, const Observer &o,
                                  const Column &c, double xp, double &Iz, double &Ix,
-                                 const TruncIdx *Xpre = nullptr) {
+                                 const TruncIdx *Xpre = nullptr,
+                                 const RecentRow *rr = nullptr) {
   const double rx = c.ax - xp * c.L.nx;
   const double ry = c.ay - xp * c.L.ny;
   const double R = sqrt(rx * rx + ry * ry);
   const double t_ret = o.t - R;
   double f[5];
-  if (!hist_lookup5<kRecip, kRecent>(H, A, t_ret, xp, c.sp - t_ret, f, Xpre) && R != 0.0) {
+  if (!hist_lookup5<kRecip, kRecent>(H, A, t_ret, xp, c.sp - t_ret, f, Xpre, rr) && R != 0.0) {
     Iz = 0.0;
     Ix = 0.0;
     return false;

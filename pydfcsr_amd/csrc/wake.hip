@@ -59,6 +59,9 @@ struct RowTable {
   int i0[kMaxRows];
   int i1[kMaxRows];
   int ok[kMaxRows];
+  // byte offsets in the recent-plane copy: row i0 of plane 0, and from row i0
+  // to row i1 (the copy spans at most 4 GiB, so 32 bits hold both)
+  RecentRow rr[kMaxRows];
 };
 constexpr int kNumXcd = 8;
 constexpr int kNearGroup = 16;   // consecutive near-phase workgroups per XCD
@@ -293,7 +296,7 @@ struct Neumaier {
 template <bool kTI, bool kRecent = false>
 __device__ inline bool csr_point_mixed(const DfcsrHistory &H, const HistAxes &A, double t,
                                        const ColumnMixed &c, double xp, int xi0, int xi1,
-                                       float xd, float &Iz, float &Ix) {
+                                       RecentRow rr, float xd, float &Iz, float &Ix) {
   const double rx = c.ax - xp * c.npx;
   const double ry = c.ay - xp * c.npy;
   const double R2 = rx * rx + ry * ry;
@@ -313,22 +316,23 @@ __device__ inline bool csr_point_mixed(const DfcsrHistory &H, const HistAxes &A,
   float f[5];
   const int cr = H.nt - min(H.recent_planes, H.nt);
   if (kRecent && T.i0 >= cr) {
-    // float32 cell (x, z) of copied plane k at ((k Xi + x) Zi + z) * 6: three 8-byte
-    // loads per corner (x0,z0), (x0,z1), (x1,z0), (x1,z1) of each plane
-    const float *R32 = reinterpret_cast<const float *>(H.recent);
-    const int64_t Zi = H.Zi;
-    const float2 *b0 = reinterpret_cast<const float2 *>(
-        R32 + (((int64_t)(T.i0 - cr) * H.Xi + xi0) * Zi + Z.i0) * DFCSR_RECENT_CELL);
-    const float2 *b1 = reinterpret_cast<const float2 *>(
-        R32 + (((int64_t)(T.i1 - cr) * H.Xi + xi0) * Zi + Z.i0) * DFCSR_RECENT_CELL);
-    const int64_t dz = (int64_t)(Z.i1 - Z.i0) * 3, dx = (int64_t)(xi1 - xi0) * Zi * 3;
-    const int64_t off[4] = {0, dz, dx, dx + dz};
+    // float32 cell (x, z) of copied plane k at byte ((k Xi + x) Zi + z) * 24 of the
+    // copy: corners (x0,z0), (x0,z1), (x1,z0), (x1,z1) of each plane, addressed by
+    // 32-bit offsets from the copy's base
+    constexpr uint32_t cb = DFCSR_RECENT_CELL * sizeof(float);
+    const char *base = reinterpret_cast<const char *>(H.recent);
+    const uint32_t planeB = (uint32_t)H.Xi * (uint32_t)H.Zi * cb;
+    const uint32_t o0 = (uint32_t)(T.i0 - cr) * planeB + rr.rb + (uint32_t)Z.i0 * cb;
+    const uint32_t ot = T.i1 != T.i0 ? planeB : 0u, oz = Z.i1 != Z.i0 ? cb : 0u;
+    const uint32_t off[4] = {o0, o0 + oz, o0 + rr.dxb, o0 + rr.dxb + oz};
     float v0[4][6], v1[4][6];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
+      const float2 *b0 = reinterpret_cast<const float2 *>(base + off[q]);
+      const float2 *b1 = reinterpret_cast<const float2 *>(base + (off[q] + ot));
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        const float2 a = b0[off[q] + k], b = b1[off[q] + k];
+        const float2 a = b0[k], b = b1[k];
         v0[q][2 * k] = a.x;
         v0[q][2 * k + 1] = a.y;
         v1[q][2 * k] = b.x;
@@ -562,6 +566,12 @@ __device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int
     rt.i0[r] = X.i0;
     rt.i1[r] = X.i1;
     rt.ok[r] = X.ok;
+    if (kRecent) {
+      const uint32_t rowB = (uint32_t)H.Zi * (uint32_t)(DFCSR_RECENT_CELL *
+                                                        (kMixed ? sizeof(float) : sizeof(double)));
+      rt.rr[r].rb = (uint32_t)X.i0 * rowB;
+      rt.rr[r].dxb = (uint32_t)(X.i1 - X.i0) * rowB;
+    }
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
@@ -584,7 +594,7 @@ __device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int
         if (!rt.ok[r]) continue;
         float Iz, Ix;
         if (!csr_point_mixed<kTI, kRecent>(H, A, t, col, rt.xp[r], rt.i0[r], rt.i1[r],
-                                           (float)rt.frac[r], Iz, Ix))
+                                           rt.rr[r], (float)rt.frac[r], Iz, Ix))
           continue;
         const float wx = (float)rt.w[r];
         cz.add(wx * Iz);
@@ -599,7 +609,7 @@ __device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int
         if (!rt.ok[r]) continue;
         const TruncIdx X = {rt.i0[r], rt.i1[r], rt.frac[r], true};
         double Iz, Ix;
-        if (!csr_point<true, kRecent>(H, A, o, col, rt.xp[r], Iz, Ix, &X)) continue;
+        if (!csr_point<true, kRecent>(H, A, o, col, rt.xp[r], Iz, Ix, &X, &rt.rr[r])) continue;
         const double wx = rt.w[r];
         cz += wx * Iz;
         cx += wx * Ix;
@@ -835,6 +845,11 @@ int check_rings(const DfcsrHistory &H, int precision, const char *who) {
   if (H.recent && H.recent_planes < 1)
     return set_error(DFCSR_EINVAL, "%s: recent-plane buffer with %d planes", who,
                      H.recent_planes);
+  // the wake addresses the copy with 32-bit byte offsets
+  const int64_t cellB = DFCSR_RECENT_CELL * (precision == DFCSR_PREC_MIXED ? 4 : 8);
+  if (H.recent && (int64_t)H.recent_planes * H.Xi * H.Zi * cellB > ((int64_t)1 << 32))
+    return set_error(DFCSR_EINVAL, "%s: recent-plane copy of %d planes of %d x %d cells "
+                     "exceeds 4 GiB", who, H.recent_planes, H.Xi, H.Zi);
   if (precision == DFCSR_PREC_MIXED && !H.data32 && !H.data32t)
     return set_error(DFCSR_EINVAL, "%s: mixed precision needs an FP32 history ring", who);
   return DFCSR_OK;

@@ -148,11 +148,15 @@ class DF_tracker:
         if self.Xi != Xi or self.Zi != Zi or (self.hist is None and self.hist32 is None):
             self.hist = self.hist32 = self.hist32t = self.recent = None   # free first
             self.hist, self.hist32, self.hist32t = self._rings(Xi, Zi, self.cap)
-            if self.recent_planes > 0:
-                # float64 cells when FP64 wakes may run, else float32 cells
-                dt = torch.float64 if self.keep_fp64 else torch.float32
-                self.recent = torch.empty((self.recent_planes, Xi, Zi, _lib.RECENT_CELL),
-                                          dtype=dt, device=self.device)
+            # float64 cells when FP64 wakes may run, else float32 cells; the
+            # wakes address the copy with 32-bit byte offsets, so it holds at
+            # most as many planes as fit in 4 GiB
+            dt = torch.float64 if self.keep_fp64 else torch.float32
+            plane = Xi * Zi * _lib.RECENT_CELL * (8 if self.keep_fp64 else 4)
+            k = min(self.recent_planes, (1 << 32) // plane)
+            if k > 0:
+                self.recent = torch.empty((k, Xi, Zi, _lib.RECENT_CELL), dtype=dt,
+                                          device=self.device)
         self.Xi, self.Zi = Xi, Zi
 
     def raw_fields(self, i=None):
@@ -269,7 +273,7 @@ class DF_tracker:
             return t.data_ptr() if t is not None else None
         return _lib.History(p(self.hist), p(self.hist32), p(self.hist32t), p(self.recent),
                             self.cap, self.head, self.nt, self.Xi, self.Zi,
-                            max(1, min(int(self.recent_use), self.recent_planes))
+                            max(1, min(int(self.recent_use), self.recent.shape[0]))
                             if self.recent is not None else 0,
                             float(self.min_x), float(self.min_y), float(self.min_z),
                             float(self.delta_x), float(self.delta_y), float(self.delta_z))

@@ -154,6 +154,16 @@ def test_wake_tasks_pack_host_logic():
     rc = lib.dfcsr_wake_tasks_pack(k, hist, lat, P, _lib.DFCSR_PREC_FP64, out, out, tbl, nb,
                                    ctypes.byref(batch))
     assert rc == 0, lib.dfcsr_last_error()
+    # the wake addresses the copy with 32-bit byte offsets: at most 4 GiB
+    # (16 planes of 3000 x 3000 float32 cells fit, float64 cells do not)
+    hist[0].recent_planes, hist[0].Xi, hist[0].Zi = 16, 3000, 3000
+    rc = lib.dfcsr_wake_tasks_pack(k, hist, lat, P, _lib.DFCSR_PREC_MIXED, out, out, tbl, nb,
+                                   ctypes.byref(batch))
+    assert rc == 0, lib.dfcsr_last_error()
+    rc = lib.dfcsr_wake_tasks_pack(k, hist, lat, P, _lib.DFCSR_PREC_FP64, out, out, tbl, nb,
+                                   ctypes.byref(batch))
+    assert rc == -1 and b"exceeds 4 GiB" in lib.dfcsr_last_error()
+    hist[0].recent_planes, hist[0].Xi, hist[0].Zi = 2, 16, 16
     rc = lib.dfcsr_wake_tasks_pack(k, hist, lat, P, _lib.DFCSR_PREC_FP64, out, out, tbl, 16,
                                    ctypes.byref(batch))
     assert rc != 0 and b"too small" in lib.dfcsr_last_error()

@@ -87,8 +87,58 @@ struct RecentRow {
   uint32_t rb, dxb;
 };
 
+// The five fields of the four corner cells at byte offsets off[c] + d of
+// the float64 recent-plane copy: two 16-byte loads and one 8-byte load each.
+__device__ inline void load_cells5(const char *base, const uint32_t off[4], uint32_t d,
+                                   double v[4][5]) {
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const double2 *p = reinterpret_cast<const double2 *>(base + (off[c] + d));
+    const double2 a = p[0], b = p[1];
+    v[c][0] = a.x;
+    v[c][1] = a.y;
+    v[c][2] = b.x;
+    v[c][3] = b.y;
+    v[c][4] = reinterpret_cast<const double *>(p + 2)[0];
+  }
+}
+
+__device__ inline void copy_cells5(const double v[4][5], double w[4][5]) {
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int f = 0; f < 5; ++f) w[c][f] = v[c][f];
+}
+
+// interpolate3D's lerp a (1 - w) + b w (interp3D.py:53-61).  kFused (the
+// wake kernels): b w added to the rounded a (1 - w) by one fused
+// multiply-add; else the reference's two rounded products and a sum.
+__device__ inline double lerp_exact(double a, double b, double w) {
+#pragma clang fp contract(off)
+  return a * (1 - w) + b * w;
+}
+
+template <bool kFused>
+__device__ inline double lerp(double a, double b, double w) {
+  return kFused ? fma(b, w, a * (1 - w)) : lerp_exact(a, b, w);
+}
+
+// The trilinear value from the corners (x0,z0), (x0,z1), (x1,z0), (x1,z1) of
+// planes t0 (a*) and t1 (b*): along t, then x, then z.  Every layout's
+// lookup calls this one form, so equal corner values give equal results.
+template <bool kFused>
+__device__ inline double trilerp(double a00, double a01, double a10, double a11, double b00,
+                                 double b01, double b10, double b11, double td, double xd,
+                                 double zd) {
+  const double c00 = lerp<kFused>(a00, b00, td), c01 = lerp<kFused>(a01, b01, td);
+  const double c10 = lerp<kFused>(a10, b10, td), c11 = lerp<kFused>(a11, b11, td);
+  return lerp<kFused>(lerp<kFused>(c00, c10, xd), lerp<kFused>(c01, c11, xd), zd);
+}
+
 // Trilinear lookup of all five fields at (t, x, z) with interpolate3D's
 // truncation and validity rules.  Returns false (and zeros) when invalid.
+// kRecip: the wake kernels' form (axes scaled by reciprocals, fused lerps);
+// else the reference's divisions and unfused lerps.
 // kRecent: lookups whose planes lie in the cell-major copy of the latest
 // planes (H.recent) read it; the values, and so the result, are the same.
 // Xpre: the x index of xq when the caller has it already (the same value);
@@ -129,49 +179,39 @@ __device__ inline bool hist_lookup5(const DfcsrHistory &H, const HistAxes &A, do
       const uint32_t ot = T.i1 != T.i0 ? planeB : 0u, oz = Z.i1 != Z.i0 ? cb : 0u;
       const uint32_t off[4] = {o0, o0 + oz, o0 + r.dxb, o0 + r.dxb + oz};
       double v0[4][5], v1[4][5];
+      load_cells5(base, off, 0u, v0);
+      // at the last plane interpolate3D clamps i1 to i0: both ends of the
+      // time lerp are the same cell, read once
+      if (ot)
+        load_cells5(base, off, ot, v1);
+      else
+        copy_cells5(v0, v1);
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const double2 *b0 = reinterpret_cast<const double2 *>(base + off[c]);
-        const double2 *b1 = reinterpret_cast<const double2 *>(base + (off[c] + ot));
-        const double2 a = b0[0], b = b0[1], e = b1[0], g = b1[1];
-        v0[c][0] = a.x;
-        v0[c][1] = a.y;
-        v0[c][2] = b.x;
-        v0[c][3] = b.y;
-        v0[c][4] = reinterpret_cast<const double *>(b0 + 2)[0];
-        v1[c][0] = e.x;
-        v1[c][1] = e.y;
-        v1[c][2] = g.x;
-        v1[c][3] = g.y;
-        v1[c][4] = reinterpret_cast<const double *>(b1 + 2)[0];
-      }
-#pragma unroll
-      for (int f = 0; f < 5; ++f) {
-        double c00 = v0[0][f] * (1 - td) + v1[0][f] * td;
-        double c01 = v0[1][f] * (1 - td) + v1[1][f] * td;
-        double c10 = v0[2][f] * (1 - td) + v1[2][f] * td;
-        double c11 = v0[3][f] * (1 - td) + v1[3][f] * td;
-        double c0v = c00 * (1 - xd) + c10 * xd;
-        double c1v = c01 * (1 - xd) + c11 * xd;
-        out[f] = c0v * (1 - zd) + c1v * zd;
-      }
+      for (int f = 0; f < 5; ++f)
+        out[f] = trilerp<kRecip>(v0[0][f], v0[1][f], v0[2][f], v0[3][f], v1[0][f], v1[1][f],
+                                 v1[2][f], v1[3][f], td, xd, zd);
       return true;
     }
   }
   // corner-packed records: p0 at (t0, x0, z0), p1 at (t1, x0, z0); element
-  // f*4 + {0,1,2,3} = (x0,z0), (x0,z1), (x1,z0), (x1,z1)
-  const double *p0 = hist_rec(H, T.i0, X.i0, Z.i0), *p1 = hist_rec(H, T.i1, X.i0, Z.i0);
+  // f*4 + {0,1,2,3} = (x0,z0), (x0,z1), (x1,z0), (x1,z1).  At the last plane
+  // (i1 == i0) the record is read once.
+  const double2 *p0 = reinterpret_cast<const double2 *>(hist_rec(H, T.i0, X.i0, Z.i0));
+  double2 a[DFCSR_HIST_REC64 / 2], b[DFCSR_HIST_REC64 / 2];
 #pragma unroll
-  for (int f = 0; f < 5; ++f) {
-    // interp3D.py:53-61: lerp along t, then x, then z
-    double c00 = p0[4 * f + 0] * (1 - td) + p1[4 * f + 0] * td;
-    double c01 = p0[4 * f + 1] * (1 - td) + p1[4 * f + 1] * td;
-    double c10 = p0[4 * f + 2] * (1 - td) + p1[4 * f + 2] * td;
-    double c11 = p0[4 * f + 3] * (1 - td) + p1[4 * f + 3] * td;
-    double c0 = c00 * (1 - xd) + c10 * xd;
-    double c1 = c01 * (1 - xd) + c11 * xd;
-    out[f] = c0 * (1 - zd) + c1 * zd;
+  for (int k = 0; k < DFCSR_HIST_REC64 / 2; ++k) a[k] = p0[k];
+  if (T.i1 != T.i0) {
+    const double2 *p1 = reinterpret_cast<const double2 *>(hist_rec(H, T.i1, X.i0, Z.i0));
+#pragma unroll
+    for (int k = 0; k < DFCSR_HIST_REC64 / 2; ++k) b[k] = p1[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < DFCSR_HIST_REC64 / 2; ++k) b[k] = a[k];
   }
+#pragma unroll
+  for (int f = 0; f < 5; ++f)
+    out[f] = trilerp<kRecip>(a[2 * f].x, a[2 * f].y, a[2 * f + 1].x, a[2 * f + 1].y, b[2 * f].x,
+                             b[2 * f].y, b[2 * f + 1].x, b[2 * f + 1].y, td, xd, zd);
   return true;
 }
 

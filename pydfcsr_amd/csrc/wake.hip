@@ -289,6 +289,22 @@ struct Neumaier {
   __device__ float value() const { return s + c; }
 };
 
+// The five fields of the four corner cells at byte offsets off[q] + d of
+// the float32 recent-plane copy (three 8-byte loads each; the pad too).
+__device__ inline void load_cells32(const char *base, const uint32_t off[4], uint32_t d,
+                                    float v[4][6]) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const float2 *p = reinterpret_cast<const float2 *>(base + (off[q] + d));
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float2 a = p[k];
+      v[q][2 * k] = a.x;
+      v[q][2 * k + 1] = a.y;
+    }
+  }
+}
+
 // One source point in mixed precision; false when outside the history.
 // kTI: read the time-innermost FP32 ring, else the chunked one.  kRecent:
 // lookups on the latest planes read the float32 cell-major copy in H.recent
@@ -325,29 +341,31 @@ __device__ inline bool csr_point_mixed(const DfcsrHistory &H, const HistAxes &A,
     const uint32_t o0 = (uint32_t)(T.i0 - cr) * planeB + rr.rb + (uint32_t)Z.i0 * cb;
     const uint32_t ot = T.i1 != T.i0 ? planeB : 0u, oz = Z.i1 != Z.i0 ? cb : 0u;
     const uint32_t off[4] = {o0, o0 + oz, o0 + rr.dxb, o0 + rr.dxb + oz};
-    float v0[4][6], v1[4][6];
+    float v0[4][6];
+    load_cells32(base, off, 0u, v0);
+    if (ot) {
+      float v1[4][6];
+      load_cells32(base, off, ot, v1);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const float2 *b0 = reinterpret_cast<const float2 *>(base + off[q]);
-      const float2 *b1 = reinterpret_cast<const float2 *>(base + (off[q] + ot));
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const float2 a = b0[k], b = b1[k];
-        v0[q][2 * k] = a.x;
-        v0[q][2 * k + 1] = a.y;
-        v1[q][2 * k] = b.x;
-        v1[q][2 * k + 1] = b.y;
+      for (int k = 0; k < 5; ++k) {
+        const float c00 = fmaf(td, v1[0][k] - v0[0][k], v0[0][k]);
+        const float c01 = fmaf(td, v1[1][k] - v0[1][k], v0[1][k]);
+        const float c10 = fmaf(td, v1[2][k] - v0[2][k], v0[2][k]);
+        const float c11 = fmaf(td, v1[3][k] - v0[3][k], v0[3][k]);
+        const float c0 = fmaf(xd, c10 - c00, c00);
+        const float c1 = fmaf(xd, c11 - c01, c01);
+        f[k] = fmaf(zd, c1 - c0, c0);
       }
-    }
+    } else {
+      // the last plane (interpolate3D clamps i1 to i0): the time lerp of a
+      // value with itself, fmaf(td, 0, v), is v exactly, so the cells are read
+      // once and only the (x, z) lerps remain
 #pragma unroll
-    for (int k = 0; k < 5; ++k) {
-      const float c00 = fmaf(td, v1[0][k] - v0[0][k], v0[0][k]);
-      const float c01 = fmaf(td, v1[1][k] - v0[1][k], v0[1][k]);
-      const float c10 = fmaf(td, v1[2][k] - v0[2][k], v0[2][k]);
-      const float c11 = fmaf(td, v1[3][k] - v0[3][k], v0[3][k]);
-      const float c0 = fmaf(xd, c10 - c00, c00);
-      const float c1 = fmaf(xd, c11 - c01, c01);
-      f[k] = fmaf(zd, c1 - c0, c0);
+      for (int k = 0; k < 5; ++k) {
+        const float c0 = fmaf(xd, v0[2][k] - v0[0][k], v0[0][k]);
+        const float c1 = fmaf(xd, v0[3][k] - v0[1][k], v0[1][k]);
+        f[k] = fmaf(zd, c1 - c0, c0);
+      }
     }
   } else {
     // one corner-packed record per time plane, one 16-byte chunk per field

@@ -368,8 +368,9 @@ def main():
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
                      # bytes per wake from the committed counter runs of this workload
                      "traffic": profiled_traffic() if args.workload == "chicane" else None,
-                     "kernel": "FP64 wake (wake_observer_kernel, wake_phase_kernel far and "
-                               "near, wake_finalize_kernel; one stream, timed together)",
+                     "kernel": "FP64 wake (wake_observer_kernel, wake_phase_kernel far, "
+                               "recent_copy_kernel, wake_phase_kernel near, "
+                               "wake_finalize_kernel; one stream, timed together)",
                      "flops_per_point": FLOPS_PER_POINT},
         "init_s": round(t0 - t_init, 2),
     }

@@ -363,14 +363,14 @@ def main():
         "wake_ms": round(wake_avg_ms, 3),
         "history": {"slices": tr.nt, "Xi": tr.Xi, "Zi": tr.Zi,
                     "bytes": int(tr.footprint_bytes()), "regrids": tr.n_regrid,
-                            "recent_planes": int(tr.recent_use)},
+                    "cell_major": [k for k in ("cells", "cells32")
+                                   if getattr(tr, k) is not None]},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
                      # bytes per wake from the committed counter runs of this workload
                      "traffic": profiled_traffic() if args.workload == "chicane" else None,
-                     "kernel": "FP64 wake (wake_observer_kernel, wake_phase_kernel far, "
-                               "recent_copy_kernel, wake_phase_kernel near, "
-                               "wake_finalize_kernel; one stream, timed together)",
+                     "kernel": "FP64 wake (wake_observer_kernel, wake_phase_kernel far and "
+                               "near, wake_finalize_kernel; one stream, timed together)",
                      "flops_per_point": FLOPS_PER_POINT},
         "init_s": round(t0 - t_init, 2),
     }

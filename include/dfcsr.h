@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define DFCSR_ABI_VERSION 3
+#define DFCSR_ABI_VERSION 4
 
 enum {
   DFCSR_OK = 0,
@@ -153,7 +153,8 @@ int dfcsr_df_process(double *density, double *vx, double *density_x, double *den
  * fields as [field][nbx][nbz] (field order DFCSR_F_*).
  * jobs: HOST array of njobs descriptors; hist64 (FP64), hist32 (FP32
  * chunked) and hist32t (FP32 time-innermost) are rings of cap slots in the
- * layouts above (dst_slot < cap); each may be NULL, but not all three. */
+ * layouts above (dst_slot < cap), cells and cells32 the cell-major rings of
+ * DfcsrHistory; each may be NULL, but not all five. */
 typedef struct {
   int32_t src_slot;   /* raw slice index in src */
   int32_t dst_slot;   /* ring slot in the history */
@@ -162,9 +163,9 @@ typedef struct {
 } DfcsrRegridJob;
 int64_t dfcsr_regrid_workspace_bytes(int njobs);
 int dfcsr_regrid(const double *src, int64_t src_slot_stride, const DfcsrRegridJob *jobs,
-                 int njobs, double *hist64, float *hist32, float *hist32t, int Xi, int Zi,
-                 int cap, double xi0, double xi1, double zi0, double zi1, void *ws,
-                 int64_t ws_bytes, void *stream);
+                 int njobs, double *hist64, float *hist32, float *hist32t, double *cells,
+                 float *cells32, int Xi, int Zi, int cap, double xi0, double xi1, double zi0,
+                 double zi1, void *ws, int64_t ws_bytes, void *stream);
 
 /* ---- a11/a12: lookup primitives (exposed for parity tests) -----------------
  * History descriptor: ring of cap slots, logical slice i lives in slot
@@ -173,24 +174,25 @@ int dfcsr_regrid(const double *src, int64_t src_slot_stride, const DfcsrRegridJo
  * dx = (xi1 - xi0)/(Xi - 1), same for z.  The FP64 ring is needed by the
  * lookup primitives, the integrand and FP64 wakes; a mixed-precision wake
  * needs at least one FP32 ring (with both, its region-0 phases read the
- * time-innermost one and the other phases the chunked one). */
+ * time-innermost one and the other phases the chunked one).
+ * The cell-major rings (optional) hold the same slices plane by plane:
+ * cell (slot, x, z) at ((slot Xi + x) Zi + z) DFCSR_CELL, the five fields
+ * of that grid node then a pad (48 B in float64, 24 B in float32, against
+ * 160 B / 80 B corner-packed records).  With them the near-region phases of
+ * a wake read cells (FP64 wakes: cells, mixed: cells32), whose neighbouring
+ * z lie in neighbouring cells; results are identical with or without them.
+ * One plane must span at most 4 GiB (the wakes address cells within a plane
+ * with 32-bit offsets). */
 typedef struct {
   const double *data;    /* [Xi][Zi][cap][DFCSR_HIST_REC64] or NULL */
   const float *data32;   /* [Xi][cap][DFCSR_HIST_CHUNKS32][Zi][4] or NULL */
   const float *data32t;  /* [Xi][Zi][cap][DFCSR_HIST_REC32] or NULL */
-  /* Optional wake scratch, [recent_planes][Xi][Zi][DFCSR_RECENT_CELL]
-   * doubles for FP64 wakes, floats for mixed ones (or NULL): each wake first
-   * copies the latest K = min(recent_planes, nt) planes of the ring it reads
-   * into it, cell-major (the five fields of cell (x, z), then a pad), and its
-   * near-region lookups that fall on those planes read the copy -- less
-   * than a third of the bytes of the corner-packed records, neighbouring z
-   * in neighbouring cells.  Results are identical with or without it. */
-  double *recent;
+  const double *cells;   /* [cap][Xi][Zi][DFCSR_CELL] or NULL */
+  const float *cells32;  /* [cap][Xi][Zi][DFCSR_CELL] or NULL */
   int32_t cap, head, nt, Xi, Zi;
-  int32_t recent_planes;
   double min_t, min_x, min_z, dt, dx, dz;
 } DfcsrHistory;
-#define DFCSR_RECENT_CELL 6
+#define DFCSR_CELL 6
 
 /* Lattice tables (lattice.py:4-110, 136-150): table[k][0..5] =
  * coords_x, coords_y, n_x, n_y, tau_x, tau_y at s = min_s + k*ds, k < ns;

@@ -20,7 +20,7 @@ c_i32 = ctypes.c_int32
 c_i64 = ctypes.c_int64
 c_f64 = ctypes.c_double
 
-ABI_VERSION = 3   # DFCSR_ABI_VERSION
+ABI_VERSION = 4   # DFCSR_ABI_VERSION
 DFCSR_PREC_FP64 = 0
 DFCSR_PREC_MIXED = 1
 MOMENTS_COUNT = 10
@@ -31,9 +31,9 @@ TWISS_KEYS = ("alpha", "beta", "gamma", "emit", "eta", "etap", "norm_emit")
 HIST_REC64 = 20   # corner-packed records: field f, corner c at [4 f + c]
 HIST_REC32 = 20
 HIST_CHUNKS32 = 5
-RECENT_CELL = 6   # doubles per cell of the FP64 wake's recent-plane copy
+CELL = 6   # values per cell of the cell-major rings: five fields, a pad
 # rings: FP64 (Xi, Zi, cap, 20); FP32 chunked (Xi, cap, 5, Zi, 4); FP32
-# time-innermost (Xi, Zi, cap, 20)
+# time-innermost (Xi, Zi, cap, 20); cell-major (cap, Xi, Zi, 6) float64 / float32
 FIELDS = ("density", "density_x", "density_z", "vx", "vx_x")   # DFCSR_F_* order
 
 
@@ -47,9 +47,9 @@ class RegridJob(ctypes.Structure):
 
 
 class History(ctypes.Structure):
-    _fields_ = [("data", c_dp), ("data32", c_dp), ("data32t", c_dp), ("recent", c_dp),
-                ("cap", c_i32), ("head", c_i32), ("nt", c_i32), ("Xi", c_i32), ("Zi", c_i32),
-                ("recent_planes", c_i32), ("min_t", c_f64), ("min_x", c_f64), ("min_z", c_f64),
+    _fields_ = [("data", c_dp), ("data32", c_dp), ("data32t", c_dp), ("cells", c_dp),
+                ("cells32", c_dp), ("cap", c_i32), ("head", c_i32), ("nt", c_i32), ("Xi", c_i32),
+                ("Zi", c_i32), ("min_t", c_f64), ("min_x", c_f64), ("min_z", c_f64),
                 ("dt", c_f64), ("dx", c_f64), ("dz", c_f64)]
 
 
@@ -102,8 +102,8 @@ _SIGNATURES = [
     ("dfcsr_savgol_operator", c_i32, [c_i32, c_i32, c_dp, c_dp]),
     ("dfcsr_regrid_workspace_bytes", c_i64, [c_i32]),
     ("dfcsr_regrid", c_i32, [c_dp, c_i64, ctypes.POINTER(RegridJob), c_i32, c_dp, c_dp, c_dp,
-                             c_i32, c_i32, c_i32, c_f64, c_f64, c_f64, c_f64, c_dp, c_i64,
-                             c_dp]),
+                             c_dp, c_dp, c_i32, c_i32, c_i32, c_f64, c_f64, c_f64, c_f64, c_dp,
+                             c_i64, c_dp]),
     ("dfcsr_interpolate3d", c_i32, [ctypes.POINTER(History), c_i32, c_dp, c_i64, c_dp, c_dp]),
     ("dfcsr_interpolate1d", c_i32, [ctypes.POINTER(Lattice), c_i32, c_dp, c_i64, c_dp, c_dp]),
     ("dfcsr_csr_integrand", c_i32, [ctypes.POINTER(History), ctypes.POINTER(Lattice), c_f64,

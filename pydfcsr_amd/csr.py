@@ -13,7 +13,6 @@ geometry and final sum).
 """
 from __future__ import annotations
 
-import math
 import os
 import time
 
@@ -298,41 +297,6 @@ class CSR2D:
         self.CSR_zrange = np.linspace(*self.mesh_z_axis)
         self.CSR_xrange_transformed = np.linspace(*self.mesh_x_axis)
         self._mesh_slope = np.array(m["slope"])
-        self.DF_tracker.recent_use = self.near_planes()
-
-    def near_planes(self):
-        """How many of the latest history planes the near regions' retarded
-        lookups can reach on the current mesh, from get_CSR_wake's region
-        bounds (CSR.py:461-553) and |r - r'| <= |s - s'| + |x| + |x'|: the
-        depth of the wake's recent-plane copy (lookups beyond it read the
-        ring; only speed depends on it)."""
-        tr = self.DF_tracker
-        dt = float(getattr(tr, "delta_x", float("nan")))
-        if tr.nt < 2 or not (dt > 0) or not math.isfinite(dt):
-            return tr.recent_planes
-        b = self.beam
-        sz, sx, k = float(b._sigma_z), float(b._sigma_x), float(b._slope[0])
-        zs = self.mesh_z_axis[:2]
-        xo = [xt + k * z + float(self._mesh_slope[1]) for xt in self.mesh_x_axis[:2] for z in zs]
-        xmax = max(abs(v) for v in xo)
-        zmax = max(abs(z) for z in zs)
-        if abs(k) <= 1:            # s' in [s - 500 sz, s + 5 sz], x' = x0 +- 10 sx
-            reach = 500 * sz
-            xpmax = zmax * abs(k) + 10 * sx
-        else:                      # s' in [s3 - 200 sz, s + 3 sz], s3 = max(0, s - d)
-            with np.errstate(all="ignore"):
-                if k > 0:
-                    ta = -2 * k / (1 - k * k)
-                    d = [(10 * sx + b._mean_x - x) / ta for x in xo]
-                else:
-                    ta = 2 * k / (1 - k * k)
-                    d = [-(b._mean_x - x - 10 * sx) / ta for x in xo]
-            reach = max(abs(v) for v in d) + 200 * sz
-            xpmax = max(xmax + 10 * sx, zmax * abs(k) + 5 * sx)
-        R = reach + xmax + xpmax
-        if not math.isfinite(R):
-            return tr.recent_planes
-        return int(min(tr.recent_planes, tr.nt, math.ceil(R / dt) + 2))
 
     def wake_params(self):
         b = self.beam

@@ -80,20 +80,19 @@ __device__ inline const double *hist_rec(const DfcsrHistory &H, int ti, int xi, 
   return H.data + (((int64_t)xi * H.Zi + zi) * H.cap + hist_slot(H, ti)) * DFCSR_HIST_REC64;
 }
 
-// Byte offsets of an x row in the recent-plane copy: row i0 of plane 0, and
-// from row i0 to row i1.  The copy spans at most 4 GiB (checked on the
-// host), so lookups address it with 32-bit offsets from its base.
-struct RecentRow {
+// Byte offsets of an x row within a plane of a cell-major ring: row i0, and
+// from row i0 to row i1.  A plane spans at most 4 GiB (checked on the
+// host), so lookups address cells with 32-bit offsets from the plane.
+struct CellRow {
   uint32_t rb, dxb;
 };
 
-// The five fields of the four corner cells at byte offsets off[c] + d of
-// the float64 recent-plane copy: two 16-byte loads and one 8-byte load each.
-__device__ inline void load_cells5(const char *base, const uint32_t off[4], uint32_t d,
-                                   double v[4][5]) {
+// The five fields of the four corner cells at byte offsets off[c] of a
+// float64 cell plane: two 16-byte loads and one 8-byte load each.
+__device__ inline void load_cells5(const char *plane, const uint32_t off[4], double v[4][5]) {
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
-    const double2 *p = reinterpret_cast<const double2 *>(base + (off[c] + d));
+    const double2 *p = reinterpret_cast<const double2 *>(plane + off[c]);
     const double2 a = p[0], b = p[1];
     v[c][0] = a.x;
     v[c][1] = a.y;
@@ -139,15 +138,15 @@ __device__ inline double trilerp(double a00, double a01, double a10, double a11,
 // truncation and validity rules.  Returns false (and zeros) when invalid.
 // kRecip: the wake kernels' form (axes scaled by reciprocals, fused lerps);
 // else the reference's divisions and unfused lerps.
-// kRecent: lookups whose planes lie in the cell-major copy of the latest
-// planes (H.recent) read it; the values, and so the result, are the same.
+// kCells: read the cell-major FP64 ring H.cells instead of the corner-packed
+// one; the values, and so the result, are the same.
 // Xpre: the x index of xq when the caller has it already (the same value);
-// rr: that row's offsets in the copy (computed here when absent).
-template <bool kRecip, bool kRecent = false>
+// rr: that row's offsets in a cell plane (computed here when absent).
+template <bool kRecip, bool kCells = false>
 __device__ inline bool hist_lookup5(const DfcsrHistory &H, const HistAxes &A, double tq,
                                     double xq, double zq, double out[5],
                                     const TruncIdx *Xpre = nullptr,
-                                    const RecentRow *rr = nullptr) {
+                                    const CellRow *rr = nullptr) {
   double ut = kRecip ? (tq - A.min_t) * A.it : (tq - A.min_t) / A.dt;
   double uz = kRecip ? (zq - A.min_z) * A.iz : (zq - A.min_z) / A.dz;
   TruncIdx X;
@@ -163,35 +162,30 @@ __device__ inline bool hist_lookup5(const DfcsrHistory &H, const HistAxes &A, do
     return false;
   }
   const double td = T.frac, xd = X.frac, zd = Z.frac;
-  if (kRecent) {
-    const int c0 = H.nt - min(H.recent_planes, H.nt);
-    if (T.i0 >= c0) {
-      // cell (x, z) of copied plane k at byte ((k Xi + x) Zi + z) * 48 of the copy:
-      // fields 0..3 as two 16-byte loads and field 4, per corner (x0,z0), (x0,z1),
-      // (x1,z0), (x1,z1)
-      constexpr uint32_t cb = DFCSR_RECENT_CELL * sizeof(double);
-      const uint32_t rowB = (uint32_t)H.Zi * cb;
-      const RecentRow r = rr ? *rr : RecentRow{(uint32_t)X.i0 * rowB,
-                                               (uint32_t)(X.i1 - X.i0) * rowB};
-      const char *base = reinterpret_cast<const char *>(H.recent);
-      const uint32_t planeB = (uint32_t)H.Xi * rowB;
-      const uint32_t o0 = (uint32_t)(T.i0 - c0) * planeB + r.rb + (uint32_t)Z.i0 * cb;
-      const uint32_t ot = T.i1 != T.i0 ? planeB : 0u, oz = Z.i1 != Z.i0 ? cb : 0u;
-      const uint32_t off[4] = {o0, o0 + oz, o0 + r.dxb, o0 + r.dxb + oz};
-      double v0[4][5], v1[4][5];
-      load_cells5(base, off, 0u, v0);
-      // at the last plane interpolate3D clamps i1 to i0: both ends of the
-      // time lerp are the same cell, read once
-      if (ot)
-        load_cells5(base, off, ot, v1);
-      else
-        copy_cells5(v0, v1);
+  if (kCells) {
+    // cell (slot, x, z) at byte ((slot Xi + x) Zi + z) * 48 of the cell-major ring:
+    // fields 0..3 as two 16-byte loads and field 4, per corner (x0,z0), (x0,z1),
+    // (x1,z0), (x1,z1)
+    constexpr uint32_t cb = DFCSR_CELL * sizeof(double);
+    const uint32_t rowB = (uint32_t)H.Zi * cb;
+    const CellRow r = rr ? *rr : CellRow{(uint32_t)X.i0 * rowB, (uint32_t)(X.i1 - X.i0) * rowB};
+    const int64_t planeB = (int64_t)H.Xi * rowB;
+    const char *ring = reinterpret_cast<const char *>(H.cells);
+    const uint32_t o0 = r.rb + (uint32_t)Z.i0 * cb, oz = Z.i1 != Z.i0 ? cb : 0u;
+    const uint32_t off[4] = {o0, o0 + oz, o0 + r.dxb, o0 + r.dxb + oz};
+    double v0[4][5], v1[4][5];
+    load_cells5(ring + hist_slot(H, T.i0) * planeB, off, v0);
+    // at the last plane interpolate3D clamps i1 to i0: both ends of the time
+    // lerp are the same cell, read once
+    if (T.i1 != T.i0)
+      load_cells5(ring + hist_slot(H, T.i1) * planeB, off, v1);
+    else
+      copy_cells5(v0, v1);
 #pragma unroll
-      for (int f = 0; f < 5; ++f)
-        out[f] = trilerp<kRecip>(v0[0][f], v0[1][f], v0[2][f], v0[3][f], v1[0][f], v1[1][f],
-                                 v1[2][f], v1[3][f], td, xd, zd);
-      return true;
-    }
+    for (int f = 0; f < 5; ++f)
+      out[f] = trilerp<kRecip>(v0[0][f], v0[1][f], v0[2][f], v0[3][f], v1[0][f], v1[1][f],
+                               v1[2][f], v1[3][f], td, xd, zd);
+    return true;
   }
   // corner-packed records: p0 at (t0, x0, z0), p1 at (t1, x0, z0); element
   // f*4 + {0,1,2,3} = (x0,z0), (x0,z1), (x1,z0), (x1,z1).  At the last plane
@@ -260,17 +254,17 @@ __device__ inline Column make_column(const DfcsrLattice &L, const Observer &o, d
 // lookup falls outside the history, where every gathered field is zero and
 // the reference's formula gives exactly 0 -- except at R == 0, where its
 // divisions give 0/0: that point is evaluated (with zero fields) as it is.
-template <bool kRecip, bool kRecent = false>
+template <bool kRecip, bool kCells = false>
 __device__ inline bool csr_point(const DfcsrHistory &H, const HistAxes &A, const Observer &o,
                                  const Column &c, double xp, double &Iz, double &Ix,
                                  const TruncIdx *Xpre = nullptr,
-                                 const RecentRow *rr = nullptr) {
+                                 const CellRow *rr = nullptr) {
   const double rx = c.ax - xp * c.L.nx;
   const double ry = c.ay - xp * c.L.ny;
   const double R = sqrt(rx * rx + ry * ry);
   const double t_ret = o.t - R;
   double f[5];
-  if (!hist_lookup5<kRecip, kRecent>(H, A, t_ret, xp, c.sp - t_ret, f, Xpre, rr) && R != 0.0) {
+  if (!hist_lookup5<kRecip, kCells>(H, A, t_ret, xp, c.sp - t_ret, f, Xpre, rr) && R != 0.0) {
     Iz = 0.0;
     Ix = 0.0;
     return false;

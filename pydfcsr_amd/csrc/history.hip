@@ -67,12 +67,13 @@ __device__ inline void rgi5(const DfcsrRegridJob &J, const double *__restrict__ 
 }
 
 // One thread per target record: evaluates the four corner points of the
-// record and writes the corner-packed record into each ring present.
+// record and writes the corner-packed record into each ring present, and
+// the record's own grid node (its first corner) into each cell-major ring.
 __global__ __launch_bounds__(kThreads) void regrid_kernel(
     const double *__restrict__ src, int64_t slot_stride, const DfcsrRegridJob *__restrict__ jobs,
     const double *__restrict__ means, double *__restrict__ h64, float *__restrict__ h32,
-    float *__restrict__ h32t, int Xi, int Zi, int cap, double xi0, double xi1, double zi0,
-    double zi1) {
+    float *__restrict__ h32t, double *__restrict__ c64, float *__restrict__ c32, int Xi, int Zi,
+    int cap, double xi0, double xi1, double zi0, double zi1) {
   const DfcsrRegridJob J = jobs[blockIdx.y];
   const int64_t cells = (int64_t)Xi * Zi;
   const double *base = src + (int64_t)J.src_slot * slot_stride;
@@ -110,6 +111,19 @@ __global__ __launch_bounds__(kThreads) void regrid_kernel(
         s4[f * (int64_t)Zi] = make_float4((float)v[0][f], (float)v[1][f], (float)v[2][f],
                                           (float)v[3][f]);
     }
+    const int64_t cell = ((int64_t)J.dst_slot * cells + c) * DFCSR_CELL;
+    if (c64) {
+      double2 *o = reinterpret_cast<double2 *>(c64 + cell);
+      o[0] = make_double2(v[0][0], v[0][1]);
+      o[1] = make_double2(v[0][2], v[0][3]);
+      o[2] = make_double2(v[0][4], 0.0);
+    }
+    if (c32) {
+      float2 *o = reinterpret_cast<float2 *>(c32 + cell);
+      o[0] = make_float2((float)v[0][0], (float)v[0][1]);
+      o[1] = make_float2((float)v[0][2], (float)v[0][3]);
+      o[2] = make_float2((float)v[0][4], 0.0f);
+    }
   }
 }
 
@@ -124,11 +138,13 @@ extern "C" int64_t dfcsr_regrid_workspace_bytes(int njobs) {
 }
 
 extern "C" int dfcsr_regrid(const double *src, int64_t src_slot_stride, const DfcsrRegridJob *jobs,
-                            int njobs, double *hist64, float *hist32, float *hist32t, int Xi,
-                            int Zi, int cap, double xi0, double xi1, double zi0, double zi1,
-                            void *ws, int64_t ws_bytes, void *stream) {
+                            int njobs, double *hist64, float *hist32, float *hist32t,
+                            double *cells, float *cells32, int Xi, int Zi, int cap, double xi0,
+                            double xi1, double zi0, double zi1, void *ws, int64_t ws_bytes,
+                            void *stream) {
   if (!src || !jobs || !ws) return set_error(DFCSR_EINVAL, "regrid: NULL pointer");
-  if (!hist64 && !hist32 && !hist32t) return set_error(DFCSR_EINVAL, "regrid: no history ring");
+  if (!hist64 && !hist32 && !hist32t && !cells && !cells32)
+    return set_error(DFCSR_EINVAL, "regrid: no history ring");
   if (njobs < 1) return DFCSR_OK;
   if (njobs > kMaxJobs) return set_error(DFCSR_ERANGE, "regrid: %d jobs > %d", njobs, kMaxJobs);
   if (Xi < 2 || Zi < 2) return set_error(DFCSR_EINVAL, "regrid: interp grid %d x %d", Xi, Zi);
@@ -152,12 +168,12 @@ extern "C" int dfcsr_regrid(const double *src, int64_t src_slot_stride, const Df
                                  hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(vxx_mean_kernel, dim3(njobs), dim3(kThreads), 0, s, src, src_slot_stride,
                      (const DfcsrRegridJob *)djobs, means);
-  int64_t cells = (int64_t)Xi * Zi;
-  int gx = (int)((cells + kThreads - 1) / kThreads);
+  const int64_t npts = (int64_t)Xi * Zi;
+  int gx = (int)((npts + kThreads - 1) / kThreads);
   if (gx > 2048) gx = 2048;
   hipLaunchKernelGGL(regrid_kernel, dim3(gx, njobs), dim3(kThreads), 0, s, src, src_slot_stride,
                      (const DfcsrRegridJob *)djobs, (const double *)means, hist64, hist32, hist32t,
-                     Xi, Zi, cap, xi0, xi1, zi0, zi1);
+                     cells, cells32, Xi, Zi, cap, xi0, xi1, zi0, zi1);
   DFCSR_LAUNCH_CHECK("regrid");
   return DFCSR_OK;
 }

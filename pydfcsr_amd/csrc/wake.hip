@@ -59,9 +59,9 @@ struct RowTable {
   int i0[kMaxRows];
   int i1[kMaxRows];
   int ok[kMaxRows];
-  // byte offsets in the recent-plane copy: row i0 of plane 0, and from row i0
-  // to row i1 (the copy spans at most 4 GiB, so 32 bits hold both)
-  RecentRow rr[kMaxRows];
+  // byte offsets of row i0, and from row i0 to row i1, within a plane of the
+  // cell-major ring (a plane spans at most 4 GiB, so 32 bits hold both)
+  CellRow rr[kMaxRows];
 };
 constexpr int kNumXcd = 8;
 constexpr int kNearGroup = 16;   // consecutive near-phase workgroups per XCD
@@ -289,13 +289,12 @@ struct Neumaier {
   __device__ float value() const { return s + c; }
 };
 
-// The five fields of the four corner cells at byte offsets off[q] + d of
-// the float32 recent-plane copy (three 8-byte loads each; the pad too).
-__device__ inline void load_cells32(const char *base, const uint32_t off[4], uint32_t d,
-                                    float v[4][6]) {
+// The five fields of the four corner cells at byte offsets off[q] of a
+// float32 cell plane (three 8-byte loads each; the pad too).
+__device__ inline void load_cells32(const char *plane, const uint32_t off[4], float v[4][6]) {
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const float2 *p = reinterpret_cast<const float2 *>(base + (off[q] + d));
+    const float2 *p = reinterpret_cast<const float2 *>(plane + off[q]);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       const float2 a = p[k];
@@ -306,13 +305,13 @@ __device__ inline void load_cells32(const char *base, const uint32_t off[4], uin
 }
 
 // One source point in mixed precision; false when outside the history.
-// kTI: read the time-innermost FP32 ring, else the chunked one.  kRecent:
-// lookups on the latest planes read the float32 cell-major copy in H.recent
-// (same values, same arithmetic).
-template <bool kTI, bool kRecent = false>
+// kTI: read the time-innermost FP32 ring, else the chunked one.  kCells:
+// read the float32 cell-major ring H.cells32 instead (same values, same
+// arithmetic).
+template <bool kTI, bool kCells = false>
 __device__ inline bool csr_point_mixed(const DfcsrHistory &H, const HistAxes &A, double t,
                                        const ColumnMixed &c, double xp, int xi0, int xi1,
-                                       RecentRow rr, float xd, float &Iz, float &Ix) {
+                                       CellRow rr, float xd, float &Iz, float &Ix) {
   const double rx = c.ax - xp * c.npx;
   const double ry = c.ay - xp * c.npy;
   const double R2 = rx * rx + ry * ry;
@@ -330,22 +329,20 @@ __device__ inline bool csr_point_mixed(const DfcsrHistory &H, const HistAxes &A,
   if (!(T.ok && Z.ok)) return false;
   const float td = (float)T.frac, zd = (float)Z.frac;
   float f[5];
-  const int cr = H.nt - min(H.recent_planes, H.nt);
-  if (kRecent && T.i0 >= cr) {
-    // float32 cell (x, z) of copied plane k at byte ((k Xi + x) Zi + z) * 24 of the
-    // copy: corners (x0,z0), (x0,z1), (x1,z0), (x1,z1) of each plane, addressed by
-    // 32-bit offsets from the copy's base
-    constexpr uint32_t cb = DFCSR_RECENT_CELL * sizeof(float);
-    const char *base = reinterpret_cast<const char *>(H.recent);
-    const uint32_t planeB = (uint32_t)H.Xi * (uint32_t)H.Zi * cb;
-    const uint32_t o0 = (uint32_t)(T.i0 - cr) * planeB + rr.rb + (uint32_t)Z.i0 * cb;
-    const uint32_t ot = T.i1 != T.i0 ? planeB : 0u, oz = Z.i1 != Z.i0 ? cb : 0u;
+  if (kCells) {
+    // float32 cell (slot, x, z) at byte ((slot Xi + x) Zi + z) * 24 of the cell-major
+    // ring: corners (x0,z0), (x0,z1), (x1,z0), (x1,z1) of each plane, addressed by
+    // 32-bit offsets within the plane
+    constexpr uint32_t cb = DFCSR_CELL * sizeof(float);
+    const int64_t planeB = (int64_t)H.Xi * H.Zi * cb;
+    const char *ring = reinterpret_cast<const char *>(H.cells32);
+    const uint32_t o0 = rr.rb + (uint32_t)Z.i0 * cb, oz = Z.i1 != Z.i0 ? cb : 0u;
     const uint32_t off[4] = {o0, o0 + oz, o0 + rr.dxb, o0 + rr.dxb + oz};
     float v0[4][6];
-    load_cells32(base, off, 0u, v0);
-    if (ot) {
+    load_cells32(ring + hist_slot(H, T.i0) * planeB, off, v0);
+    if (T.i1 != T.i0) {
       float v1[4][6];
-      load_cells32(base, off, ot, v1);
+      load_cells32(ring + hist_slot(H, T.i1) * planeB, off, v1);
 #pragma unroll
       for (int k = 0; k < 5; ++k) {
         const float c00 = fmaf(td, v1[0][k] - v0[0][k], v0[0][k]);
@@ -518,10 +515,9 @@ __device__ inline void observer_point(const WakeTask &T, int i, double *__restri
 
 // Integrates phase ph of the mesh point whose observer data is od; every
 // thread returns the block sums of wz_k wx_j (I_z, I_x) over the phase.
-// kTI (mixed only): read the time-innermost FP32 ring.  kRecent: lookups
-// on the latest planes read the cell-major copy H.recent (float64 cells for
-// FP64, float32 cells for mixed).
-template <bool kMixed, bool kTI, bool kRecent = false>
+// kTI (mixed only): read the time-innermost FP32 ring.  kCells: read the
+// cell-major ring (H.cells for FP64, H.cells32 for mixed).
+template <bool kMixed, bool kTI, bool kCells = false>
 __device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int ph, double *red,
                            RowTable &rt, double &out_z, double &out_x) {
   const DfcsrHistory &H = T.H;
@@ -584,8 +580,8 @@ __device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int
     rt.i0[r] = X.i0;
     rt.i1[r] = X.i1;
     rt.ok[r] = X.ok;
-    if (kRecent) {
-      const uint32_t rowB = (uint32_t)H.Zi * (uint32_t)(DFCSR_RECENT_CELL *
+    if (kCells) {
+      const uint32_t rowB = (uint32_t)H.Zi * (uint32_t)(DFCSR_CELL *
                                                         (kMixed ? sizeof(float) : sizeof(double)));
       rt.rr[r].rb = (uint32_t)X.i0 * rowB;
       rt.rr[r].dxb = (uint32_t)(X.i1 - X.i0) * rowB;
@@ -611,7 +607,7 @@ __device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int
       for (int r = q; r < v1 - v0; r += nq) {
         if (!rt.ok[r]) continue;
         float Iz, Ix;
-        if (!csr_point_mixed<kTI, kRecent>(H, A, t, col, rt.xp[r], rt.i0[r], rt.i1[r],
+        if (!csr_point_mixed<kTI, kCells>(H, A, t, col, rt.xp[r], rt.i0[r], rt.i1[r],
                                            rt.rr[r], (float)rt.frac[r], Iz, Ix))
           continue;
         const float wx = (float)rt.w[r];
@@ -627,7 +623,7 @@ __device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int
         if (!rt.ok[r]) continue;
         const TruncIdx X = {rt.i0[r], rt.i1[r], rt.frac[r], true};
         double Iz, Ix;
-        if (!csr_point<true, kRecent>(H, A, o, col, rt.xp[r], Iz, Ix, &X, &rt.rr[r])) continue;
+        if (!csr_point<true, kCells>(H, A, o, col, rt.xp[r], Iz, Ix, &X, &rt.rr[r])) continue;
         const double wx = rt.w[r];
         cz += wx * Iz;
         cx += wx * Ix;
@@ -642,9 +638,9 @@ __device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int
 
 // Workgroup l of a task's far (kNear = false) or near launch -> (mesh
 // point, phase); writes the phase's partial sums.  kTI (mixed only): the
-// phases read the time-innermost FP32 ring, else the chunked one; kRecent:
-// the latest planes come from the cell-major copy.
-template <bool kMixed, bool kNear, bool kTI, bool kRecent = false>
+// phases read the time-innermost FP32 ring, else the chunked one; kCells:
+// the cell-major ring.
+template <bool kMixed, bool kNear, bool kTI, bool kCells = false>
 __device__ inline void run_phase_block(const WakeTask &T, int l, double *__restrict__ partials,
                                        double *red) {
   const int xb = T.P.mesh_xbins, zb = T.P.mesh_zbins;
@@ -659,7 +655,7 @@ __device__ inline void run_phase_block(const WakeTask &T, int l, double *__restr
   ph = kNear ? T.nA + q : q;
   __shared__ RowTable rt;
   double sz, sx;
-  wake_phase<kMixed, kTI, kRecent>(T, partials + T.obs + (int64_t)i * kObsStride, ph, red, rt, sz,
+  wake_phase<kMixed, kTI, kCells>(T, partials + T.obs + (int64_t)i * kObsStride, ph, red, rt, sz,
                                    sx);
   if (threadIdx.x == 0) {
     double *p = partials + T.part + ((int64_t)i * (T.nA + T.nB) + ph) * 2;
@@ -675,82 +671,15 @@ __host__ __device__ inline bool mixed_reads_ti(const DfcsrHistory &H, bool near)
   return near ? (H.data32 == nullptr) : (H.data32t != nullptr);
 }
 
-// The near phases with the recent-plane copy hold the copy's loads as well
-// as the ring's: capped at 3 workgroups per CU, the others at 2.
-template <bool kMixed, bool kNear, bool kTI, bool kRecent = false>
-__global__ __launch_bounds__(kThreads, kRecent ? 3 : DFCSR_WAKE_MINBLOCKS) void wake_phase_kernel(
+// Workgroups per CU the register allocation must allow: the near phases on
+// the cell-major ring 3 (FP64) or 4 (mixed: 128 registers), the others 2.
+template <bool kMixed, bool kNear, bool kTI, bool kCells = false>
+__global__ __launch_bounds__(kThreads, kCells ? (kMixed ? 4 : 3) : DFCSR_WAKE_MINBLOCKS) void wake_phase_kernel(
     WakeTask T, double *__restrict__ partials) {
   __shared__ double red[kThreads / 64];
   const int b = blockIdx.x, nb = gridDim.x;
-  run_phase_block<kMixed, kNear, kTI, kRecent>(
+  run_phase_block<kMixed, kNear, kTI, kCells>(
       T, kNear ? xcd_group(b, nb, kNearGroup) : xcd_remap(b, nb), partials, red);
-}
-
-// Copies the latest K = min(recent_planes, nt) planes of the FP64 ring into
-// H.recent, cell-major: cell (x, z) of plane k at ((k Xi + x) Zi + z) * 6 =
-// the five fields at corner (x, z) of its record, then a zero pad.
-__device__ inline void recent_copy(const DfcsrHistory &H, int64_t c, int64_t stride) {
-  const int K = min(H.recent_planes, H.nt), c0 = H.nt - K;
-  const int64_t cells = (int64_t)H.Xi * H.Zi;
-  for (; c < cells; c += stride) {
-    const int xi = (int)(c / H.Zi), zi = (int)(c - (int64_t)xi * H.Zi);
-    for (int k = 0; k < K; ++k) {
-      const double *rec = hist_rec(H, c0 + k, xi, zi);
-      double2 *o = reinterpret_cast<double2 *>(H.recent + ((int64_t)k * cells + c) * DFCSR_RECENT_CELL);
-      o[0] = make_double2(rec[0], rec[4]);
-      o[1] = make_double2(rec[8], rec[12]);
-      o[2] = make_double2(rec[16], 0.0);
-    }
-  }
-}
-
-// The same for mixed-precision wakes: float32 cells (five fields, a pad)
-// from the time-innermost float32 ring, else from the chunked one.
-__device__ inline void recent_copy32(const DfcsrHistory &H, int64_t c, int64_t stride) {
-  const int K = min(H.recent_planes, H.nt), c0 = H.nt - K;
-  const int64_t cells = (int64_t)H.Xi * H.Zi;
-  float *R32 = reinterpret_cast<float *>(H.recent);
-  for (; c < cells; c += stride) {
-    const int xi = (int)(c / H.Zi), zi = (int)(c - (int64_t)xi * H.Zi);
-    for (int k = 0; k < K; ++k) {
-      float v[5];
-      if (H.data32t) {
-        const float4 *r = hist_rec32t(H, c0 + k, xi, zi);
-#pragma unroll
-        for (int f = 0; f < 5; ++f) v[f] = r[f].x;
-      } else {
-        const float4 *r = hist_rec32(H, c0 + k, xi, zi);
-#pragma unroll
-        for (int f = 0; f < 5; ++f) v[f] = r[(int64_t)f * H.Zi].x;
-      }
-      float2 *o = reinterpret_cast<float2 *>(R32 + ((int64_t)k * cells + c) * DFCSR_RECENT_CELL);
-      o[0] = make_float2(v[0], v[1]);
-      o[1] = make_float2(v[2], v[3]);
-      o[2] = make_float2(v[4], 0.0f);
-    }
-  }
-}
-
-template <bool kMixed>
-__global__ __launch_bounds__(kThreads) void recent_copy_kernel(DfcsrHistory H) {
-  const int64_t c = (int64_t)blockIdx.x * kThreads + threadIdx.x, st = (int64_t)gridDim.x * kThreads;
-  if (kMixed)
-    recent_copy32(H, c, st);
-  else
-    recent_copy(H, c, st);
-}
-
-// blockIdx.y = task; tasks without a copy buffer return at once.
-template <bool kMixed>
-__global__ __launch_bounds__(kThreads) void recent_copy_batched_kernel(
-    const WakeTask *__restrict__ tasks) {
-  const DfcsrHistory &H = tasks[blockIdx.y].H;
-  if (!H.recent) return;
-  const int64_t c = (int64_t)blockIdx.x * kThreads + threadIdx.x, st = (int64_t)gridDim.x * kThreads;
-  if (kMixed)
-    recent_copy32(H, c, st);
-  else
-    recent_copy(H, c, st);
 }
 
 // Last task whose first index (workgroup or observer) is <= b.
@@ -775,19 +704,14 @@ __global__ __launch_bounds__(kThreads, DFCSR_WAKE_MINBLOCKS) void wake_phase_bat
                           [](const WakeTask &T) { return kNear ? T.blockB0 : T.blockA0; });
   const WakeTask &T = tasks[j];
   const int l = b - (kNear ? T.blockB0 : T.blockA0);
-  if constexpr (kMixed) {   // per task: its FP32 rings, a recent-plane copy or not
-    const bool ti = mixed_reads_ti(T.H, kNear);
-    if (kNear && T.H.recent) {
-      if (ti)
-        run_phase_block<true, kNear, true, true>(T, l, partials, red);
-      else
-        run_phase_block<true, kNear, false, true>(T, l, partials, red);
-    } else if (ti) {
+  if constexpr (kMixed) {   // per task: its FP32 rings, cell-major or not
+    if (kNear && T.H.cells32)
+      run_phase_block<true, kNear, false, true>(T, l, partials, red);
+    else if (mixed_reads_ti(T.H, kNear))
       run_phase_block<true, kNear, true>(T, l, partials, red);
-    } else {
+    else
       run_phase_block<true, kNear, false>(T, l, partials, red);
-    }
-  } else if (kNear && T.H.recent) {   // per task: a recent-plane copy or not
+  } else if (kNear && T.H.cells) {   // per task: the cell-major ring or not
     run_phase_block<false, kNear, false, true>(T, l, partials, red);
   } else {
     run_phase_block<false, kNear, false>(T, l, partials, red);
@@ -860,14 +784,13 @@ int check_params(const DfcsrWakeParams *p, const char *who) {
 int check_rings(const DfcsrHistory &H, int precision, const char *who) {
   if (precision == DFCSR_PREC_FP64 && !H.data)
     return set_error(DFCSR_EINVAL, "%s: FP64 precision needs the FP64 history ring", who);
-  if (H.recent && H.recent_planes < 1)
-    return set_error(DFCSR_EINVAL, "%s: recent-plane buffer with %d planes", who,
-                     H.recent_planes);
-  // the wake addresses the copy with 32-bit byte offsets
-  const int64_t cellB = DFCSR_RECENT_CELL * (precision == DFCSR_PREC_MIXED ? 4 : 8);
-  if (H.recent && (int64_t)H.recent_planes * H.Xi * H.Zi * cellB > ((int64_t)1 << 32))
-    return set_error(DFCSR_EINVAL, "%s: recent-plane copy of %d planes of %d x %d cells "
-                     "exceeds 4 GiB", who, H.recent_planes, H.Xi, H.Zi);
+  // the wake addresses a cell within a plane of a cell-major ring with 32-bit
+  // byte offsets
+  const bool cells = precision == DFCSR_PREC_MIXED ? H.cells32 != nullptr : H.cells != nullptr;
+  const int64_t cellB = DFCSR_CELL * (precision == DFCSR_PREC_MIXED ? 4 : 8);
+  if (cells && (int64_t)H.Xi * H.Zi * cellB > ((int64_t)1 << 32))
+    return set_error(DFCSR_EINVAL, "%s: a cell-major plane of %d x %d cells exceeds 4 GiB", who,
+                     H.Xi, H.Zi);
   if (precision == DFCSR_PREC_MIXED && !H.data32 && !H.data32t)
     return set_error(DFCSR_EINVAL, "%s: mixed precision needs an FP32 history ring", who);
   return DFCSR_OK;
@@ -952,17 +875,10 @@ extern "C" int dfcsr_wake_mesh(const DfcsrHistory *hist, const DfcsrLattice *lat
       hipLaunchKernelGGL((wake_phase_kernel<true, false, true>), ga, dim3(kThreads), 0, s, T, part);
     else
       hipLaunchKernelGGL((wake_phase_kernel<true, false, false>), ga, dim3(kThreads), 0, s, T, part);
-    const int64_t cells = (int64_t)hist->Xi * hist->Zi;
-    const int gc = (int)std::min<int64_t>((cells + kThreads - 1) / kThreads, 4096);
     const bool ti = mixed_reads_ti(*hist, true);
-    if (hist->recent) {
-      hipLaunchKernelGGL(recent_copy_kernel<true>, dim3(gc), dim3(kThreads), 0, s, *hist);
-      if (ti)
-        hipLaunchKernelGGL((wake_phase_kernel<true, true, true, true>), gb, dim3(kThreads), 0, s,
-                           T, part);
-      else
-        hipLaunchKernelGGL((wake_phase_kernel<true, true, false, true>), gb, dim3(kThreads), 0, s,
-                           T, part);
+    if (hist->cells32) {
+      hipLaunchKernelGGL((wake_phase_kernel<true, true, false, true>), gb, dim3(kThreads), 0, s,
+                         T, part);
     } else if (ti) {
       hipLaunchKernelGGL((wake_phase_kernel<true, true, true>), gb, dim3(kThreads), 0, s, T, part);
     } else {
@@ -970,10 +886,7 @@ extern "C" int dfcsr_wake_mesh(const DfcsrHistory *hist, const DfcsrLattice *lat
     }
   } else {
     hipLaunchKernelGGL((wake_phase_kernel<false, false, false>), ga, dim3(kThreads), 0, s, T, part);
-    if (hist->recent) {
-      const int64_t cells = (int64_t)hist->Xi * hist->Zi;
-      const int gc = (int)std::min<int64_t>((cells + kThreads - 1) / kThreads, 4096);
-      hipLaunchKernelGGL(recent_copy_kernel<false>, dim3(gc), dim3(kThreads), 0, s, *hist);
+    if (hist->cells) {
       hipLaunchKernelGGL((wake_phase_kernel<false, true, false, true>), gb, dim3(kThreads), 0, s, T,
                          part);
     } else {
@@ -1061,11 +974,9 @@ extern "C" int dfcsr_wake_batched_launch(const void *dev_tasks, const DfcsrWakeB
                      batch->observers, part);
   if (precision == DFCSR_PREC_MIXED) {
     hipLaunchKernelGGL((wake_phase_batched_kernel<true, false>), ga, dim3(kThreads), 0, s, dt, k, part);
-    hipLaunchKernelGGL(recent_copy_batched_kernel<true>, dim3(512, k), dim3(kThreads), 0, s, dt);
     hipLaunchKernelGGL((wake_phase_batched_kernel<true, true>), gb, dim3(kThreads), 0, s, dt, k, part);
   } else if (precision == DFCSR_PREC_FP64) {
     hipLaunchKernelGGL((wake_phase_batched_kernel<false, false>), ga, dim3(kThreads), 0, s, dt, k, part);
-    hipLaunchKernelGGL(recent_copy_batched_kernel<false>, dim3(512, k), dim3(kThreads), 0, s, dt);
     hipLaunchKernelGGL((wake_phase_batched_kernel<false, true>), gb, dim3(kThreads), 0, s, dt, k, part);
   } else {
     return set_error(DFCSR_EINVAL, "wake_batched_launch: unknown precision %d", precision);

@@ -11,17 +11,18 @@ whole run:
   raw    [cap][5][nbmax*nbmax] float64  fields on the slice's own grid
   hist   [Xi][Zi][cap][20]     float64  re-gridded corner-packed records
   hist32t[Xi][Zi][cap][20]     float32  the same records (mixed precision)
-  hist32 [Xi][cap][5][Zi][4]   float32  the same, chunked (mixed precision)
+  hist32 [Xi][cap][5][Zi][4]   float32  the same, chunked (optional)
+  cells  [cap][Xi][Zi][6]      float64  cell-major: the five fields at (x, z)
+  cells32[cap][Xi][Zi][6]      float32  the same in float32
 (record (x, z) holds every field at (x, z), (x, z+1), (x+1, z), (x+1, z+1),
 clamped to the last plane: one record per time plane per lookup.  hist and
 hist32t keep the slot innermost (the two planes of a lookup are neighbours:
-what the far region's lookups want); hist32 stores each field's four corners
-as one 16-byte chunk, z fastest, so neighbouring z of one plane are
-neighbours: what the near regions' lookups want.  keep_fp64 / keep_fp32
-choose the float64 ring and the two float32 rings.)  The wakes also get a
-scratch buffer, recent[K][Xi][Zi][6] (float64, or float32 without the float64
-ring), that each wake fills with a cell-major copy of the latest K planes for
-its near-region lookups.
+what the far region's lookups want); the cell-major rings keep a plane's
+cells together, neighbouring z in neighbouring cells: what the near
+regions' lookups want.  hist32 stores each field's four corners as one
+16-byte chunk, z fastest; it serves the near regions when the cell-major
+rings are not kept.  keep_fp64 chooses hist (and cells), keep_fp32 hist32t
+(and cells32), keep_cells the cell-major rings, keep_chunked hist32.)
 Appending writes slot (head + nt) % cap; popping advances head.  Nothing is
 copied per step: the reference's np.array(deque) stacking of the whole
 history (deposit.py:422-426) becomes a descriptor update.
@@ -58,12 +59,12 @@ class SliceMeta:
 
 class DF_tracker:
     def __init__(self, input_dic=None, device="cuda", keep_fp32=True, capacity=16,
-                 keep_fp64=True, recent_planes=16):
-        """keep_fp64: hold the float64 history ring (FP64 wakes, lookups);
-        keep_fp32: hold the two float32 rings (mixed-precision wakes);
-        recent_planes: planes allocated for the wakes' cell-major copy of the
-        latest slices (0: none); ``recent_use`` sets how many the next wake
-        copies (default: all)."""
+                 keep_fp64=True, keep_cells=True, keep_chunked=None):
+        """keep_fp64: hold the float64 history rings (FP64 wakes, lookups);
+        keep_fp32: hold the float32 rings (mixed-precision wakes);
+        keep_cells: with them the cell-major rings the wakes' near regions
+        read; keep_chunked: the chunked float32 ring (default: when the
+        cell-major rings are not kept)."""
         self.params = Deposition_params(input_dic)
         p = self.params
         self.xbins, self.zbins, self.xlim, self.zlim = p.xbins, p.zbins, p.xlim, p.zlim
@@ -72,9 +73,8 @@ class DF_tracker:
         self.device = torch.device(device)
         self.keep_fp32 = keep_fp32
         self.keep_fp64 = keep_fp64
-        self.recent_planes = int(recent_planes)
-        self.recent_use = self.recent_planes
-        self.recent = None    # (K, Xi, Zi, 6) scratch of the wakes
+        self.keep_cells = bool(keep_cells)
+        self.keep_chunked = (not self.keep_cells) if keep_chunked is None else bool(keep_chunked)
         if not (keep_fp32 or keep_fp64):
             raise DfcsrError("DF_tracker needs the float64 or the float32 history rings")
         self.ws = _lib.Workspace(self.device)
@@ -92,6 +92,8 @@ class DF_tracker:
         self.hist = None      # float64 (Xi, Zi, cap, 20)
         self.hist32 = None    # float32 chunked (Xi, cap, 5, Zi, 4)
         self.hist32t = None   # float32 time-innermost (Xi, Zi, cap, 20)
+        self.cells = None     # float64 cell-major (cap, Xi, Zi, 6)
+        self.cells32 = None   # float32 cell-major (cap, Xi, Zi, 6)
         self.cur = None
         self.n_regrid = 0
         self.info = torch.zeros(4, dtype=torch.float64, device=self.device)
@@ -113,10 +115,10 @@ class DF_tracker:
             return
         new_cap = max(need, 2 * self.cap, 16)
         raw = torch.empty((new_cap, self._raw_stride()), dtype=torch.float64, device=self.device)
-        rings = (None, None, None)
+        rings = (None,) * 5
         if self.Xi:
             rings = self._rings(self.Xi, self.Zi, new_cap)
-        hist, hist32, hist32t = rings
+        hist, hist32, hist32t, cells, cells32 = rings
         if self.cap:
             # keep the ring's logical order, pending slot included (index nt)
             for i in range(min(self.nt + 1, self.cap)):
@@ -128,35 +130,39 @@ class DF_tracker:
                     hist32t[:, :, i].copy_(self.hist32t[:, :, s])
                 if hist32 is not None and self.hist32 is not None:
                     hist32[:, i].copy_(self.hist32[:, s])
+                for new, old in ((cells, self.cells), (cells32, self.cells32)):
+                    if new is not None and old is not None:
+                        new[i].copy_(old[s])
         self.raw, self.hist, self.hist32, self.hist32t = raw, hist, hist32, hist32t
+        self.cells, self.cells32 = cells, cells32
         self.cap = new_cap
         self.head = 0
 
     def _rings(self, Xi, Zi, cap):
-        hist = hist32 = hist32t = None
+        """(hist, hist32, hist32t, cells, cells32) for an Xi x Zi grid.  The
+        wakes address cells within a plane with 32-bit offsets: a cell-major
+        ring whose plane would pass 4 GiB is not kept (the rings serve)."""
+        hist = hist32 = hist32t = cells = cells32 = None
+        dev = self.device
         if self.keep_fp64:
-            hist = torch.empty((Xi, Zi, cap, _lib.HIST_REC64), dtype=torch.float64,
-                               device=self.device)
+            hist = torch.empty((Xi, Zi, cap, _lib.HIST_REC64), dtype=torch.float64, device=dev)
+            if self.keep_cells and Xi * Zi * _lib.CELL * 8 <= (1 << 32):
+                cells = torch.empty((cap, Xi, Zi, _lib.CELL), dtype=torch.float64, device=dev)
         if self.keep_fp32:
-            hist32 = torch.empty((Xi, cap, _lib.HIST_CHUNKS32, Zi, 4), dtype=torch.float32,
-                                 device=self.device)
-            hist32t = torch.empty((Xi, Zi, cap, _lib.HIST_REC32), dtype=torch.float32,
-                                  device=self.device)
-        return hist, hist32, hist32t
+            hist32t = torch.empty((Xi, Zi, cap, _lib.HIST_REC32), dtype=torch.float32, device=dev)
+            if self.keep_cells and Xi * Zi * _lib.CELL * 4 <= (1 << 32):
+                cells32 = torch.empty((cap, Xi, Zi, _lib.CELL), dtype=torch.float32, device=dev)
+            if self.keep_chunked:
+                hist32 = torch.empty((Xi, cap, _lib.HIST_CHUNKS32, Zi, 4), dtype=torch.float32,
+                                     device=dev)
+        return hist, hist32, hist32t, cells, cells32
 
     def _alloc_hist(self, Xi, Zi):
-        if self.Xi != Xi or self.Zi != Zi or (self.hist is None and self.hist32 is None):
-            self.hist = self.hist32 = self.hist32t = self.recent = None   # free first
-            self.hist, self.hist32, self.hist32t = self._rings(Xi, Zi, self.cap)
-            # float64 cells when FP64 wakes may run, else float32 cells; the
-            # wakes address the copy with 32-bit byte offsets, so it holds at
-            # most as many planes as fit in 4 GiB
-            dt = torch.float64 if self.keep_fp64 else torch.float32
-            plane = Xi * Zi * _lib.RECENT_CELL * (8 if self.keep_fp64 else 4)
-            k = min(self.recent_planes, (1 << 32) // plane)
-            if k > 0:
-                self.recent = torch.empty((k, Xi, Zi, _lib.RECENT_CELL), dtype=dt,
-                                          device=self.device)
+        if self.Xi != Xi or self.Zi != Zi or (self.hist is None and self.hist32t is None):
+            # free first
+            self.hist = self.hist32 = self.hist32t = self.cells = self.cells32 = None
+            (self.hist, self.hist32, self.hist32t, self.cells,
+             self.cells32) = self._rings(Xi, Zi, self.cap)
         self.Xi, self.Zi = Xi, Zi
 
     def raw_fields(self, i=None):
@@ -255,7 +261,8 @@ class DF_tracker:
         for k in range(0, len(jobs), 4096):
             kernels.regrid(self.raw, self._raw_stride(), jobs[k:k + 4096], self.hist, self.hist32,
                            self.Xi, self.Zi, self.xi[0], self.xi[1], self.zi[0], self.zi[1],
-                           self.ws, hist32t=self.hist32t)
+                           self.ws, hist32t=self.hist32t, cells=self.cells,
+                           cells32=self.cells32)
 
     def build_interpolant(self):   # deposit.py:395-426
         nt = self.nt
@@ -271,10 +278,8 @@ class DF_tracker:
         """ctypes history descriptor for the kernels (after build_interpolant)."""
         def p(t):
             return t.data_ptr() if t is not None else None
-        return _lib.History(p(self.hist), p(self.hist32), p(self.hist32t), p(self.recent),
-                            self.cap, self.head, self.nt, self.Xi, self.Zi,
-                            max(1, min(int(self.recent_use), self.recent.shape[0]))
-                            if self.recent is not None else 0,
+        return _lib.History(p(self.hist), p(self.hist32), p(self.hist32t), p(self.cells),
+                            p(self.cells32), self.cap, self.head, self.nt, self.Xi, self.Zi,
                             float(self.min_x), float(self.min_y), float(self.min_z),
                             float(self.delta_x), float(self.delta_y), float(self.delta_z))
 
@@ -290,9 +295,7 @@ class DF_tracker:
         b = self.raw.numel() * 8 if self.raw is not None else 0
         if self.hist is not None:
             b += self.hist.numel() * 8
-        for r in (self.hist32, self.hist32t):
+        for r in (self.hist32, self.hist32t, self.cells, self.cells32):
             if r is not None:
-                b += r.numel() * 4
-        if self.recent is not None:
-            b += self.recent.numel() * self.recent.element_size()
+                b += r.numel() * r.element_size()
         return b

@@ -178,10 +178,11 @@ def savgol_operator(window, order):
 
 
 def regrid(src: torch.Tensor, slot_stride: int, jobs, hist64, hist32, Xi, Zi, xi0, xi1, zi0, zi1,
-           ws: _lib.Workspace, hist32t=None):
+           ws: _lib.Workspace, hist32t=None, cells=None, cells32=None):
     """jobs: list of (src_slot, dst_slot, nbx, nbz, x0, x1, z0, z1).  Rings (any
     may be None, not all): hist64 float64 (Xi, Zi, cap, 20), hist32 float32
-    chunked (Xi, cap, 5, Zi, 4), hist32t float32 time-innermost (Xi, Zi, cap, 20)."""
+    chunked (Xi, cap, 5, Zi, 4), hist32t float32 time-innermost (Xi, Zi, cap, 20),
+    cells / cells32 cell-major float64 / float32 (cap, Xi, Zi, 6)."""
     lib = _lib.load()
     n = len(jobs)
     if n == 0:
@@ -193,12 +194,16 @@ def regrid(src: torch.Tensor, slot_stride: int, jobs, hist64, hist32, Xi, Zi, xi
         cap = int(rings[0].shape[2])
     elif hist32 is not None:
         cap = int(hist32.shape[1])
+    elif cells is not None or cells32 is not None:
+        cap = int((cells if cells is not None else cells32).shape[0])
     else:
         raise DfcsrError("regrid: no history ring")
     want = {id(hist64): ((Xi, Zi, cap, _lib.HIST_REC64), torch.float64),
             id(hist32t): ((Xi, Zi, cap, _lib.HIST_REC32), torch.float32),
-            id(hist32): ((Xi, cap, _lib.HIST_CHUNKS32, Zi, 4), torch.float32)}
-    for r in (hist64, hist32, hist32t):
+            id(hist32): ((Xi, cap, _lib.HIST_CHUNKS32, Zi, 4), torch.float32),
+            id(cells): ((cap, Xi, Zi, _lib.CELL), torch.float64),
+            id(cells32): ((cap, Xi, Zi, _lib.CELL), torch.float32)}
+    for r in (hist64, hist32, hist32t, cells, cells32):
         if r is None:
             continue
         shape, dtype = want[id(r)]
@@ -206,8 +211,9 @@ def regrid(src: torch.Tensor, slot_stride: int, jobs, hist64, hist32, Xi, Zi, xi
             raise DfcsrError(f"regrid: history ring {tuple(r.shape)} {r.dtype} does not match "
                              f"{shape} {dtype}")
     check(lib.dfcsr_regrid(ptr(src), int(slot_stride), arr, n, ptr(hist64), ptr(hist32),
-                           ptr(hist32t), int(Xi), int(Zi), cap, float(xi0), float(xi1), float(zi0),
-                           float(zi1), ptr(b), b.numel(), stream_handle(src.device)), "regrid")
+                           ptr(hist32t), ptr(cells), ptr(cells32), int(Xi), int(Zi), cap,
+                           float(xi0), float(xi1), float(zi0), float(zi1), ptr(b), b.numel(),
+                           stream_handle(src.device)), "regrid")
 
 
 def interpolate3D(hist: _lib.History, field: int, q: torch.Tensor, device):

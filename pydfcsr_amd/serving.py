@@ -65,12 +65,10 @@ def request_footprint(config: dict, steps: int, precision: str = "mixed") -> int
     ul = dep.get("upper_limit")
     grid = int(ul) if isinstance(ul, int) else 2000
     slots = steps + 2
-    # corner-packed records: 160 B in the float64 ring, or 80 B in each of the
-    # two float32 rings a mixed-precision simulation keeps instead
-    rec = 160
+    # per slot and grid node: the corner-packed record (160 B in float64, 80 B
+    # in the time-innermost float32 ring) and the cell-major cell (48 B / 24 B)
+    rec = (80 + 4 * _lib.CELL) if precision == "mixed" else (160 + 8 * _lib.CELL)
     hist = slots * grid * grid * rec
-    # the wake's cell-major copy of the latest 16 planes (float64 or float32 cells)
-    hist += 16 * grid * grid * (4 if precision == "mixed" else 8) * _lib.RECENT_CELL
     raw = slots * 5 * nb * nb * 8
     particles = n * 8 * (6 * 2 + 4)           # coordinates, tracking temporary, scratch
     deposit = 4 * nb * nb * 8 * 2 + 3 * nb * nb * 8

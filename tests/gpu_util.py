@@ -49,15 +49,27 @@ def device_ring(records, cap, head, width=4):
     return ring
 
 
+def cell_ring(data, cap, head, dtype):
+    """(nt, Xi, Zi, 5) field stacks -> a cell-major ring (cap, Xi, Zi, 6): the
+    five fields of node (x, z) then a pad, logical slice i in slot
+    (head + i) % cap; slots holding no slice are NaN."""
+    nt, Xi, Zi, _ = data.shape
+    ring = np.full((cap, Xi, Zi, _lib.CELL), np.nan, dtype=dtype)
+    for i in range(nt):
+        ring[(head + i) % cap, :, :, :5] = data[i]
+        ring[(head + i) % cap, :, :, 5] = 0
+    return ring
+
+
 class DeviceHistory:
     """A device ring holding an oracle DFTracker's stacked history.
 
     wrap=k places logical slice 0 at slot cap-k so the ring wraps around.
     rings: which of the float64, float32 chunked and float32 time-innermost
-    rings to build."""
+    rings and of the float64 / float32 cell-major rings ("cells",
+    "cells32") to build."""
 
-    def __init__(self, tracker, device="cuda", wrap=0, rings=("fp64", "fp32", "fp32t"),
-                 recent_planes=0):
+    def __init__(self, tracker, device="cuda", wrap=0, rings=("fp64", "fp32", "fp32t")):
         data = np.stack([tracker.data[f] for f in FIELDS], axis=-1)   # (nt, Xi, Zi, 5)
         nt, Xi, Zi, _ = data.shape
         cap = nt + (wrap + 1 if wrap else 0)
@@ -71,22 +83,20 @@ class DeviceHistory:
                                                     4)).to(device)
         if "fp32t" in rings:
             self.h32t = torch.from_numpy(ring64(packed.astype(np.float32), cap, head)).to(device)
-        # the wakes' recent-plane scratch (float64-sized), every byte 0xff: NaN
-        # read as float64 and as float32, so a cell the wake reads without
-        # writing it first shows up in the result
-        self.recent = None
-        if recent_planes:
-            n = recent_planes * Xi * Zi * _lib.RECENT_CELL
-            self.recent = torch.full((n * 8,), 255, dtype=torch.uint8, device=device) \
-                .view(torch.float64).view(recent_planes, Xi, Zi, _lib.RECENT_CELL)
+        self.cells = self.cells32 = None
+        if "cells" in rings:
+            self.cells = torch.from_numpy(cell_ring(data, cap, head, np.float64)).to(device)
+        if "cells32" in rings:
+            self.cells32 = torch.from_numpy(cell_ring(data.astype(np.float32), cap, head,
+                                                      np.float32)).to(device)
         a = tracker.axes()
 
         def p(t):
             return t.data_ptr() if t is not None else None
-        self.desc = _lib.History(p(self.h64), p(self.h32), p(self.h32t), p(self.recent), cap, head,
-                                 nt, Xi, Zi, recent_planes, float(a["min_t"]), float(a["min_x"]),
-                                 float(a["min_z"]), float(a["dt"]), float(a["dx"]),
-                                 float(a["dz"]))
+        self.desc = _lib.History(p(self.h64), p(self.h32), p(self.h32t), p(self.cells),
+                                 p(self.cells32), cap, head, nt, Xi, Zi, float(a["min_t"]),
+                                 float(a["min_x"]), float(a["min_z"]), float(a["dt"]),
+                                 float(a["dx"]), float(a["dz"]))
 
 
 class DeviceLattice:

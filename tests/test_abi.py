@@ -27,7 +27,7 @@ def test_library_exports_header_symbols():
     for s in syms:
         assert hasattr(lib, s), s
     assert set(syms) <= set(_lib.EXPORTED) | {"dfcsr_savgol_operator"}
-    assert lib.dfcsr_abi_version() == _lib.ABI_VERSION == 3
+    assert lib.dfcsr_abi_version() == _lib.ABI_VERSION == 4
 
 
 def test_argument_errors_without_device():
@@ -40,14 +40,14 @@ def test_argument_errors_without_device():
     with pytest.raises(_lib.DfcsrError, match="workspace"):
         _lib.check(lib.dfcsr_df_process(1, 1, 1, 1, 1, 8, 0.0, 1.0, 8, 0.0, 1.0, 5.0, 5, 2, None,
                                         1, 8, None), "df_process")
-    H = _lib.History(1, None, None, None, 4, 0, 1, 8, 8, 0, 0, 0, 0, 1, 1, 1)
+    H = _lib.History(1, None, None, None, None, 4, 0, 1, 8, 8, 0, 0, 0, 1, 1, 1)
     rc = lib.dfcsr_csr_integrand(H, _lib.Lattice(), 0.0, 0.0, 0.0, 1, 1, 1, 1, 1, 1, None)
     assert rc == -1 and b"slice" in lib.dfcsr_last_error()
     # the integrand reads the float64 ring; a history needs at least one ring
-    H = _lib.History(None, 2, 3, None, 4, 0, 2, 8, 8, 0, 0, 0, 0, 1, 1, 1)
+    H = _lib.History(None, 2, 3, None, None, 4, 0, 2, 8, 8, 0, 0, 0, 1, 1, 1)
     rc = lib.dfcsr_csr_integrand(H, _lib.Lattice(), 0.0, 0.0, 0.0, 1, 1, 1, 1, 1, 1, None)
     assert rc == -1 and b"FP64 ring" in lib.dfcsr_last_error()
-    H = _lib.History(None, None, None, None, 4, 0, 2, 8, 8, 0, 0, 0, 0, 1, 1, 1)
+    H = _lib.History(None, None, None, None, None, 4, 0, 2, 8, 8, 0, 0, 0, 1, 1, 1)
     rc = lib.dfcsr_csr_integrand(H, _lib.Lattice(), 0.0, 0.0, 0.0, 1, 1, 1, 1, 1, 1, None)
     assert rc == -1 and b"without a ring" in lib.dfcsr_last_error()
 
@@ -104,7 +104,7 @@ def test_wake_tasks_pack_host_logic():
     import ctypes
     lib = _lib.load()
     k = 3
-    hist = (_lib.History * k)(*[_lib.History(1, 2, 3, None, 8, 0, 5, 16, 16, 0, 0, 0, 0, 1, 1, 1)
+    hist = (_lib.History * k)(*[_lib.History(1, 2, 3, None, None, 8, 0, 5, 16, 16, 0, 0, 0, 1, 1, 1)
                                  for _ in range(k)])
     lat = (_lib.Lattice * k)(*[_lib.Lattice(3, 100, 1, 0.0, 0.01, 4, 5) for _ in range(k)])
     P = (_lib.WakeParams * k)(_params(-1.25, 8), _params(0.5, 16), _params(-1.25, 4))
@@ -145,25 +145,22 @@ def test_wake_tasks_pack_host_logic():
                                    ctypes.byref(batch))
     assert rc == -1 and b"FP64 precision" in lib.dfcsr_last_error()
     hist[2].data = 1
-    # a recent-plane buffer needs a plane count
-    hist[0].recent = 7
-    rc = lib.dfcsr_wake_tasks_pack(k, hist, lat, P, _lib.DFCSR_PREC_FP64, out, out, tbl, nb,
-                                   ctypes.byref(batch))
-    assert rc == -1 and b"recent-plane" in lib.dfcsr_last_error()
-    hist[0].recent_planes = 2
-    rc = lib.dfcsr_wake_tasks_pack(k, hist, lat, P, _lib.DFCSR_PREC_FP64, out, out, tbl, nb,
-                                   ctypes.byref(batch))
-    assert rc == 0, lib.dfcsr_last_error()
-    # the wake addresses the copy with 32-bit byte offsets: at most 4 GiB
-    # (16 planes of 3000 x 3000 float32 cells fit, float64 cells do not)
-    hist[0].recent_planes, hist[0].Xi, hist[0].Zi = 16, 3000, 3000
+    # the cell-major rings are optional
+    hist[0].cells, hist[0].cells32 = 7, 8
+    for prec in (_lib.DFCSR_PREC_FP64, _lib.DFCSR_PREC_MIXED):
+        rc = lib.dfcsr_wake_tasks_pack(k, hist, lat, P, prec, out, out, tbl, nb,
+                                       ctypes.byref(batch))
+        assert rc == 0, lib.dfcsr_last_error()
+    # the wake addresses a cell within a plane with 32-bit byte offsets: a
+    # plane of 10000 x 10000 float32 cells (2.4 GB) fits, float64 (4.8 GB) not
+    hist[0].Xi = hist[0].Zi = 10000
     rc = lib.dfcsr_wake_tasks_pack(k, hist, lat, P, _lib.DFCSR_PREC_MIXED, out, out, tbl, nb,
                                    ctypes.byref(batch))
     assert rc == 0, lib.dfcsr_last_error()
     rc = lib.dfcsr_wake_tasks_pack(k, hist, lat, P, _lib.DFCSR_PREC_FP64, out, out, tbl, nb,
                                    ctypes.byref(batch))
     assert rc == -1 and b"exceeds 4 GiB" in lib.dfcsr_last_error()
-    hist[0].recent_planes, hist[0].Xi, hist[0].Zi = 2, 16, 16
+    hist[0].Xi = hist[0].Zi = 16
     rc = lib.dfcsr_wake_tasks_pack(k, hist, lat, P, _lib.DFCSR_PREC_FP64, out, out, tbl, 16,
                                    ctypes.byref(batch))
     assert rc != 0 and b"too small" in lib.dfcsr_last_error()

@@ -153,7 +153,7 @@ def test_interpolate_golden_bitexact(K, golden_dir):
     stack = np.zeros((nt, nx, nz, 5))
     stack[..., 0] = data
     h = _t(ring64(corner_records(stack), nt, 0))
-    H = _lib.History(h.data_ptr(), None, None, None, nt, 0, nt, nx, nz, 0,
+    H = _lib.History(h.data_ptr(), None, None, None, None, nt, 0, nt, nx, nz,
                      *[float(v) for v in d["mins"]],
                      *[float(v) for v in d["dels"]])
     r = K.interpolate3D(H, 0, _t(d["q3"]), DEV).cpu().numpy()
@@ -249,15 +249,14 @@ def test_wake_golden(K, ws, golden_dir, golden_history):
     _, _, DHw, _, _ = _golden_wake_inputs(golden_dir, golden_history, wrap=3)
     out3 = K.wake_mesh(DHw.desc, DL.desc, P, 0, DEV, ws)
     assert torch.equal(out["dE_dct"], out3["dE_dct"]) and torch.equal(out["x_kick"], out3["x_kick"])
-    # with the recent-plane copy (NaN-filled scratch: every cell read must be
-    # written) of 1, 2, all and more than all planes, wrapped or not: identical
+    # with the cell-major ring (its slots without a slice hold NaN: a read of
+    # the wrong slot shows), wrapped or not: identical
     _, T = golden_history
-    for wrap in (0, 3):
-        for kp in (1, 2, T.data["density"].shape[0], 40):
-            DHr = DeviceHistory(T, DEV, wrap=wrap, recent_planes=kp)
-            o = K.wake_mesh(DHr.desc, DL.desc, P, 0, DEV, ws)
-            assert torch.equal(out["dE_dct"], o["dE_dct"]), (wrap, kp)
-            assert torch.equal(out["x_kick"], o["x_kick"]), (wrap, kp)
+    for wrap in (0, 1, 3):
+        DHc = DeviceHistory(T, DEV, wrap=wrap, rings=("fp64", "fp32", "fp32t", "cells"))
+        o = K.wake_mesh(DHc.desc, DL.desc, P, 0, DEV, ws)
+        assert torch.equal(out["dE_dct"], o["dE_dct"]), wrap
+        assert torch.equal(out["x_kick"], o["x_kick"]), wrap
 
 
 @pytest.mark.parametrize("slope0,key", [(2.5, "chirp_pos"), (-2.5, "chirp_neg")])
@@ -314,9 +313,9 @@ def test_wake_mixed_precision_golden(K, ws, golden_dir, golden_history):
 def test_wake_mixed_precision_ring_layouts(K, ws, golden_dir, golden_history):
     """The mixed form reads either float32 ring layout (chunked, time-innermost,
     or both: region 0 the time-innermost one, the later regions the chunked
-    one), with or without the cell-major copy of the latest planes (NaN-filled
-    scratch: every cell read must be written), with the same arithmetic, so the
-    wakes are bit-identical across layouts; without the float64 ring, vx at the
+    one), with or without the float32 cell-major ring for the later regions
+    (slots without a slice hold NaN), with the same arithmetic, so the wakes
+    are bit-identical across layouts; without the float64 ring, vx at the
     observer comes from a float32 ring, within 1e-4 of max|W| of the
     reference's float64 wakes."""
     w, beam, _, DL, P = _golden_wake_inputs(golden_dir, golden_history)
@@ -324,14 +323,14 @@ def test_wake_mixed_precision_ring_layouts(K, ws, golden_dir, golden_history):
     res = {}
     for rings in (("fp64", "fp32", "fp32t"), ("fp64", "fp32"), ("fp64", "fp32t"),
                   ("fp32", "fp32t"), ("fp32",), ("fp32t",)):
-        for kp in (0, 1, 3, 40):   # with the float32 recent-plane copy of kp planes
-            DH = DeviceHistory(T, DEV, wrap=2, rings=rings, recent_planes=kp)
+        for cells in ((), ("cells32",)):
+            DH = DeviceHistory(T, DEV, wrap=2, rings=rings + cells)
             out = K.wake_mesh(DH.desc, DL.desc, P, 1, DEV, ws)
-            res[rings + (kp,)] = np.stack([out["dE_dct"].cpu().numpy(),
+            res[rings + cells] = np.stack([out["dE_dct"].cpu().numpy(),
                                            out["x_kick"].cpu().numpy()])
-            e_z = rel_max(res[rings + (kp,)][0].reshape(3, 4), w["dE_dct"])
-            e_x = rel_max(res[rings + (kp,)][1].reshape(3, 4), w["x_kick"])
-            assert e_z <= 1e-4 and e_x <= 1e-4, (rings, kp, e_z, e_x)
+            e_z = rel_max(res[rings + cells][0].reshape(3, 4), w["dE_dct"])
+            e_x = rel_max(res[rings + cells][1].reshape(3, 4), w["x_kick"])
+            assert e_z <= 1e-4 and e_x <= 1e-4, (rings, cells, e_z, e_x)
     with64 = [r for r in res if "fp64" in r]
     without = [r for r in res if "fp64" not in r]
     for group in (with64, without):

@@ -60,52 +60,3 @@ def test_reference_tables_closed_form_arc():
     assert np.allclose(coords[:, 0], R * np.sin(th), atol=1e-12)
     assert np.allclose(coords[:, 1], R * (1 - np.cos(th)), atol=1e-12)
 
-
-@pytest.mark.parametrize("slope0", [0.3, -0.8, 2.5, -2.5, 6.0])
-def test_near_planes_covers_the_near_regions_reach(slope0):
-    """CSR2D.near_planes (the depth of the wakes' recent-plane copy) covers
-    every near-region lookup: for mesh observers, the oldest retarded plane
-    their regions 1.. reach (get_CSR_wake's bounds, oracle.wake_regions, and
-    |r - r'| from the lattice tables) lies within the copied planes, unless
-    the copy is capped (then older lookups read the ring, as designed)."""
-    import math
-    from types import SimpleNamespace
-    from oracle import cpu_ref as O
-    from pydfcsr_amd.csr import CSR2D
-    elements = [dict(type="drift", L=0.3, nsep=1), dict(type="dipole", L=0.5, angle=0.05, nsep=1),
-                dict(type="drift", L=1.0, nsep=1)]
-    lat = O.LatticeTables(elements)
-    sz, sx, mean_x, slope1 = 60e-6, 40e-6, 3e-6, -1e-6
-    t, dt, nt, cap = 0.7, 5e-3, 140, 16
-    zr = (-5 * sz, 5 * sz, 9)
-    xr = (-5 * sx, 5 * sx, 9)
-    fake = SimpleNamespace(
-        DF_tracker=SimpleNamespace(nt=nt, delta_x=dt, recent_planes=cap),
-        beam=SimpleNamespace(_sigma_z=sz, _sigma_x=sx, _slope=np.array([slope0, slope1]),
-                             _mean_x=mean_x),
-        mesh_z_axis=zr, mesh_x_axis=xr, _mesh_slope=np.array([slope0, slope1]))
-    K = CSR2D.near_planes(fake)
-    assert 1 <= K <= min(cap, nt)
-    min_t = t - (nt - 1) * dt
-
-    def col(v, c):
-        return O.interpolate1D(np.atleast_1d(v), lat.coords[:, c] if c < 2 else lat.n_vec[:, c - 2],
-                               lat.min_x, lat.delta_x)
-
-    need = 0
-    for zm in np.linspace(*zr):
-        for xt in np.linspace(*xr):
-            s, x = t + zm, xt + slope0 * zm + slope1
-            regs = O.wake_regions(s, x, t, sz, sx, slope0, mean_x, 1.0, 4, 24, 24)
-            X0, Y0, nx, ny = (col(s, c)[0] for c in range(4))
-            for xp, sp in regs[1:]:
-                sp = sp[(sp >= lat.min_x) & (sp <= lat.max_x)]
-                if sp.size == 0:
-                    continue
-                XP, SP = np.meshgrid(xp, sp, indexing="ij")
-                rx = X0 - col(SP.ravel(), 0) + x * nx - XP.ravel() * col(SP.ravel(), 2)
-                ry = Y0 - col(SP.ravel(), 1) + x * ny - XP.ravel() * col(SP.ravel(), 3)
-                R = np.sqrt(rx * rx + ry * ry).max()
-                i0 = math.trunc((t - R - min_t) / dt)
-                need = max(need, nt - max(i0, 0))
-    assert K >= min(need, cap), (K, need)

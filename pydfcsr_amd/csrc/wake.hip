@@ -9,7 +9,9 @@
 //   workgroup  = one (mesh point, phase); lanes of a wavefront take
 //                consecutive s' columns, so they share x' rows (the x-range
 //                test is wave-uniform) and their retarded lookups are
-//                neighbouring records of the time-innermost history.
+//                neighbouring records of the time-innermost history (region
+//                0) or neighbouring cells of the cell-major one (regions 1..,
+//                when the history has it).
 //   per column : lattice lookups at s', rho(s'), the trapezoid weight in s'.
 //   per point  : |r - r'|, t_ret, the trilinear gather of five fields, the
 //                integrand; points whose gather is outside the history add 0

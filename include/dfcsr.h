@@ -177,8 +177,8 @@ int dfcsr_regrid(const double *src, int64_t src_slot_stride, const DfcsrRegridJo
  * time-innermost one and the other phases the chunked one).
  * The cell-major rings (optional) hold the same slices plane by plane:
  * cell (slot, x, z) at ((slot Xi + x) Zi + z) DFCSR_CELL, the five fields
- * of that grid node then a pad (48 B in float64, 24 B in float32, against
- * 160 B / 80 B corner-packed records).  With them the near-region phases of
+ * of that grid node (40 B in float64, 20 B in float32, against 160 B / 80 B
+ * corner-packed records).  With them the near-region phases of
  * a wake read cells (FP64 wakes: cells, mixed: cells32), whose neighbouring
  * z lie in neighbouring cells; results are identical with or without them.
  * One plane must span at most 4 GiB (the wakes address cells within a plane
@@ -192,7 +192,7 @@ typedef struct {
   int32_t cap, head, nt, Xi, Zi;
   double min_t, min_x, min_z, dt, dx, dz;
 } DfcsrHistory;
-#define DFCSR_CELL 6
+#define DFCSR_CELL 5
 
 /* Lattice tables (lattice.py:4-110, 136-150): table[k][0..5] =
  * coords_x, coords_y, n_x, n_y, tau_x, tau_y at s = min_s + k*ds, k < ns;

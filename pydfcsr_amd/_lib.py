@@ -31,9 +31,9 @@ TWISS_KEYS = ("alpha", "beta", "gamma", "emit", "eta", "etap", "norm_emit")
 HIST_REC64 = 20   # corner-packed records: field f, corner c at [4 f + c]
 HIST_REC32 = 20
 HIST_CHUNKS32 = 5
-CELL = 6   # values per cell of the cell-major rings: five fields, a pad
+CELL = 5   # values per cell of the cell-major rings: the five fields
 # rings: FP64 (Xi, Zi, cap, 20); FP32 chunked (Xi, cap, 5, Zi, 4); FP32
-# time-innermost (Xi, Zi, cap, 20); cell-major (cap, Xi, Zi, 6) float64 / float32
+# time-innermost (Xi, Zi, cap, 20); cell-major (cap, Xi, Zi, 5) float64 / float32
 FIELDS = ("density", "density_x", "density_z", "vx", "vx_x")   # DFCSR_F_* order
 
 

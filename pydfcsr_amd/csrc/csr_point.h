@@ -88,17 +88,13 @@ struct CellRow {
 };
 
 // The five fields of the four corner cells at byte offsets off[c] of a
-// float64 cell plane: two 16-byte loads and one 8-byte load each.
+// float64 cell plane (40-byte cells).
 __device__ inline void load_cells5(const char *plane, const uint32_t off[4], double v[4][5]) {
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
-    const double2 *p = reinterpret_cast<const double2 *>(plane + off[c]);
-    const double2 a = p[0], b = p[1];
-    v[c][0] = a.x;
-    v[c][1] = a.y;
-    v[c][2] = b.x;
-    v[c][3] = b.y;
-    v[c][4] = reinterpret_cast<const double *>(p + 2)[0];
+    const double *p = reinterpret_cast<const double *>(plane + off[c]);
+#pragma unroll
+    for (int f = 0; f < 5; ++f) v[c][f] = p[f];
   }
 }
 
@@ -163,9 +159,8 @@ __device__ inline bool hist_lookup5(const DfcsrHistory &H, const HistAxes &A, do
   }
   const double td = T.frac, xd = X.frac, zd = Z.frac;
   if (kCells) {
-    // cell (slot, x, z) at byte ((slot Xi + x) Zi + z) * 48 of the cell-major ring:
-    // fields 0..3 as two 16-byte loads and field 4, per corner (x0,z0), (x0,z1),
-    // (x1,z0), (x1,z1)
+    // cell (slot, x, z) at byte ((slot Xi + x) Zi + z) * 40 of the cell-major ring,
+    // per corner (x0,z0), (x0,z1), (x1,z0), (x1,z1)
     constexpr uint32_t cb = DFCSR_CELL * sizeof(double);
     const uint32_t rowB = (uint32_t)H.Zi * cb;
     const CellRow r = rr ? *rr : CellRow{(uint32_t)X.i0 * rowB, (uint32_t)(X.i1 - X.i0) * rowB};

@@ -112,17 +112,10 @@ __global__ __launch_bounds__(kThreads) void regrid_kernel(
                                           (float)v[3][f]);
     }
     const int64_t cell = ((int64_t)J.dst_slot * cells + c) * DFCSR_CELL;
-    if (c64) {
-      double2 *o = reinterpret_cast<double2 *>(c64 + cell);
-      o[0] = make_double2(v[0][0], v[0][1]);
-      o[1] = make_double2(v[0][2], v[0][3]);
-      o[2] = make_double2(v[0][4], 0.0);
-    }
-    if (c32) {
-      float2 *o = reinterpret_cast<float2 *>(c32 + cell);
-      o[0] = make_float2((float)v[0][0], (float)v[0][1]);
-      o[1] = make_float2((float)v[0][2], (float)v[0][3]);
-      o[2] = make_float2((float)v[0][4], 0.0f);
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+      if (c64) c64[cell + f] = v[0][f];
+      if (c32) c32[cell + f] = (float)v[0][f];
     }
   }
 }

@@ -292,17 +292,13 @@ struct Neumaier {
 };
 
 // The five fields of the four corner cells at byte offsets off[q] of a
-// float32 cell plane (three 8-byte loads each; the pad too).
-__device__ inline void load_cells32(const char *plane, const uint32_t off[4], float v[4][6]) {
+// float32 cell plane (20-byte cells).
+__device__ inline void load_cells32(const char *plane, const uint32_t off[4], float v[4][5]) {
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const float2 *p = reinterpret_cast<const float2 *>(plane + off[q]);
+    const float *p = reinterpret_cast<const float *>(plane + off[q]);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const float2 a = p[k];
-      v[q][2 * k] = a.x;
-      v[q][2 * k + 1] = a.y;
-    }
+    for (int k = 0; k < 5; ++k) v[q][k] = p[k];
   }
 }
 
@@ -332,7 +328,7 @@ __device__ inline bool csr_point_mixed(const DfcsrHistory &H, const HistAxes &A,
   const float td = (float)T.frac, zd = (float)Z.frac;
   float f[5];
   if (kCells) {
-    // float32 cell (slot, x, z) at byte ((slot Xi + x) Zi + z) * 24 of the cell-major
+    // float32 cell (slot, x, z) at byte ((slot Xi + x) Zi + z) * 20 of the cell-major
     // ring: corners (x0,z0), (x0,z1), (x1,z0), (x1,z1) of each plane, addressed by
     // 32-bit offsets within the plane
     constexpr uint32_t cb = DFCSR_CELL * sizeof(float);
@@ -340,10 +336,10 @@ __device__ inline bool csr_point_mixed(const DfcsrHistory &H, const HistAxes &A,
     const char *ring = reinterpret_cast<const char *>(H.cells32);
     const uint32_t o0 = rr.rb + (uint32_t)Z.i0 * cb, oz = Z.i1 != Z.i0 ? cb : 0u;
     const uint32_t off[4] = {o0, o0 + oz, o0 + rr.dxb, o0 + rr.dxb + oz};
-    float v0[4][6];
+    float v0[4][5];
     load_cells32(ring + hist_slot(H, T.i0) * planeB, off, v0);
     if (T.i1 != T.i0) {
-      float v1[4][6];
+      float v1[4][5];
       load_cells32(ring + hist_slot(H, T.i1) * planeB, off, v1);
 #pragma unroll
       for (int k = 0; k < 5; ++k) {

@@ -12,8 +12,8 @@ whole run:
   hist   [Xi][Zi][cap][20]     float64  re-gridded corner-packed records
   hist32t[Xi][Zi][cap][20]     float32  the same records (mixed precision)
   hist32 [Xi][cap][5][Zi][4]   float32  the same, chunked (optional)
-  cells  [cap][Xi][Zi][6]      float64  cell-major: the five fields at (x, z)
-  cells32[cap][Xi][Zi][6]      float32  the same in float32
+  cells  [cap][Xi][Zi][5]      float64  cell-major: the five fields at (x, z)
+  cells32[cap][Xi][Zi][5]      float32  the same in float32
 (record (x, z) holds every field at (x, z), (x, z+1), (x+1, z), (x+1, z+1),
 clamped to the last plane: one record per time plane per lookup.  hist and
 hist32t keep the slot innermost (the two planes of a lookup are neighbours:

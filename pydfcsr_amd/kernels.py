@@ -182,7 +182,7 @@ def regrid(src: torch.Tensor, slot_stride: int, jobs, hist64, hist32, Xi, Zi, xi
     """jobs: list of (src_slot, dst_slot, nbx, nbz, x0, x1, z0, z1).  Rings (any
     may be None, not all): hist64 float64 (Xi, Zi, cap, 20), hist32 float32
     chunked (Xi, cap, 5, Zi, 4), hist32t float32 time-innermost (Xi, Zi, cap, 20),
-    cells / cells32 cell-major float64 / float32 (cap, Xi, Zi, 6)."""
+    cells / cells32 cell-major float64 / float32 (cap, Xi, Zi, 5)."""
     lib = _lib.load()
     n = len(jobs)
     if n == 0:

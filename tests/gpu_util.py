@@ -50,14 +50,13 @@ def device_ring(records, cap, head, width=4):
 
 
 def cell_ring(data, cap, head, dtype):
-    """(nt, Xi, Zi, 5) field stacks -> a cell-major ring (cap, Xi, Zi, 6): the
-    five fields of node (x, z) then a pad, logical slice i in slot
-    (head + i) % cap; slots holding no slice are NaN."""
+    """(nt, Xi, Zi, 5) field stacks -> a cell-major ring (cap, Xi, Zi, 5): the
+    five fields of node (x, z), logical slice i in slot (head + i) % cap;
+    slots holding no slice are NaN."""
     nt, Xi, Zi, _ = data.shape
     ring = np.full((cap, Xi, Zi, _lib.CELL), np.nan, dtype=dtype)
     for i in range(nt):
-        ring[(head + i) % cap, :, :, :5] = data[i]
-        ring[(head + i) % cap, :, :, 5] = 0
+        ring[(head + i) % cap] = data[i]
     return ring
 
 

@@ -152,8 +152,8 @@ def test_wake_tasks_pack_host_logic():
                                        ctypes.byref(batch))
         assert rc == 0, lib.dfcsr_last_error()
     # the wake addresses a cell within a plane with 32-bit byte offsets: a
-    # plane of 10000 x 10000 float32 cells (2.4 GB) fits, float64 (4.8 GB) not
-    hist[0].Xi = hist[0].Zi = 10000
+    # plane of 11000 x 11000 float32 cells (2.4 GB) fits, float64 (4.8 GB) not
+    hist[0].Xi = hist[0].Zi = 11000
     rc = lib.dfcsr_wake_tasks_pack(k, hist, lat, P, _lib.DFCSR_PREC_MIXED, out, out, tbl, nb,
                                    ctypes.byref(batch))
     assert rc == 0, lib.dfcsr_last_error()

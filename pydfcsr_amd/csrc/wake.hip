@@ -693,7 +693,7 @@ __device__ inline int find_task(const WakeTask *__restrict__ tasks, int ntask, i
 }
 
 template <bool kMixed, bool kNear>
-__global__ __launch_bounds__(kThreads, DFCSR_WAKE_MINBLOCKS) void wake_phase_batched_kernel(
+__global__ __launch_bounds__(kThreads, kMixed && kNear ? 4 : DFCSR_WAKE_MINBLOCKS) void wake_phase_batched_kernel(
     const WakeTask *__restrict__ tasks, int ntask, double *__restrict__ partials) {
   __shared__ double red[kThreads / 64];
   const int b = kNear ? xcd_group(blockIdx.x, gridDim.x, kNearGroup)

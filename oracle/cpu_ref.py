@@ -28,8 +28,12 @@ Pinning (what checks each function, see tests/test_oracle_*.py):
   track_*               bmadx's Drift / SBend / Quadrupole single-particle results
                         held by the reference's test notebook (1e-12); the
                         symplectic condition of the maps' Jacobians; the linear
-                        limit against the r_gen6 transfer matrices.  Sextupole:
-                        symplectic only (parity unpinned).
+                        limit against the reference's r_gen6 matrices (golden
+                        vectors: pole faces, the fringe term at FINT = 1/2,
+                        tilt, the z row).  Sextupole: symplectic, feed-down to
+                        the quadrupole map, second-order convergence to its
+                        field equations; the choice of symmetric splitting
+                        and FINT other than 1/2 are parity unpinned.
   beam_statistics       twiss: a beam built with known Twiss/dispersion (exact
                         sample moments), beta*gamma - alpha^2 = 1, emittance =
                         sqrt(det) of the dispersion-free matrix; energy
@@ -201,8 +205,10 @@ def beam_statistics(x, px, y, py, z, pz, p0c, mc2):
 # (x, px = Px/P0, y, py, z = -beta c (t - t_ref), pz = dP/P0), restated from the
 # physics, and pinned on the Drift / SBend / Quadrupole single-particle results
 # the reference's test notebook holds (test/test_BmadX_tracking.ipynb cells
-# 25, 28, 31; tests/golden/ref_bmadx_tracking.json).  Sextupole: no
-# reference result exists -- parity unpinned.
+# 25, 28, 31; tests/golden/ref_bmadx_tracking.json) and on the reference's
+# r_gen6 matrices in the linear limit (tests/golden/ref_r_gen6.npz).
+# Sextupole: no reference result exists; its field equations pin it up to
+# the choice of symmetric splitting (parity unpinned there).
 def sqrt_one(v):
     """sqrt(1 + v) - 1 without cancellation."""
     return v / (np.sqrt(1 + v) + 1)
@@ -286,7 +292,8 @@ def track_quadrupole(c, L, K1, p0c, mc2, num_steps=1, x_off=0.0, y_off=0.0, tilt
 def _bend_edge(c, g, e, fint=0.0, hgap=0.0):
     """Hard-edge pole face rotated by e (linear edge kicks); the fringe
     integral softens the vertical focusing by psi = fint (2 hgap) g
-    (1 + sin^2 e) / cos e (FINT > 0: parity unpinned)."""
+    (1 + sin^2 e) / cos e (r_gen6.py:49-56 at FINT = 1/2; other FINT values:
+    parity unpinned)."""
     x, px, y, py, z, pz = c
     psi = fint * (2 * hgap) * g * (1 + np.sin(e) ** 2) / np.cos(e)
     if e == 0 and psi == 0:
@@ -326,7 +333,9 @@ def track_sbend(c, L, g, e1, e2, p0c, mc2, fint=0.0, hgap=0.0, fintx=0.0, hgapx=
 
 def track_sextupole(c, L, K2, p0c, mc2, num_steps=1, x_off=0.0, y_off=0.0, tilt=0.0):
     """Sextupole as drift - thin kick - drift per step (the kick of K2 ds:
-    dpx = -K2 ds (x^2 - y^2) / 2, dpy = K2 ds x y).  Parity unpinned."""
+    dpx = -K2 ds (x^2 - y^2) / 2, dpy = K2 ds x y).  Pinned to its field
+    equations; drift-kick-drift rather than kick-drift-kick is parity
+    unpinned."""
     cc = _offset_set(tuple(np.asarray(v, dtype=np.float64) for v in c), x_off, y_off, tilt)
     ds = L / num_steps
     for _ in range(num_steps):

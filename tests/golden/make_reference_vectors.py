@@ -2,7 +2,9 @@
 
 Run ONLY in the build container, where the reference source tree is mounted:
 
-    python tests/golden/make_reference_vectors.py /root/reference/pyDFCSR_2D
+    python tests/golden/make_reference_vectors.py /root/reference/pyDFCSR_2D [r_gen6]
+
+(with ``r_gen6`` only ``ref_r_gen6.npz``, the transfer matrices, is written)
 
 The reference package cannot be imported as shipped (numba, bmadx, mpi4py,
 h5py, distgen and pmd_beamphysics are absent).  The functions captured here
@@ -267,5 +269,42 @@ def main(ref_dir):
     print("golden vectors written to", HERE)
 
 
+# (L, angle, k1, roll, E1, E2, hgap): drift, quadrupoles, sector and
+# combined-function bends, pole faces, the fringe term (hgap, FINT = 1/2)
+# and rolls
+R_GEN6_CASES = [
+    (0.3, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0),
+    (0.3, 0.0, 5.0, 0.0, 0.0, 0.0, 0.0),
+    (0.3, 0.0, -5.0, 0.0, 0.0, 0.0, 0.0),
+    (0.3, 0.05, 0.0, 0.0, 0.0, 0.0, 0.0),
+    (0.3, 0.05, 0.0, 0.0, 0.05, 0.0, 0.0),
+    (0.3, -0.05, 0.0, 0.0, 0.0, -0.05, 0.0),
+    (0.3, 0.05, 0.0, 0.0, 0.02, -0.03, 0.017),
+    (0.3, -0.08, 0.0, 0.0, 0.0, 0.0, 0.02),
+    (0.5, 0.2, 0.0, 0.0, 0.1, 0.1, 0.025),
+    (0.3, 0.05, 2.0, 0.0, 0.0, 0.0, 0.0),
+    (0.3, 0.0, 5.0, np.pi / 4, 0.0, 0.0, 0.0),
+    (0.0, 0.0, 0.0, 0.3, 0.0, 0.0, 0.0),
+]
+
+
+def transfer_matrices(ref_dir):
+    """r_gen6 (r_gen6.py:4-122) on R_GEN6_CASES -> ref_r_gen6.npz."""
+    R = _load_reference(ref_dir)
+    rg = R["r_gen6"].r_gen6
+    cases = np.array(R_GEN6_CASES, dtype=np.float64)
+    mats = np.stack([rg(L, angle=a, k1=k, roll=r, E1=e1, E2=e2, hgap=hg)
+                     for (L, a, k, r, e1, e2, hg) in R_GEN6_CASES])
+    np.savez(os.path.join(HERE, "ref_r_gen6.npz"), cases=cases, R=mats)
+    print("r_gen6 vectors written to", HERE)
+
+
 if __name__ == "__main__":
-    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference/pyDFCSR_2D")
+    # make_reference_vectors.py [REF_DIR] [r_gen6]: every fixture, or only
+    # the transfer matrices
+    ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference/pyDFCSR_2D"
+    if sys.argv[2:] == ["r_gen6"]:
+        transfer_matrices(ref)
+    else:
+        main(ref)
+        transfer_matrices(ref)

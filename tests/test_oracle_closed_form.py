@@ -330,22 +330,95 @@ def test_tracking_maps_are_symplectic(name):
     assert np.abs(J.T @ S @ J - S).max() < 1e-7, np.abs(J.T @ S @ J - S).max()
 
 
-@pytest.mark.parametrize("L,angle,k1,e1,e2", [(0.3, 0.0, 0.0, 0.0, 0.0), (0.3, 0.0, 5.0, 0.0, 0.0),
-                                            (0.3, 0.0, -5.0, 0.0, 0.0),
-                                            (0.3, 0.05, 0.0, 0.05, 0.0),
-                                            (0.3, -0.05, 0.0, 0.0, -0.05)])
-def test_tracking_linear_limit_is_r_gen6(L, angle, k1, e1, e2):
-    """At small amplitude the maps' transverse rows (x, px, y, py over x, px,
-    y, py, pz) equal the r_gen6 transfer matrix (r_gen6.py:4-122)."""
+def _r_gen6_vectors():
+    import os
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_r_gen6.npz"))
+    return d["cases"], d["R"]
+
+
+def test_transfer_matrix_matches_r_gen6_vectors():
+    """The host's linear map equals the reference's r_gen6 (r_gen6.py:4-122)
+    on every unrolled case of tests/golden/ref_r_gen6.npz, pole faces and the
+    fringe term included."""
     from pydfcsr_amd.beams import transfer_matrix
+    cases, mats = _r_gen6_vectors()
+    n = 0
+    for (L, angle, k1, roll, e1, e2, hgap), R in zip(cases, mats):
+        if roll:
+            continue
+        T = transfer_matrix(L, angle=angle, k1=k1, E1=e1, E2=e2, hgap=hgap)
+        assert np.allclose(T, R, rtol=0, atol=1e-15), (L, angle, k1, e1, e2, hgap)
+        n += 1
+    assert n == 10
+
+
+@pytest.mark.parametrize("case", [0, 1, 2, 3, 4, 5, 6, 7, 8, 10])
+def test_tracking_linear_limit_is_r_gen6(case):
+    """At small amplitude the maps equal the reference's r_gen6 matrices
+    (tests/golden/ref_r_gen6.npz): the transverse rows exactly, so the pole
+    faces and the fringe term at r_gen6's FINT = 1/2 (r_gen6.py:49-56) are
+    pinned, and the z row up to the relativistic slip L / (beta gamma)^2 that
+    r_gen6 leaves out (R56 of an ultrarelativistic beam).  Case 10 is a
+    quadrupole rolled by pi/4 (r_gen6's roll is the element's tilt); cases 9
+    (a combined-function bend) and 11 (a bare rotation) are no tracked
+    element."""
+    cases, mats = _r_gen6_vectors()
+    L, angle, k1, roll, e1, e2, hgap = cases[case]
+    R = mats[case].copy()
     p0c, mc2 = 1e10, 510998.95069
     if angle:
-        f = lambda c: O.track_sbend(c, L, angle / L, e1, e2, p0c, mc2)
+        f = lambda c: O.track_sbend(c, L, angle / L, e1, e2, p0c, mc2, fint=0.5, hgap=hgap,
+                                    fintx=0.5, hgapx=hgap)
     elif k1:
-        f = lambda c: O.track_quadrupole(c, L, k1, p0c, mc2)
+        f = lambda c: O.track_quadrupole(c, L, k1, p0c, mc2, tilt=roll)
     else:
         f = lambda c: O.track_drift(c, L, p0c, mc2)
-    J = _jacobian(f, np.zeros(6), h=1e-9)
-    R = transfer_matrix(L, angle=angle, k1=k1, E1=e1, E2=e2)
-    rows, cols = [0, 1, 2, 3], [0, 1, 2, 3, 5]
-    assert np.allclose(J[np.ix_(rows, cols)], R[np.ix_(rows, cols)], rtol=1e-6, atol=1e-8)
+    J = _jacobian(f, np.zeros(6), h=1e-6)
+    assert np.abs(J[:4] - R[:4]).max() < 1e-12
+    R[4, 5] += L * (mc2 / p0c) ** 2
+    assert np.abs(J[4] - R[4]).max() < 1e-10
+    assert np.array_equal(np.round(J[5], 6), [0, 0, 0, 0, 0, 1])
+
+
+def test_sextupole_feed_down_is_a_quadrupole():
+    """Off axis by x0 a short sextupole focuses like a quadrupole of
+    K1 = K2 x0 (the feed-down of (x + iy)^2): its linear map about the orbit
+    equals the quadrupole map (pinned on bmadx results) to third order in L.
+    A flipped sign, a lost 1/2 or swapped planes change the focusing term
+    K1 L = 1.2e-3 by its own size."""
+    p0c, mc2 = 1e9, 510998.95069
+    L, K2, x0 = 0.02, 30.0, 2e-3
+    c0 = np.array([x0, 0.0, 0.0, 0.0, 0.0, 0.0])
+    Js = _jacobian(lambda c: O.track_sextupole(c, L, K2, p0c, mc2), c0)
+    Jq = _jacobian(lambda c: O.track_quadrupole(c, L, K2 * x0, p0c, mc2), c0)
+    assert np.abs(Js[:4, :4] - Jq[:4, :4]).max() < 1e-7
+    assert Js[1, 0] < -1e-3 and Js[3, 2] > 1e-3
+
+
+def test_sextupole_converges_to_its_field_equations():
+    """With many steps the sextupole map is the flow of x' = px/ps,
+    px' = -K2 (x^2 - y^2)/2, y' = py/ps, py' = K2 x y,
+    z' = beta/beta0 - (1 + pz)/ps (ps = sqrt((1+pz)^2 - px^2 - py^2)),
+    integrated here by scipy's DOP853; the error falls as the square of the
+    step (a symmetric splitting).  Which symmetric splitting bmadx uses
+    (drift-kick-drift or kick-drift-kick) no reference result fixes."""
+    from scipy.integrate import solve_ivp
+    p0c, mc2, L, K2 = 1e9, 510998.95069, 0.2, 300.0
+    b0 = p0c / np.hypot(p0c, mc2)
+
+    def rhs(s, u):
+        x, px, y, py, z, pz = u
+        P = 1 + pz
+        ps = np.sqrt(P * P - px * px - py * py)
+        b = P * p0c / np.hypot(P * p0c, mc2)
+        return [px / ps, -K2 * (x * x - y * y) / 2, py / ps, K2 * x * y, b / b0 - P / ps, 0.0]
+
+    u0 = np.array([2e-3, 1e-3, 1e-3, -5e-4, 1e-4, 1e-3])
+    exact = solve_ivp(rhs, (0.0, L), u0, method="DOP853", rtol=1e-13, atol=1e-16).y[:, -1]
+    e10 = np.abs(O.track_sextupole(u0, L, K2, p0c, mc2, num_steps=10) - exact).max()
+    e100 = np.abs(O.track_sextupole(u0, L, K2, p0c, mc2, num_steps=100) - exact).max()
+    e1000 = np.abs(O.track_sextupole(u0, L, K2, p0c, mc2, num_steps=1000) - exact).max()
+    assert e1000 < 2e-12
+    assert 80 < e10 / e100 < 120 and 80 < e100 / e1000 < 120
+    # the field matters at this size: a plain drift misses by 1e-4
+    assert np.abs(O.track_drift(u0, L, p0c, mc2) - exact).max() > 1e-4

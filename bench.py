@@ -163,10 +163,11 @@ def cpu_baseline(sim, gpu_out, budget_s=15.0, max_points=None):
             "max_rel_diff_vs_gpu_dE": err}, dE
 
 
-def profiled_traffic():
-    """HBM bytes per wake launch from the committed rocprofv3 counter runs
-    (FETCH_SIZE and WRITE_SIZE, each divided by its copy-kernel factor)."""
-    path = os.path.join(ROOT, "profiles", "wake_traffic.json")
+def profiled_traffic(name="wake_traffic.json"):
+    """HBM bytes per launch (the wake's four, or one other kernel's) from the
+    committed rocprofv3 counter runs (FETCH_SIZE and WRITE_SIZE, each divided
+    by its copy-kernel factor)."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -268,6 +269,82 @@ def serve_main(args, dev):
     print(json.dumps(res), flush=True)
 
 
+SG2D_GRID, SG2D_WINDOW, SG2D_ORDER = 8192, 7, 2
+SG2D_BYTES_PER_POINT = 24    # 'both': one float64 read, two float64 writes
+
+
+def sgolay_main(args, dev):
+    """The 2D Savitzky-Golay filter (SGolay_filter.sgolay2d) as get_DF's
+    commented alternative calls it for the gradients (derivative 'both',
+    deposit.py:227, 232): one launch on an 8192 x 8192 float64 grid, window
+    7, order 2; also the three calls get_DF would make per step at the
+    chicane's 128 x 128 deposition grid (smoothing, 'both' on the density,
+    'both' on vx)."""
+    import numpy as np
+    import torch
+    from pydfcsr_amd import kernels
+    n, w, order = SG2D_GRID, SG2D_WINDOW, SG2D_ORDER
+    gen = torch.Generator(device=dev).manual_seed(0)
+    z = torch.rand((n, n), dtype=torch.float64, device=dev, generator=gen)
+    for _ in range(args.warmup if args.warmup is not None else 3):
+        kernels.sgolay2d(z, w, order, "both")
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream(dev)     # the stream the library launches on
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for a, b in evs:
+        a.record(stream)
+        kernels.sgolay2d(z, w, order, "both")
+        b.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
+    achieved = SG2D_BYTES_PER_POINT * n * n / (ms * 1e-3) / 1e9
+    small = torch.rand((128, 128), dtype=torch.float64, device=dev, generator=gen)
+    vx = torch.rand((128, 128), dtype=torch.float64, device=dev, generator=gen)
+    use = []
+    for _ in range(20):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        kernels.sgolay2d(small, w, order, None)
+        kernels.sgolay2d(small, w, order, "both")
+        kernels.sgolay2d(vx, w, order, "both")
+        b.record(stream)
+        torch.cuda.synchronize()
+        use.append(a.elapsed_time(b) * 1e3)
+    res = {"metric": "sgolay2d grid points/s", "value": round(n * n * args.steps / wall / 1e9, 4),
+           "unit": "Gpts/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup if args.warmup is not None else 3,
+           "ms_per_step": round(wall * 1e3 / args.steps, 4), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (uniform)",
+           "config": {"workload": f"sgolay2d derivative 'both' on a {n} x {n} float64 grid, "
+                                  f"window {w}, order {order}"},
+           "kernel_ms": round(ms, 4),
+           "get_df_calls_128x128_us": round(float(np.median(use[5:])), 2),
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0,
+                        "unit": "GB/s", "frac": round(achieved / 8000.0, 4),
+                        "traffic": profiled_traffic("sgolay2d_traffic.json"),
+                        "kernel": "sgolay2d_kernel",
+                        "bytes_per_point": SG2D_BYTES_PER_POINT}}
+    if not args.no_cpu_baseline:
+        from oracle import cpu_ref as O
+        m = 2048
+        zs = z[:m, :m].cpu().numpy()
+        t = time.perf_counter()
+        reps = 0
+        while reps == 0 or time.perf_counter() - t < min(args.cpu_budget, 10.0):
+            O.sgolay2d(zs, w, order, "both")
+            reps += 1
+        dt = (time.perf_counter() - t) / reps
+        res["cpu_baseline"] = {"value": round(m * m / dt / 1e9, 5), "unit": "Gpts/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"oracle sgolay2d (numpy pinv + scipy fftconvolve, as "
+                                         f"the reference) on a {m} x {m} corner of the grid, "
+                                         f"{reps} calls"}
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -276,7 +353,7 @@ def main():
                     help="untimed steps before the timed ones (default: 200 chicane, 90 "
                          "dipole, 600 large)")
     ap.add_argument("--workload", default="chicane",
-                    choices=["chicane", "dipole", "large", "serving"])
+                    choices=["chicane", "dipole", "large", "serving", "sgolay2d"])
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrivals per second")
     ap.add_argument("--serve-seconds", type=float, default=90.0)
@@ -308,6 +385,9 @@ def main():
     dev = torch.device("cuda:0")
     if args.workload == "serving":
         serve_main(args, dev)
+        return
+    if args.workload == "sgolay2d":
+        sgolay_main(args, dev)
         return
     make_cfg, default_warmup, description = WORKLOADS[args.workload]
     if args.warmup is None:

@@ -143,6 +143,34 @@ int dfcsr_df_process(double *density, double *vx, double *density_x, double *den
                      double velocity_threshold, int window, int order, double *info,
                      void *ws, int64_t ws_bytes, void *stream);
 
+/* ---- a7: 2D Savitzky-Golay filter -------------------------------------------
+ * Replaces SGolay_filter.sgolay2d(z, window_size, order, derivative)
+ * (SGolay_filter.py:3-81; get_DF's commented alternative to a5/a6,
+ * deposit.py:187, 194, 227, 232).  z: row-major [nx][nz] float64 on the
+ * device.  derivative: DFCSR_SG2D_NONE (the smoothed grid into out),
+ * _COL (the slope along the first axis, per index step), _ROW (along the
+ * second), _BOTH (col into out, row into out2) -- the reference's None,
+ * 'col', 'row', 'both'.  The grid is padded as the reference pads it
+ * (mirrored bands about the edge values) and convolved with the rows of the
+ * fit's pseudo-inverse; the sum runs in a fixed order where the reference
+ * uses an FFT (results agree to rounding).  Limits: window_size odd,
+ * 3 <= window_size <= DFCSR_SG2D_MAX_WINDOW (else DFCSR_ERANGE);
+ * 0 <= order < window_size (the reference also accepts rank-deficient fits,
+ * order >= window_size); order >= 1 for a derivative; nx, nz >=
+ * window_size / 2 + 1 (the reference's padding slices need as much).
+ * One launch on `stream`; no allocation, no workspace. */
+#define DFCSR_SG2D_NONE 0
+#define DFCSR_SG2D_COL 1
+#define DFCSR_SG2D_ROW 2
+#define DFCSR_SG2D_BOTH 3
+#define DFCSR_SG2D_MAX_WINDOW 13
+int dfcsr_sgolay2d(const double *z, int nx, int nz, int window_size, int order, int derivative,
+                   double *out, double *out2, void *stream);
+/* Host-side (no device work): the [window_size][window_size] kernels
+ * dfcsr_sgolay2d convolves with -- pinv row 0, or the negated rows 1 / 2
+ * (SGolay_filter.py:69-81) -- into k0 (and k1 for _BOTH). */
+int dfcsr_sgolay2d_kernels(int window_size, int order, int derivative, double *k0, double *k1);
+
 /* ---- a8-a10: history re-grid ------------------------------------------------
  * Replaces DF_tracker.DF_interp / append_interpolant's re-gridding
  * (deposit.py:296-309, 328-337, 379-390): bilinear RegularGridInterpolator

@@ -34,6 +34,11 @@ Pinning (what checks each function, see tests/test_oracle_*.py):
                         the quadrupole map, second-order convergence to its
                         field equations; the choice of symmetric splitting
                         and FINT other than 1/2 are parity unpinned.
+  sgolay2d              golden vectors from the reference module (every
+                        derivative mode, minimal grids); the notebook's ramp
+                        (test/test.ipynb cell 23); exact on polynomials of
+                        the fit's degree away from the border; padding of a
+                        ramp is its linear continuation.
   beam_statistics       twiss: a beam built with known Twiss/dispersion (exact
                         sample moments), beta*gamma - alpha^2 = 1, emittance =
                         sqrt(det) of the dispersion-free matrix; energy
@@ -404,6 +409,78 @@ def get_DF(x, z, px, t, xbins=100, zbins=100, xlim=5, zlim=5, filter_order=0,
 
 
 FIELDS = ("density", "density_x", "density_z", "vx", "vx_x")
+
+
+# ---------------------------------------------------------------------------
+# a7  SGolay_filter.sgolay2d   (SGolay_filter.py:3-81; get_DF's commented
+#     alternative, deposit.py:187, 194, 227, 232)
+# ---------------------------------------------------------------------------
+def sgolay2d_pad(z, h):
+    """The grid padded by h on every side (SGolay_filter.py:36-66), by
+    distance k = 1..h from each edge: top/left bands mirror row/column k about
+    the edge value from below (e - |z_k - e|), bottom/right bands from above
+    (e + |z_k - e|); the top-left and bottom-right corners mirror about the
+    corner value; the top-right corner mirrors the right band about its first
+    row, the bottom-left corner the bottom band about its first column."""
+    z = np.asarray(z, dtype=np.float64)
+    nx, nz = z.shape
+    Z = np.zeros((nx + 2 * h, nz + 2 * h))
+    Z[h:h + nx, h:h + nz] = z
+    for k in range(1, h + 1):
+        Z[h - k, h:h + nz] = z[0] - np.abs(z[k] - z[0])
+        Z[h + nx - 1 + k, h:h + nz] = z[-1] + np.abs(z[nx - 1 - k] - z[-1])
+        Z[h:h + nx, h - k] = z[:, 0] - np.abs(z[:, k] - z[:, 0])
+        Z[h:h + nx, h + nz - 1 + k] = z[:, -1] + np.abs(z[:, nz - 1 - k] - z[:, -1])
+    for k1 in range(1, h + 1):
+        for k2 in range(1, h + 1):
+            e = z[0, 0]
+            Z[h - k1, h - k2] = e - abs(z[k1, k2] - e)
+            e = z[-1, -1]
+            Z[h + nx - 1 + k1, h + nz - 1 + k2] = e + abs(z[nx - 1 - k1, nz - 1 - k2] - e)
+            col = h + nz - 1 + k2                  # right band, real rows 0 and k1
+            e = Z[h, col]
+            Z[h - k1, col] = e - abs(Z[h + k1, col] - e)
+            row = h + nx - 1 + k1                  # bottom band, real columns 0 and k2
+            e = Z[row, h]
+            Z[row, h - k2] = e - abs(Z[row, h + k2] - e)
+    return Z
+
+
+def sgolay2d_fit_terms(order):
+    """(x power, y power) of the fit's monomials in the reference's order
+    (SGolay_filter.py:24): by degree, x power falling within a degree."""
+    return [(deg - n, n) for deg in range(order + 1) for n in range(deg + 1)]
+
+
+def sgolay2d(z, window_size, order, derivative=None):
+    """SGolay_filter.sgolay2d: least-squares fit of a degree-`order` 2D
+    polynomial over each window, evaluated as convolutions of the padded
+    grid with rows of pinv(A), A[p, t] = dx_p^ex_t dy_p^ey_t at window
+    offsets dx (first axis, slow) and dy; 'col' / 'row' convolve with the
+    negated slope rows 1 / 2, 'both' returns (col, row)."""
+    if window_size % 2 == 0:
+        raise ValueError("window_size must be odd")
+    terms = sgolay2d_fit_terms(order)
+    if window_size ** 2 < len(terms):
+        raise ValueError("order is too high for the window size")
+    from scipy.signal import fftconvolve
+    h = window_size // 2
+    off = np.arange(-h, h + 1, dtype=np.float64)
+    dx, dy = np.meshgrid(off, off, indexing="ij")
+    A = np.stack([dx.ravel() ** ex * dy.ravel() ** ey for ex, ey in terms], axis=1)
+    Ainv = np.linalg.pinv(A)
+    Z = sgolay2d_pad(z, h)
+    kern = lambda t, sign: sign * Ainv[t].reshape(window_size, window_size)
+    if derivative is None:
+        return fftconvolve(Z, kern(0, 1.0), mode="valid")
+    if derivative == "col":
+        return fftconvolve(Z, kern(1, -1.0), mode="valid")
+    if derivative == "row":
+        return fftconvolve(Z, kern(2, -1.0), mode="valid")
+    if derivative == "both":
+        return (fftconvolve(Z, kern(1, -1.0), mode="valid"),
+                fftconvolve(Z, kern(2, -1.0), mode="valid"))
+    raise ValueError(f"unknown derivative {derivative!r}")
 
 
 # ---------------------------------------------------------------------------

@@ -32,6 +32,9 @@ HIST_REC64 = 20   # corner-packed records: field f, corner c at [4 f + c]
 HIST_REC32 = 20
 HIST_CHUNKS32 = 5
 CELL = 5   # values per cell of the cell-major rings: the five fields
+# sgolay2d's derivative argument: the reference's None, 'col', 'row', 'both'
+SG2D_DERIVATIVE = {None: 0, "col": 1, "row": 2, "both": 3}
+SG2D_MAX_WINDOW = 13
 # rings: FP64 (Xi, Zi, cap, 20); FP32 chunked (Xi, cap, 5, Zi, 4); FP32
 # time-innermost (Xi, Zi, cap, 20); cell-major (cap, Xi, Zi, 5) float64 / float32
 FIELDS = ("density", "density_x", "density_z", "vx", "vx_x")   # DFCSR_F_* order
@@ -100,6 +103,8 @@ _SIGNATURES = [
     ("dfcsr_df_process", c_i32, [c_dp, c_dp, c_dp, c_dp, c_dp, c_i32, c_f64, c_f64, c_i32,
                                  c_f64, c_f64, c_f64, c_i32, c_i32, c_dp, c_dp, c_i64, c_dp]),
     ("dfcsr_savgol_operator", c_i32, [c_i32, c_i32, c_dp, c_dp]),
+    ("dfcsr_sgolay2d", c_i32, [c_dp, c_i32, c_i32, c_i32, c_i32, c_i32, c_dp, c_dp, c_dp]),
+    ("dfcsr_sgolay2d_kernels", c_i32, [c_i32, c_i32, c_i32, c_dp, c_dp]),
     ("dfcsr_regrid_workspace_bytes", c_i64, [c_i32]),
     ("dfcsr_regrid", c_i32, [c_dp, c_i64, ctypes.POINTER(RegridJob), c_i32, c_dp, c_dp, c_dp,
                              c_dp, c_dp, c_i32, c_i32, c_i32, c_f64, c_f64, c_f64, c_f64, c_dp,

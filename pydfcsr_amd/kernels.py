@@ -177,6 +177,38 @@ def savgol_operator(window, order):
     return c, e[: window // 2]
 
 
+def sgolay2d(z: torch.Tensor, window_size: int, order: int, derivative=None):
+    """SGolay_filter.sgolay2d (SGolay_filter.py:3-81) on the device: z a
+    float64 (nx, nz) tensor; derivative None (smoothed grid), 'col' (slope
+    along the first axis, per index step), 'row' (along the second) or
+    'both' (the tuple (col, row)), as the reference returns them."""
+    lib = _lib.load()
+    if derivative not in _lib.SG2D_DERIVATIVE:
+        raise ValueError(f"derivative must be one of None, 'col', 'row', 'both', not {derivative!r}")
+    if not isinstance(z, torch.Tensor) or z.dim() != 2:
+        raise DfcsrError("sgolay2d: z must be a 2-D tensor")
+    z = require_device_tensor(z.contiguous(), torch.float64, "z")
+    nx, nz = z.shape
+    out = torch.empty_like(z)
+    out2 = torch.empty_like(z) if derivative == "both" else None
+    check(lib.dfcsr_sgolay2d(ptr(z), nx, nz, int(window_size), int(order),
+                             _lib.SG2D_DERIVATIVE[derivative], ptr(out), ptr(out2),
+                             stream_handle(z.device)), "sgolay2d")
+    return (out, out2) if derivative == "both" else out
+
+
+def sgolay2d_kernels(window_size: int, order: int, derivative=None):
+    """Host-side: the (window, window) kernels sgolay2d convolves with (one,
+    or two for 'both')."""
+    lib = _lib.load()
+    w = int(window_size)
+    k0 = np.zeros((max(w, 1), max(w, 1)))
+    k1 = np.zeros_like(k0)
+    check(lib.dfcsr_sgolay2d_kernels(w, int(order), _lib.SG2D_DERIVATIVE[derivative],
+                                     k0.ctypes.data, k1.ctypes.data), "sgolay2d_kernels")
+    return (k0, k1) if derivative == "both" else k0
+
+
 def regrid(src: torch.Tensor, slot_stride: int, jobs, hist64, hist32, Xi, Zi, xi0, xi1, zi0, zi1,
            ws: _lib.Workspace, hist32t=None, cells=None, cells32=None):
     """jobs: list of (src_slot, dst_slot, nbx, nbz, x0, x1, z0, z1).  Rings (any

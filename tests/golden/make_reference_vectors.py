@@ -2,9 +2,10 @@
 
 Run ONLY in the build container, where the reference source tree is mounted:
 
-    python tests/golden/make_reference_vectors.py /root/reference/pyDFCSR_2D [r_gen6]
+    python tests/golden/make_reference_vectors.py /root/reference/pyDFCSR_2D [r_gen6|sgolay2d]
 
-(with ``r_gen6`` only ``ref_r_gen6.npz``, the transfer matrices, is written)
+(with ``r_gen6`` / ``sgolay2d`` only ``ref_r_gen6.npz`` / ``ref_sgolay2d.npz``
+is written)
 
 The reference package cannot be imported as shipped (numba, bmadx, mpi4py,
 h5py, distgen and pmd_beamphysics are absent).  The functions captured here
@@ -299,12 +300,46 @@ def transfer_matrices(ref_dir):
     print("r_gen6 vectors written to", HERE)
 
 
+# (nx, nz, window_size, order): grids with a smooth part and noise; the last
+# two at the smallest size the padding allows (window_size // 2 + 1)
+SGOLAY_CASES = [(40, 50, 3, 1), (40, 50, 5, 2), (33, 17, 7, 3), (24, 29, 9, 2),
+                (4, 4, 7, 2), (6, 5, 9, 3)]
+SGOLAY_MODES = (None, "col", "row", "both")
+
+
+def sgolay_vectors(ref_dir):
+    """SGolay_filter.sgolay2d (SGolay_filter.py:3-81) on SGOLAY_CASES, every
+    derivative mode -> ref_sgolay2d.npz (keys z{i}, {mode}{i}; 'both' as
+    bothcol{i}, bothrow{i})."""
+    spec = importlib.util.spec_from_file_location("SGolay_filter",
+                                                  os.path.join(ref_dir, "SGolay_filter.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    rng = np.random.default_rng(2718)
+    out = {"cases": np.array(SGOLAY_CASES, dtype=np.int64)}
+    for i, (nx, nz, w, order) in enumerate(SGOLAY_CASES):
+        xg, zg = np.meshgrid(np.linspace(-1, 1, nx), np.linspace(-1, 1, nz), indexing="ij")
+        z = np.exp(-(xg ** 2 + 2 * zg ** 2)) + 0.3 * xg * zg + 0.05 * rng.normal(size=(nx, nz))
+        out[f"z{i}"] = z
+        for mode in SGOLAY_MODES:
+            r = mod.sgolay2d(z, w, order, derivative=mode)
+            if mode == "both":
+                out[f"bothcol{i}"], out[f"bothrow{i}"] = r
+            else:
+                out[f"{mode or 'none'}{i}"] = r
+    np.savez(os.path.join(HERE, "ref_sgolay2d.npz"), **out)
+    print("sgolay2d vectors written to", HERE)
+
+
 if __name__ == "__main__":
-    # make_reference_vectors.py [REF_DIR] [r_gen6]: every fixture, or only
-    # the transfer matrices
+    # make_reference_vectors.py [REF_DIR] [r_gen6 | sgolay2d]: every
+    # fixture, or only the transfer matrices / the 2D filter's
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference/pyDFCSR_2D"
     if sys.argv[2:] == ["r_gen6"]:
         transfer_matrices(ref)
+    elif sys.argv[2:] == ["sgolay2d"]:
+        sgolay_vectors(ref)
     else:
         main(ref)
         transfer_matrices(ref)
+        sgolay_vectors(ref)

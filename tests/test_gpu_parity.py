@@ -415,3 +415,53 @@ def test_track_beam_vs_oracle(kind):
     scale = np.abs(ref).max(axis=1, keepdims=True)
     err = (np.abs(r - ref) / scale).max()
     assert err <= 1e-12, (kind, err)
+
+
+# -- a7 sgolay2d (SGolay_filter.py:3-81) -------------------------------------
+# Tolerance: |d| <= 1e-12 * max|z| (the device sums the window in a fixed
+# order; the reference and the oracle convolve by FFT).
+def _sg_run(K, z, w, order, mode):
+    print(f"sgolay2d {z.shape} window {w} order {order} {mode}", flush=True)
+    r = K.sgolay2d(_t(z), w, order, mode)
+    torch.cuda.synchronize()
+    return tuple(x.cpu().numpy() for x in r) if mode == "both" else r.cpu().numpy()
+
+
+def test_sgolay2d_golden(K, golden_dir):
+    d = _g(golden_dir, "ref_sgolay2d.npz")
+    for i, (nx, nz, w, order) in enumerate(d["cases"]):
+        z = d[f"z{i}"]
+        tol = 1e-12 * np.abs(z).max()
+        for mode in (None, "col", "row"):
+            got = _sg_run(K, z, int(w), int(order), mode)
+            assert np.abs(got - d[f"{mode or 'none'}{i}"]).max() <= tol, (i, mode)
+        col, row = _sg_run(K, z, int(w), int(order), "both")
+        assert np.abs(col - d[f"bothcol{i}"]).max() <= tol, (i, "both")
+        assert np.abs(row - d[f"bothrow{i}"]).max() <= tol, (i, "both")
+
+
+@pytest.mark.parametrize("shape,w,order", [((100, 37), 3, 1), ((257, 130), 5, 2),
+                                           ((16, 16), 7, 3), ((17, 15), 13, 4),
+                                           ((7, 40), 13, 2), ((33, 64), 11, 4)])
+def test_sgolay2d_vs_oracle(K, shape, w, order):
+    """Ragged tiles (sizes off multiples of 16), the largest window, grids
+    at the smallest size the padding allows, an offset and scale far from
+    1."""
+    rng = np.random.default_rng(w * 100 + order)
+    nx, nz = shape
+    xg, zg = np.meshgrid(np.linspace(-2, 2, nx), np.linspace(-1, 1, nz), indexing="ij")
+    z = 1e6 * (3.0 + np.sin(2 * xg) * np.cos(3 * zg) + 0.1 * rng.normal(size=shape))
+    tol = 1e-12 * np.abs(z).max()
+    for mode in (None, "col", "row", "both"):
+        got = _sg_run(K, z, w, order, mode)
+        ref = O.sgolay2d(z, w, order, mode)
+        if mode == "both":
+            assert max(np.abs(got[0] - ref[0]).max(), np.abs(got[1] - ref[1]).max()) <= tol
+        else:
+            assert np.abs(got - ref).max() <= tol, mode
+
+
+def test_sgolay2d_notebook_ramp(K):
+    """test/test.ipynb cell 23: 'row' of the 3 x 3 ramp is 1 everywhere."""
+    got = _sg_run(K, np.arange(1.0, 10.0).reshape(3, 3), 3, 1, "row")
+    assert np.allclose(got, 1.0, rtol=0, atol=1e-14)

@@ -39,6 +39,12 @@ Pinning (what checks each function, see tests/test_oracle_*.py):
                         (test/test.ipynb cell 23); exact on polynomials of
                         the fit's degree away from the border; padding of a
                         ramp is its linear continuation.
+  run_loop              golden vectors of the reference's own CSR2D.run on six
+                        lattices (leading drift, nsep 2 and 5, elements shorter
+                        than a step incl. the first and a one-piece bend, the
+                        chicane workload), bmadx replaced by a deterministic
+                        stand-in map: pieces, sigma_z, positions, formation
+                        lengths and the history window bit-exact.
   beam_statistics       twiss: a beam built with known Twiss/dispersion (exact
                         sample moments), beta*gamma - alpha^2 = 1, emittance =
                         sqrt(det) of the dispersion-free matrix; energy
@@ -349,6 +355,56 @@ def track_sextupole(c, L, K2, p0c, mc2, num_steps=1, x_off=0.0, y_off=0.0, tilt=
         cc = (x, px - K2 * ds * (x * x - y * y) / 2, y, py + K2 * ds * x * y, z, pz)
         cc = tuple(track_drift(cc, ds / 2, p0c, mc2))
     return np.stack(_offset_unset(cc, x_off, y_off, tilt))
+
+
+# ---------------------------------------------------------------------------
+# linear maps of element pieces (r_gen6.py:4-122), for linear-optics checks
+# ---------------------------------------------------------------------------
+def transfer_matrix(L, angle=0.0, k1=0.0, E1=0.0, E2=0.0, hgap=0.0):
+    """6x6 linear map (x, px, y, py, z, pz) of a bend/quad/drift piece: the
+    linear optics r_gen6.py:4-122 computes, written from the textbook
+    sector-bend and quadrupole matrices with pole-face kicks (the vertical
+    one corrected by the fringe term, FINT = 1/2)."""
+    R = np.eye(6)
+    if L == 0:
+        return R
+    h = angle / L
+    kx2 = k1 + h * h
+    ky2 = -k1
+
+    def plane(k2):
+        k = math.sqrt(abs(k2))
+        ph = k * L
+        if abs(ph) < 1e-12:
+            return np.array([[1.0, L], [0.0, 1.0]]), None, None, None
+        if k2 > 0:
+            co, si = math.cos(ph), math.sin(ph)
+            return np.array([[co, si / k], [-k * si, co]]), co, si, k
+        co, si = math.cosh(ph), math.sinh(ph)
+        return np.array([[co, si / k], [k * si, co]]), co, si, k
+
+    Rx, co, si, kx = plane(kx2)
+    Ry = plane(ky2)[0]
+    R[0:2, 0:2] = Rx
+    R[2:4, 2:4] = Ry
+    if co is not None:
+        D0 = h * (1 - co) / kx2
+        D1 = h * si / kx
+        R[0, 5] = D0
+        R[1, 5] = D1
+        R[4, 0] = -D1
+        R[4, 1] = -D0
+        R[4, 5] = -(h ** 2) * (kx * L - kx * Rx[0, 1]) / kx ** 3
+    if angle:
+        def face(e):
+            F = np.eye(6)
+            psi = h * hgap * (1 + math.sin(e) ** 2) / math.cos(e)
+            F[1, 0] = math.tan(e) * h
+            F[3, 2] = -math.tan(e - psi) * h
+            return F
+        R = face(E2) @ R @ face(E1)
+    return R
+
 
 
 # ---------------------------------------------------------------------------
@@ -924,6 +980,128 @@ def apply_wakes(x, z, px, pz, dE_dct, x_kick, xrange, zrange, step_size, init_en
     return px_new, pz_new
 
 
-def formation_length_bend(R, sigma_z):
-    """CSR.py:140-142 (called with sigma_z = 5 * beam sigma_z, CSR.py:256,265)."""
+# ---------------------------------------------------------------------------
+# a17  run loop: step schedule, element pieces, formation length
+#      (CSR.py:140-144, 146-199, 202-358; lattice.py:152-173)
+# ---------------------------------------------------------------------------
+def lattice_steps(distance, step_size):
+    """(positions_record, steps_per_element) of Lattice.get_steps
+    (lattice.py:152-173): the step positions 0, ds, 2 ds, ... up to the
+    lattice end (arange with a half-step margin), and per element the number
+    of step positions in (distance[k-1], distance[k]] (position 0 excluded
+    from the first element)."""
+    positions = np.arange(0, distance[-1] + step_size / 2, step_size)
+    spe = np.zeros(len(distance), dtype=int)
+    prev = 0
+    for k, d in enumerate(distance):
+        ind = np.searchsorted(positions, d, side="right")
+        spe[k] = ind - prev - 1 if k == 0 else ind - prev
+        prev = ind
+    return positions, spe
+
+
+def formation_length(R, sigma_z):
+    """CSR.py:140-142 (in-bend form; the loop passes 5 * the beam's sigma_z)."""
     return (24 * (R ** 2) * sigma_z) ** (1 / 3)
+
+
+def element_piece(ele, DL, entrance=False, exit=False):
+    """The bmadx element a piece of length DL is tracked as (CSR.py:146-199),
+    as a dict: kind, L and, for a bend, G (the G key, else angle / L), E1,
+    E2 and FRINGE_AT -- the element's own E1/E2/FRINGE_AT when the piece is
+    the whole element, E1 and 'entrance_end' at an entrance piece, E2 and
+    'exit_end' at an exit piece, neither ('no_end') inside; a quadrupole's
+    K1, a sextupole's K2.  A bend tracked in one piece without a FRINGE_AT
+    key fails, as in the reference (CSR.py:177 reads it unbound)."""
+    typ = ele["type"]
+    if typ == "dipole":
+        G = ele["G"] if "G" in ele else ele["angle"] / ele["L"]
+        E1, E2 = ele.get("E1", 0), ele.get("E2", 0)
+        if entrance and exit:
+            if "FRINGE_AT" not in ele:
+                raise KeyError("FRINGE_AT: a bend tracked in one piece needs it (CSR.py:177)")
+            return dict(kind="SBend", L=DL, G=G, E1=E1, E2=E2, FRINGE_AT=ele["FRINGE_AT"])
+        if entrance:
+            return dict(kind="SBend", L=DL, G=G, E1=E1, E2=0.0, FRINGE_AT="entrance_end")
+        if exit:
+            return dict(kind="SBend", L=DL, G=G, E1=0.0, E2=E2, FRINGE_AT="exit_end")
+        return dict(kind="SBend", L=DL, G=G, E1=0.0, E2=0.0, FRINGE_AT="no_end")
+    if typ == "drift":
+        return dict(kind="Drift", L=DL)
+    if typ == "quad":
+        return dict(kind="Quadrupole", L=DL, K1=ele["K1"])
+    if typ == "sextupole":
+        return dict(kind="Sextupole", L=DL, K2=ele["K2"])
+    raise ValueError(f"element type {typ!r}")
+
+
+def run_loop(lattice_config, track, sigma_z, stop_time=None):
+    """The element/step loop of CSR2D.run (CSR.py:202-358) without the CSR
+    work: a generator yielding one dict per step (step, element, position,
+    formation_length -- the value append_interpolant receives --, csr: whether
+    the step computes the wake, step % nsep == 0).
+
+    lattice_config: ordered mapping, step_size first, then the elements.
+    track(piece, DL, update_step): moves the caller's beam through one piece
+    (a dict of element_piece); sigma_z(): the beam's sigma_z now.
+
+    Per element: the rest of the step that crossed into it is tracked in
+    the previous element (exit piece) unless that element was taken whole;
+    an element holding no step position is tracked whole (entrance and
+    exit); the formation length is set from 5 sigma_z in a bend and in
+    every element after the first bend (with the last bend's radius), and
+    accumulates element lengths before it; the first step in an element is
+    the part of the step past the element boundary (entrance piece)."""
+    items = list(lattice_config.items())
+    DL = items[0][1]
+    assert items[0][0] == "step_size"
+    names = [k for k, _ in items[1:]]
+    eles = [dict(v) for _, v in items[1:]]
+    distance = np.zeros(len(eles))
+    for k, e in enumerate(eles):
+        distance[k] = e["L"] if k == 0 else e["L"] + distance[k - 1]
+    positions, spe = lattice_steps(distance, DL)
+    nsep = np.array([float(e["nsep"]) for e in eles])
+    position = 0
+    step_count = 1
+    skip_ele = False
+    afterbend = False
+    R_rec = None
+    fl = 0.0
+    prev = None
+    for k, (name, e) in enumerate(zip(names, eles)):
+        L = e["L"]
+        steps = spe[k]
+        if (not skip_ele) and k > 0:
+            DL_1 = distance[k - 1] - position
+            if DL_1 > 1.0e-6:
+                track(element_piece(prev, DL_1, exit=True), DL_1, False)
+                position += DL_1
+        if steps == 0:
+            skip_ele = True
+            track(element_piece(e, L, entrance=True, exit=True), L, False)
+            position += L
+        if e["type"] == "dipole":
+            R = L / e["angle"]
+            afterbend = True
+            R_rec = R
+            fl = formation_length(R, 5 * sigma_z())
+        elif afterbend:
+            fl = formation_length(R_rec, 5 * sigma_z())
+        else:
+            fl += L
+        for step in range(steps):
+            if step == 0 and k > 0:
+                DL_2 = positions[step_count] - distance[k - 1]
+                track(element_piece(e, DL_2, entrance=True), DL_2, True)
+                position += DL_2
+                skip_ele = False
+            else:
+                track(element_piece(e, DL), DL, True)
+                position += DL
+            yield dict(step=step_count, element=name, position=position, formation_length=fl,
+                       csr=bool(step % nsep[k] == 0))
+            step_count += 1
+            if stop_time and position > stop_time:
+                return
+        prev = e

@@ -15,12 +15,10 @@ Tracking replaces bmadx's track_element (absent here) with the gfx950
 element kernel (dfcsr_track_element): exact drift, exact sector bend with
 hard-edge faces, quadrupole with second-order path length and low-energy z
 terms, sextupole drift-kick-drift; the drift, bend and quadrupole maps
-reproduce the reference notebook's bmadx results.  transfer_matrix is the
-linear (r_gen6.py) map of a piece, kept for linear optics checks.
+reproduce the reference notebook's bmadx results.
 """
 from __future__ import annotations
 
-import math
 
 import numpy as np
 import torch
@@ -32,55 +30,15 @@ from . import _lib, h5io, kernels
 MC2 = 510998.95069
 
 
-def transfer_matrix(L, angle=0.0, k1=0.0, E1=0.0, E2=0.0, hgap=0.0):
-    """6x6 linear map (x, px, y, py, z, pz) of a bend/quad/drift piece."""
-    R = np.eye(6)
-    if L == 0:
-        return R
-    h = angle / L
-    kx2 = k1 + h * h
-    ky2 = -k1
-
-    def plane(k2):
-        k = math.sqrt(abs(k2))
-        ph = k * L
-        if abs(ph) < 1e-12:
-            return np.array([[1.0, L], [0.0, 1.0]]), None, None, None
-        if k2 > 0:
-            co, si = math.cos(ph), math.sin(ph)
-            return np.array([[co, si / k], [-k * si, co]]), co, si, k
-        co, si = math.cosh(ph), math.sinh(ph)
-        return np.array([[co, si / k], [k * si, co]]), co, si, k
-
-    Rx, co, si, kx = plane(kx2)
-    Ry = plane(ky2)[0]
-    R[0:2, 0:2] = Rx
-    R[2:4, 2:4] = Ry
-    if co is not None:
-        D0 = h * (1 - co) / kx2
-        D1 = h * si / kx
-        R[0, 5] = D0
-        R[1, 5] = D1
-        R[4, 0] = -D1
-        R[4, 1] = -D0
-        R[4, 5] = -(h ** 2) * (kx * L - kx * Rx[0, 1]) / kx ** 3
-    if angle:
-        def face(e):
-            F = np.eye(6)
-            psi = h * hgap * (1 + math.sin(e) ** 2) / math.cos(e)
-            F[1, 0] = math.tan(e) * h
-            F[3, 2] = -math.tan(e - psi) * h
-            return F
-        R = face(E2) @ R @ face(E1)
-    return R
-
-
 def bmadx_element(ele: dict, DL: float, entrance=False, exit=False) -> _lib.Element:
     """The element of a step piece of length DL, as CSR2D.get_bmadx_element
     builds it for bmadx (CSR.py:146-199): a dipole's G (G key, else angle/L),
     its faces E1 / E2 and their fringe (FINT, HGAP / FINTX, HGAPX) only at the
-    piece that holds them; a quad's K1, a sextupole's K2, with X_OFFSET,
-    Y_OFFSET, TILT and NUM_STEPS when given."""
+    piece that holds them (FRINGE_AT entrance_end / exit_end); a dipole taken
+    in one piece keeps the faces its own FRINGE_AT names (both_ends,
+    entrance_end, exit_end or no_end) and, as in the reference, must have
+    that key; a quad's K1, a sextupole's K2, with X_OFFSET, Y_OFFSET, TILT
+    and NUM_STEPS when given."""
     typ = ele["type"]
     e = _lib.Element()
     e.L = float(DL)
@@ -88,6 +46,15 @@ def bmadx_element(ele: dict, DL: float, entrance=False, exit=False) -> _lib.Elem
     if typ == "dipole":
         e.type = _lib.DFCSR_ELE_SBEND
         e.g = float(ele["G"]) if "G" in ele else float(ele["angle"]) / float(ele["L"])
+        if entrance and exit:
+            if "FRINGE_AT" not in ele:
+                raise ValueError("a dipole tracked in one piece needs FRINGE_AT (the reference "
+                                 "reads it unbound otherwise, CSR.py:177)")
+            at = ele["FRINGE_AT"]
+            if at not in ("both_ends", "entrance_end", "exit_end", "no_end"):
+                raise ValueError(f"FRINGE_AT {at!r}")
+            entrance = at in ("both_ends", "entrance_end")
+            exit = at in ("both_ends", "exit_end")
         if entrance:
             e.e1 = float(ele.get("E1", 0.0))
             e.fint = float(ele.get("FINT", 0.0) or 0.0)

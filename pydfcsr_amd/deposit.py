@@ -92,8 +92,8 @@ class DF_tracker:
         self.hist = None      # float64 (Xi, Zi, cap, 20)
         self.hist32 = None    # float32 chunked (Xi, cap, 5, Zi, 4)
         self.hist32t = None   # float32 time-innermost (Xi, Zi, cap, 20)
-        self.cells = None     # float64 cell-major (cap, Xi, Zi, 6)
-        self.cells32 = None   # float32 cell-major (cap, Xi, Zi, 6)
+        self.cells = None     # float64 cell-major (cap, Xi, Zi, 5)
+        self.cells32 = None   # float32 cell-major (cap, Xi, Zi, 5)
         self.cur = None
         self.n_regrid = 0
         self.info = torch.zeros(4, dtype=torch.float64, device=self.device)

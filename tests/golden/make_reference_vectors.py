@@ -2,10 +2,10 @@
 
 Run ONLY in the build container, where the reference source tree is mounted:
 
-    python tests/golden/make_reference_vectors.py /root/reference/pyDFCSR_2D [r_gen6|sgolay2d]
+    python tests/golden/make_reference_vectors.py /root/reference/pyDFCSR_2D [r_gen6|sgolay2d|run_loop]
 
-(with ``r_gen6`` / ``sgolay2d`` only ``ref_r_gen6.npz`` / ``ref_sgolay2d.npz``
-is written)
+(with ``r_gen6`` / ``sgolay2d`` / ``run_loop`` only ``ref_r_gen6.npz`` /
+``ref_sgolay2d.npz`` / ``ref_run_loop.npz`` is written)
 
 The reference package cannot be imported as shipped (numba, bmadx, mpi4py,
 h5py, distgen and pmd_beamphysics are absent).  The functions captured here
@@ -331,15 +331,224 @@ def sgolay_vectors(ref_dir):
     print("sgolay2d vectors written to", HERE)
 
 
+SHORT_LATTICE_YAML = """step_size: 0.05
+B0:
+  type: dipole
+  L: 0.03
+  angle: 0.01
+  E1: 0.01
+  E2: 0.0
+  FRINGE_AT: both_ends
+  nsep: 1
+D1:
+  type: drift
+  L: 0.2
+  nsep: 3
+Q1:
+  type: quad
+  L: 0.02
+  K1: 2.0
+  nsep: 1
+B1:
+  type: dipole
+  L: 0.25
+  angle: -0.03
+  E1: -0.01
+  E2: -0.02
+  nsep: 2
+S1:
+  type: sextupole
+  L: 0.04
+  K2: 1.0
+  nsep: 1
+D2:
+  type: drift
+  L: 0.31
+  nsep: 1
+"""
+
+
+def _c2_chicane_yaml():
+    """The 4-bend chicane of the bench's chicane workload: 0.3 m bends at
+    +, -, -, + 0.05 rad, drifts 2/1/2 m, 400 steps."""
+    rows = [("B1", "dipole", 0.3, 0.05, 0.0, 0.05), ("D1", "drift", 2.0),
+            ("B2", "dipole", 0.3, -0.05, -0.05, 0.0), ("D2", "drift", 1.0),
+            ("B3", "dipole", 0.3, -0.05, 0.0, -0.05), ("D3", "drift", 2.0),
+            ("B4", "dipole", 0.3, 0.05, 0.05, 0.0)]
+    out = [f"step_size: {6.2 / 400!r}"]
+    for r in rows:
+        out += [f"{r[0]}:", f"  type: {r[1]}", f"  L: {r[2]!r}", "  nsep: 1"]
+        if r[1] == "dipole":
+            out += [f"  angle: {r[3]!r}", f"  E1: {r[4]!r}", f"  E2: {r[5]!r}"]
+    return "\n".join(out) + "\n"
+
+
+# (tag, lattice source, n_formation_length): a leading drift and nsep 2; the
+# reference's FODO example (a first element and two consecutive elements
+# shorter than one step, no bend: the formation length accumulates every
+# element); the reference's chicane example (leading drift, nsep 5,
+# n_formation_length 1); the chicane workload (400 steps); a lattice with a
+# bend shorter than one step (tracked in one piece with its FRINGE_AT) and a
+# short quadrupole and sextupole; the reference's dipole example (a 1 rad
+# bend with optional fringe keys).
+RUN_LOOP_CASES = [("drift_first", LATTICE_YAML, 1.0),
+                  ("fodo", "example/input/fodo_lattice.yaml", 4),
+                  ("chicane_example", "example/input/chicane_lattice.yaml", 1),
+                  ("chicane_c2", None, 4),
+                  ("short", SHORT_LATTICE_YAML, 2),
+                  ("dipole_example", "example/input/dipole_lattice.yaml", 4)]
+
+
+def run_loop_vectors(ref_dir):
+    """The reference's own run loop (CSR.py:202-358) on RUN_LOOP_CASES with
+    bmadx's element classes replaced by records and track_element by the
+    stand-in map of run_loop_tracker.py; the wake itself (calculate_2D_CSR)
+    replaced by a recorder and apply_CSR off, so only the loop's decisions
+    are captured.  Per lattice: every tracked piece (kind, L, G, E1, E2,
+    FRINGE_AT, K1, K2) with sigma_z after it; per step: position, the
+    formation length passed to append_interpolant, the history window's
+    first time and length after it, whether CSR ran, pieces so far.
+    -> ref_run_loop.npz"""
+    import contextlib
+    import io
+    import json
+    sys.path.insert(0, HERE)
+    from run_loop_tracker import make_beam, piece_row, stand_in_map
+    R = _load_reference(ref_dir)
+    CSR, beams, ypar = R["CSR"], R["beams"], R["yaml_parser"]
+    Particle = sys.modules["bmadx"].Particle
+    rec = {}
+
+    def element(kind):
+        def make(**kw):
+            d = dict(kind=kind, **kw)
+            d.pop("P0C", None)
+            return d
+        return make
+
+    def track_element(p, ele):
+        new = stand_in_map((p.x, p.px, p.y, p.py, p.z, p.pz), ele)
+        rec["pieces"].append(piece_row(ele))
+        rec["sigz"].append(np.std(new[4]))
+        return Particle(*new, p.s, p.p0c, p.mc2)
+
+    for kind in ("Drift", "SBend", "Quadrupole", "Sextupole"):
+        setattr(CSR, kind, element(kind))
+    beams.track_element = track_element
+
+    tmp = tempfile.mkdtemp()
+    beam = make_beam()
+    beam_file = os.path.join(tmp, "beam.txt")
+    np.savetxt(beam_file, beam.T, fmt="%.17g")
+    beam = np.loadtxt(beam_file).T.copy()
+    out = {"beam": beam}
+    for tag, src, nfl in RUN_LOOP_CASES:
+        if src is None:
+            src = _c2_chicane_yaml()
+        lat_file = os.path.join(tmp, f"{tag}.yaml")
+        if src.endswith(".yaml"):
+            with open(os.path.join(ref_dir, src)) as f:
+                src = f.read()
+        with open(lat_file, "w") as f:
+            f.write(src)
+        lat_cfg = ypar.parse_yaml(lat_file)
+        cfg_file = os.path.join(tmp, f"{tag}_cfg.yaml")
+        with open(cfg_file, "w") as f:
+            f.write(f"""input_beam:
+  style: from_file
+  beamfile: {beam_file}
+  charge: 1.0e-9
+  energy: 1.0e+9
+input_lattice:
+  lattice_input_file: {lat_file}
+particle_deposition:
+  xbins: 32
+  zbins: 32
+  xlim: 5
+  zlim: 5
+  filter_order: 2
+  filter_window: 5
+  velocity_threhold: 1000
+  upper_limit: 48
+CSR_integration:
+  n_formation_length: {nfl}
+  zbins: 8
+  xbins: 8
+CSR_computation:
+  compute_CSR: 1
+  apply_CSR: 0
+  transverse_on: 1
+  xbins: 4
+  zbins: 4
+  write_wakes: False
+  workdir: {tmp}
+""")
+        rec.update(pieces=[], sigz=[], steps=[], fl=None, csr=False)
+        obj = CSR.CSR2D.__new__(CSR.CSR2D)
+        obj.timestamp = "fixture"
+
+        def append_interpolant(formation_length, n_formation_length, _orig=None):
+            rec["fl"] = formation_length
+            return _orig(formation_length=formation_length,
+                         n_formation_length=n_formation_length)
+
+        def calculate_2D_CSR(self=obj):
+            rec["csr"] = True
+            self.dE_dct = np.zeros((self.CSR_params.xbins, self.CSR_params.zbins))
+            self.x_kick = np.zeros_like(self.dE_dct)
+
+        def update_statistics(step, self=obj):
+            tr = self.DF_tracker
+            rec["steps"].append([step, float(self.beam.position),
+                                 np.nan if rec["fl"] is None else rec["fl"],
+                                 tr.time_interp[0], len(tr.time_interp), float(rec["csr"]),
+                                 len(rec["pieces"])])
+            rec["fl"] = None
+            rec["csr"] = False
+
+        obj.update_statistics = update_statistics
+        obj.calculate_2D_CSR = calculate_2D_CSR
+        obj.dump_beam = lambda label: None
+        obj.write_statistics = lambda: None
+        with contextlib.redirect_stdout(io.StringIO()):
+            obj.parse_input(cfg_file)
+            tr = obj.DF_tracker
+            tr.append_interpolant = (
+                lambda formation_length, n_formation_length, _o=tr.append_interpolant:
+                append_interpolant(formation_length, n_formation_length, _o))
+            obj.formation_length = None
+            obj.initialization()
+            obj.prefix = "fixture"
+            obj.parallel = False
+            obj.run()
+        lat = obj.lattice
+        out[f"{tag}_lattice_json"] = np.array(json.dumps(
+            [["step_size", lat_cfg["step_size"]]] +
+            [[k, dict(v)] for k, v in list(lat_cfg.items())[1:]]))
+        out[f"{tag}_nfl"] = np.float64(nfl)
+        out[f"{tag}_pieces"] = np.array(rec["pieces"], dtype=np.float64)
+        out[f"{tag}_sigz"] = np.array(rec["sigz"], dtype=np.float64)
+        out[f"{tag}_steps"] = np.array(rec["steps"], dtype=np.float64)
+        out[f"{tag}_positions_record"] = lat._positions_record
+        out[f"{tag}_steps_per_element"] = lat.steps_per_element
+        print(tag, len(rec["pieces"]), "pieces", len(rec["steps"]), "step records")
+    np.savez_compressed(os.path.join(HERE, "ref_run_loop.npz"), **out)
+    print("run-loop vectors written to", HERE)
+
+
 if __name__ == "__main__":
-    # make_reference_vectors.py [REF_DIR] [r_gen6 | sgolay2d]: every
-    # fixture, or only the transfer matrices / the 2D filter's
+    # make_reference_vectors.py [REF_DIR] [r_gen6 | sgolay2d | run_loop]:
+    # every fixture, or only the transfer matrices / the 2D filter's / the
+    # run loop's
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference/pyDFCSR_2D"
     if sys.argv[2:] == ["r_gen6"]:
         transfer_matrices(ref)
     elif sys.argv[2:] == ["sgolay2d"]:
         sgolay_vectors(ref)
+    elif sys.argv[2:] == ["run_loop"]:
+        run_loop_vectors(ref)
     else:
         main(ref)
         transfer_matrices(ref)
         sgolay_vectors(ref)
+        run_loop_vectors(ref)

@@ -129,3 +129,50 @@ def rel_max(a, b):
     if den == 0:
         return float(np.nanmax(np.abs(a - b)))
     return float(np.nanmax(np.abs(a - b)) / den)
+
+
+class LoopReplay:
+    """The oracle's run loop (CSR.py:202-358) run beside a CSR2D driver.
+
+    Wraps sim._track so every piece the driver tracks is recorded with the
+    beam's sigma_z after it (torch's population std of z on the device, an
+    independent reduction); the oracle's run_loop is then advanced over those
+    pieces: each piece it asks for must be the one the driver tracked (same
+    element dict, same length bit for bit, same update_step), and it sets its
+    own formation length from the recorded sigma_z.  step(k) returns the
+    oracle's record of step k (step, element, position, formation_length,
+    csr) once the driver has tracked that step."""
+
+    def __init__(self, sim, lattice_config):
+        from collections import deque
+        from oracle import cpu_ref as O
+        self.pending = deque()
+        self.records = {}
+        self.pieces = 0
+        cfg = lattice_config
+        orig = sim._track
+
+        def sigma_z():
+            return float(torch.std(sim.beam.z, correction=0))
+
+        def track(ele_name, DL, entrance=False, exit=False, update_step=True):
+            orig(ele_name, DL, entrance=entrance, exit=exit, update_step=update_step)
+            self.pending.append((O.element_piece(dict(cfg[ele_name]), DL, entrance, exit), DL,
+                                 update_step, sigma_z()))
+
+        sim._track = track
+        self.sz = sigma_z()
+        self.gen = O.run_loop(cfg, self._track, lambda: self.sz)
+
+    def _track(self, piece, DL, update_step):
+        assert self.pending, f"the oracle's loop tracks {piece}, the driver tracked nothing"
+        got, gDL, gup, sz = self.pending.popleft()
+        assert got == piece and gDL == DL and gup == update_step, (got, piece, gDL, DL)
+        self.sz = sz
+        self.pieces += 1
+
+    def step(self, k):
+        while k not in self.records:
+            r = next(self.gen)
+            self.records[r["step"]] = r
+        return self.records[k]

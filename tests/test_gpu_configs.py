@@ -53,9 +53,12 @@ def test_dipole_full_run_fp64_and_mixed_vs_oracle():
     from pydfcsr_amd import _lib, kernels
     from pydfcsr_amd.csr import CSR2D
 
+    from tests.gpu_util import LoopReplay
+
     cfg = dipole_config()
     dep, ip, cp = cfg["particle_deposition"], cfg["CSR_integration"], cfg["CSR_computation"]
     sim = CSR2D(config=cfg, device=DEV, precision="fp64", keep_fp32=True)
+    loop = LoopReplay(sim, cfg["input_lattice"]["_config"])
     ora = O.DFTracker(xbins=dep["xbins"], zbins=dep["zbins"], xlim=dep["xlim"],
                       zlim=dep["zlim"], filter_order=dep["filter_order"],
                       filter_window=dep["filter_window"],
@@ -73,9 +76,11 @@ def test_dipole_full_run_fp64_and_mixed_vs_oracle():
             continue
         x, z, px = _np(b.x), _np(b.z), _np(b.px)
         t = float(b.position)
+        fl = loop.step(step)["formation_length"]
+        assert abs(fl - sim.formation_length) <= 1e-13 * fl, (step, fl, sim.formation_length)
         ora.get_DF(x, z, px, t)
         ora.append_DF()
-        ora.append_interpolant(sim.formation_length, ip["n_formation_length"])
+        ora.append_interpolant(fl, ip["n_formation_length"])
         H, L, P, out = sim.prepare_wake()
         kernels.wake_mesh(H, L, P, _lib.DFCSR_PREC_FP64, sim.device, sim.ws, out=out,
                           want_mesh=False)
@@ -94,7 +99,7 @@ def test_dipole_full_run_fp64_and_mixed_vs_oracle():
         beam = dict(sigma_x=m["sigma_x"], sigma_z=m["sigma_z"], slope=m["slope"],
                     mean_x=m["mean_x"])
         obs = np.stack([mesh["zmesh"], mesh["xmesh"]], axis=1)
-        dE, xk = C.wake(hist, latc, t, beam, sim.formation_length, ip["n_formation_length"],
+        dE, xk = C.wake(hist, latc, t, beam, fl, ip["n_formation_length"],
                         ip["xbins"], ip["zbins"], sim.CSR_scaling, obs, C.threads())
         inside = t + mesh["zmesh"] <= lat.s[-1]
         for name, res in (("fp64", out), ("mixed", mixed)):
@@ -121,8 +126,11 @@ def test_large_run_sampled_mesh_points_vs_oracle():
     from pydfcsr_amd import _lib, kernels
     from pydfcsr_amd.csr import CSR2D
 
+    from tests.gpu_util import LoopReplay
+
     cfg = large_config()
     sim = CSR2D(config=cfg, device=DEV, precision="fp64", keep_fp32=True)
+    loop = LoopReplay(sim, cfg["input_lattice"]["_config"])
     sim.run(max_steps=100)
     torch.cuda.synchronize()
     tr = sim.DF_tracker
@@ -152,7 +160,11 @@ def test_large_run_sampled_mesh_points_vs_oracle():
     beam = dict(sigma_x=b._sigma_x, sigma_z=b._sigma_z, slope=b._slope, mean_x=b._mean_x)
     obs = np.stack([_np(fp64["zmesh"])[rows], _np(fp64["xmesh"])[rows]], axis=1)
     ip = cfg["CSR_integration"]
-    dE, xk = C.wake(hist, latc, P.t, beam, P.formation_length, P.n_formation_length, ip["xbins"],
+    rec = loop.step(sim.beam.step)
+    assert rec["position"] == P.t
+    fl = rec["formation_length"]
+    assert abs(fl - P.formation_length) <= 1e-13 * fl, (fl, P.formation_length)
+    dE, xk = C.wake(hist, latc, P.t, beam, fl, P.n_formation_length, ip["xbins"],
                     ip["zbins"], P.csr_scaling, obs, C.threads())
     report = {}
     for name, res in (("fp64", fp64), ("mixed", mixed)):
@@ -178,8 +190,11 @@ def test_chicane_run_sampled_mesh_points_vs_oracle():
     from pydfcsr_amd import _lib, kernels
     from pydfcsr_amd.csr import CSR2D
 
+    from tests.gpu_util import LoopReplay
+
     cfg = chicane_config()
     sim = CSR2D(config=cfg, device=DEV, precision="fp64", keep_fp32=True)
+    loop = LoopReplay(sim, cfg["input_lattice"]["_config"])
     sim.run(max_steps=200)
     torch.cuda.synchronize()
     tr = sim.DF_tracker
@@ -207,7 +222,11 @@ def test_chicane_run_sampled_mesh_points_vs_oracle():
     beam = dict(sigma_x=b._sigma_x, sigma_z=b._sigma_z, slope=b._slope, mean_x=b._mean_x)
     obs = np.stack([_np(fp64["zmesh"])[rows], _np(fp64["xmesh"])[rows]], axis=1)
     ip = cfg["CSR_integration"]
-    dE, xk = C.wake(hist, latc, P.t, beam, P.formation_length, P.n_formation_length, ip["xbins"],
+    rec = loop.step(sim.beam.step)
+    assert rec["position"] == P.t
+    fl = rec["formation_length"]
+    assert abs(fl - P.formation_length) <= 1e-13 * fl, (fl, P.formation_length)
+    dE, xk = C.wake(hist, latc, P.t, beam, fl, P.n_formation_length, ip["xbins"],
                     ip["zbins"], P.csr_scaling, obs, C.threads())
     report = {}
     for name, res in (("fp64", fp64), ("mixed", mixed), ("mixed_fp32_rings", mixed32)):

@@ -1,9 +1,13 @@
 """Driver-level parity: CSR2D's run loop (CSR.py:202-358) against the oracle,
-step by step, on a short dipole + drift lattice.
+step by step, on a short drift + dipole + drift lattice.
 
 Tracking is pinned separately (tests of dfcsr_track_element against the
 reference notebook's bmadx results), so at every step the oracle is fed the
-particles the driver tracked: after
+particles the driver tracked, and the formation length comes from the
+oracle's own run loop (tests/gpu_util.LoopReplay: the oracle's loop replays
+the driver's pieces, checks each one and sets the formation length from the
+beam's sigma_z at each element entrance, including the leading drift's
+accumulation): after
 the driver's history update (external_wake hook) the oracle repeats
 get_DF -> append_DF -> append_interpolant -> build_interpolant
 (deposit.py:145-426) on the same particles, the mesh (CSR.py:361-394), the
@@ -24,6 +28,7 @@ DEV = "cuda"
 
 def tiny_config(n=20000):
     lat = OrderedDict(step_size=0.005)
+    lat["D0"] = dict(type="drift", L=0.01, nsep=1)
     lat["B1"] = dict(type="dipole", L=0.05, angle=0.01, E1=0.0, E2=0.0, nsep=1)
     lat["D1"] = dict(type="drift", L=0.03, nsep=1)
     return {
@@ -64,9 +69,11 @@ def test_run_loop_matches_oracle_step_by_step():
     from oracle import cpu_ref as O
     from pydfcsr_amd import kernels
     from pydfcsr_amd.csr import CSR2D
+    from tests.gpu_util import LoopReplay
 
     cfg = tiny_config()
     sim = CSR2D(config=cfg, device=DEV, precision="fp64")
+    loop = LoopReplay(sim, cfg["input_lattice"]["_config"])
     dep = cfg["particle_deposition"]
     ip, cp = cfg["CSR_integration"], cfg["CSR_computation"]
     ora = O.DFTracker(xbins=dep["xbins"], zbins=dep["zbins"], xlim=dep["xlim"],
@@ -89,9 +96,13 @@ def test_run_loop_matches_oracle_step_by_step():
         if kind == "wake":
             x, z, px, pz = _np(b.x), _np(b.z), _np(b.px), _np(b.pz)
             t = float(b.position)
+            rec = loop.step(step)
+            assert rec["position"] == t and rec["csr"], (step, rec, t)
+            fl = rec["formation_length"]
+            assert abs(fl - sim.formation_length) <= 1e-13 * fl, (step, fl, sim.formation_length)
             ora.get_DF(x, z, px, t)
             ora.append_DF()
-            ora.append_interpolant(sim.formation_length, ip["n_formation_length"])
+            ora.append_interpolant(fl, ip["n_formation_length"])
             ora.build_interpolant()
             tr = sim.DF_tracker
             assert tr.nt == len(ora.time_interp), (step, tr.nt, len(ora.time_interp))
@@ -114,7 +125,7 @@ def test_run_loop_matches_oracle_step_by_step():
             m = O.beam_moments(x, z)
             beam = dict(sigma_x=m["sigma_x"], sigma_z=m["sigma_z"], slope=m["slope"],
                         mean_x=m["mean_x"])
-            dE, xk = O.calculate_2D_CSR(t, mesh, beam, sim.formation_length,
+            dE, xk = O.calculate_2D_CSR(t, mesh, beam, fl,
                                         ip["n_formation_length"], ip["xbins"], ip["zbins"], lat,
                                         ora, sim.CSR_scaling, cp["xbins"], cp["zbins"])
             e = max(_rel(_np(out["dE_dct"]).reshape(dE.shape), dE),
@@ -149,8 +160,11 @@ def test_run_loop_matches_oracle_step_by_step():
             worst["kick"] = max(worst["kick"], e)
             assert e <= 1e-8, (step, e)
             pending = None
-    # positions 0, 0.005, ..., 0.08: 16 steps (total_steps counts position 0 too)
-    assert steps == int(sum(sim.lattice.steps_per_element)) == sim.lattice.total_steps - 1 == 16
+    # positions 0, 0.005, ..., 0.09: 18 steps (total_steps counts position 0 too)
+    assert steps == int(sum(sim.lattice.steps_per_element)) == sim.lattice.total_steps - 1 == 18
+    assert loop.step(steps)["step"] == steps and not loop.pending
+    # the leading drift's steps used the accumulated drift length (CSR.py:268-269)
+    assert loop.step(1)["formation_length"] == 0.01
     print("worst relative differences over", steps, "steps:", worst)
 
 
@@ -187,7 +201,8 @@ def test_run_writes_the_reference_output_files(tmp_path):
         assert np.array_equal(xk, rec[k][1], equal_nan=True), k
         assert w[g + "/longitudinal"]["attrs"]["unit"]["value"] == {"str": "MeV/m"}
     assert w[f"step_{steps[-1]}"]["attrs"]["element"]["value"] == {"str": "D1"}
-    assert w[f"step_{steps[0]}"]["attrs"]["element"]["value"] == {"str": "B1"}
+    assert w[f"step_{steps[0]}"]["attrs"]["element"]["value"] == {"str": "D0"}
+    assert w[f"step_{steps[2]}"]["attrs"]["element"]["value"] == {"str": "B1"}
     # the mesh of the last step, as get_CSR_mesh lays it out (x outer)
     xg = np.array(w[f"step_{steps[-1]}/longitudinal/x_grids"]["data"])
     assert np.allclose(xg.ravel(), _np(sim.CSR_xmesh), rtol=0, atol=1e-15)

@@ -337,10 +337,10 @@ def _r_gen6_vectors():
 
 
 def test_transfer_matrix_matches_r_gen6_vectors():
-    """The host's linear map equals the reference's r_gen6 (r_gen6.py:4-122)
+    """The oracle's linear map equals the reference's r_gen6 (r_gen6.py:4-122)
     on every unrolled case of tests/golden/ref_r_gen6.npz, pole faces and the
     fringe term included."""
-    from pydfcsr_amd.beams import transfer_matrix
+    from oracle.cpu_ref import transfer_matrix
     cases, mats = _r_gen6_vectors()
     n = 0
     for (L, angle, k1, roll, e1, e2, hgap), R in zip(cases, mats):

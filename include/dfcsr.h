@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define DFCSR_ABI_VERSION 4
+#define DFCSR_ABI_VERSION 5
 
 enum {
   DFCSR_OK = 0,
@@ -180,9 +180,10 @@ int dfcsr_sgolay2d_kernels(int window_size, int order, int derivative, double *k
  * src: raw slices, src_slot_stride doubles apart, each holding the five
  * fields as [field][nbx][nbz] (field order DFCSR_F_*).
  * jobs: HOST array of njobs descriptors; hist64 (FP64), hist32 (FP32
- * chunked) and hist32t (FP32 time-innermost) are rings of cap slots in the
- * layouts above (dst_slot < cap), cells and cells32 the cell-major rings of
- * DfcsrHistory; each may be NULL, but not all five. */
+ * chunked) and hist32t (FP32 slot-innermost) are rings of cap slots in the
+ * layouts of DfcsrHistory (dst_slot < cap; writing slot 0 also writes its
+ * copy, slot cap, of the slot-innermost rings), cells and cells32 the
+ * cell-major rings; each may be NULL, but not all five. */
 typedef struct {
   int32_t src_slot;   /* raw slice index in src */
   int32_t dst_slot;   /* ring slot in the history */
@@ -202,7 +203,14 @@ int dfcsr_regrid(const double *src, int64_t src_slot_stride, const DfcsrRegridJo
  * dx = (xi1 - xi0)/(Xi - 1), same for z.  The FP64 ring is needed by the
  * lookup primitives, the integrand and FP64 wakes; a mixed-precision wake
  * needs at least one FP32 ring (with both, its region-0 phases read the
- * time-innermost one and the other phases the chunked one).
+ * slot-innermost one and the other phases the chunked one).
+ * The slot-innermost rings (data, data32t) hold, per grid node (x, z), the
+ * corner-packed values v = 4 f + c (field f at (x, z), (x, z+1), (x+1, z),
+ * (x+1, z+1), clamped to the last row and column) with the slot fastest:
+ * value v of slot s at ((x Zi + z) DFCSR_HIST_REC64 + v) (cap + 1) + s, and
+ * slot cap a copy of slot 0, so the two planes of a lookup are neighbouring
+ * values and lookups of neighbouring planes by neighbouring lanes are
+ * neighbouring addresses.
  * The cell-major rings (optional) hold the same slices plane by plane:
  * cell (slot, x, z) at ((slot Xi + x) Zi + z) DFCSR_CELL, the five fields
  * of that grid node (40 B in float64, 20 B in float32, against 160 B / 80 B
@@ -212,9 +220,9 @@ int dfcsr_regrid(const double *src, int64_t src_slot_stride, const DfcsrRegridJo
  * One plane must span at most 4 GiB (the wakes address cells within a plane
  * with 32-bit offsets). */
 typedef struct {
-  const double *data;    /* [Xi][Zi][cap][DFCSR_HIST_REC64] or NULL */
+  const double *data;    /* [Xi][Zi][DFCSR_HIST_REC64][cap + 1] or NULL */
   const float *data32;   /* [Xi][cap][DFCSR_HIST_CHUNKS32][Zi][4] or NULL */
-  const float *data32t;  /* [Xi][Zi][cap][DFCSR_HIST_REC32] or NULL */
+  const float *data32t;  /* [Xi][Zi][DFCSR_HIST_REC32][cap + 1] or NULL */
   const double *cells;   /* [cap][Xi][Zi][DFCSR_CELL] or NULL */
   const float *cells32;  /* [cap][Xi][Zi][DFCSR_CELL] or NULL */
   int32_t cap, head, nt, Xi, Zi;
@@ -257,6 +265,12 @@ typedef struct {
    * replaces the mesh formula, e.g. for get_CSR_wake(s, x) at arbitrary
    * points (CSR.py:454). */
   const double *observers;
+  /* Optional DEVICE counter: when non-NULL the wake adds to it the number of
+   * integrand points it evaluated (the points whose retarded lookup lies in
+   * the history; the rest contribute exactly 0 and are skipped).  A
+   * diagnostic: it does not change the wake.  No counterpart in the
+   * reference, which evaluates every point. */
+  unsigned long long *points_evaluated;
 } DfcsrWakeParams;
 
 /* get_CSR_integrand (CSR.py:605-782) on an explicit (xp, sp) grid of one

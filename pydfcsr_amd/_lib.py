@@ -20,7 +20,7 @@ c_i32 = ctypes.c_int32
 c_i64 = ctypes.c_int64
 c_f64 = ctypes.c_double
 
-ABI_VERSION = 4   # DFCSR_ABI_VERSION
+ABI_VERSION = 5   # DFCSR_ABI_VERSION
 DFCSR_PREC_FP64 = 0
 DFCSR_PREC_MIXED = 1
 MOMENTS_COUNT = 10
@@ -81,7 +81,8 @@ class WakeParams(ctypes.Structure):
                 ("slope1", c_f64), ("mean_x", c_f64), ("formation_length", c_f64),
                 ("n_formation_length", c_f64), ("csr_scaling", c_f64), ("Nx", c_i32),
                 ("Nz", c_i32), ("mesh_xbins", c_i32), ("mesh_zbins", c_i32), ("xr0", c_f64),
-                ("xr1", c_f64), ("zr0", c_f64), ("zr1", c_f64), ("observers", c_dp)]
+                ("xr1", c_f64), ("zr0", c_f64), ("zr1", c_f64), ("observers", c_dp),
+                ("points_evaluated", c_dp)]
 
 
 # (name, restype, argtypes) for every exported symbol of include/dfcsr.h

@@ -249,10 +249,15 @@ __device__ inline const float4 *hist_rec32(const DfcsrHistory &H, int ti, int xi
   return reinterpret_cast<const float4 *>(H.data32) + blk * H.Zi + zi;
 }
 
-// The same record in the time-innermost ring: five consecutive chunks.
-__device__ inline const float4 *hist_rec32t(const DfcsrHistory &H, int ti, int xi, int zi) {
-  const int64_t rec = ((int64_t)xi * H.Zi + zi) * H.cap + hist_slot(H, ti);
-  return reinterpret_cast<const float4 *>(H.data32t) + rec * DFCSR_HIST_CHUNKS32;
+// Field k's four corners at planes t0 (a) and t1 (b) from the FP32
+// slot-innermost ring (the same layout as the FP64 one: value 4 k + c of a
+// node at stride cap + 1, the next plane at the next slot).  p: value 0 of
+// the node at the slot of t0; same_plane: i1 == i0 (b = a).
+__device__ inline void ring32_corners(const float *p, int64_t vs, int k, bool same_plane,
+                                      float4 &a, float4 &b) {
+  const float *q = p + (int64_t)(4 * k) * vs;
+  a = make_float4(q[0], q[vs], q[2 * vs], q[3 * vs]);
+  b = same_plane ? a : make_float4(q[1], q[vs + 1], q[2 * vs + 1], q[3 * vs + 1]);
 }
 
 // vx at (tq, xq, zq) from an FP32 ring (the time-innermost one when present),
@@ -265,8 +270,8 @@ __device__ inline double lookup_vx32(const DfcsrHistory &H, const HistAxes &A, d
   if (!(T.ok && X.ok && Z.ok)) return 0.0;
   float4 a, b;
   if (H.data32t) {
-    a = hist_rec32t(H, T.i0, X.i0, Z.i0)[DFCSR_F_VX];
-    b = hist_rec32t(H, T.i1, X.i0, Z.i0)[DFCSR_F_VX];
+    ring32_corners(H.data32t + hist_node_index(H, T.i0, X.i0, Z.i0), (int64_t)H.cap + 1,
+                   DFCSR_F_VX, T.i1 == T.i0, a, b);
   } else {
     a = hist_rec32(H, T.i0, X.i0, Z.i0)[(int64_t)DFCSR_F_VX * H.Zi];
     b = hist_rec32(H, T.i1, X.i0, Z.i0)[(int64_t)DFCSR_F_VX * H.Zi];
@@ -363,13 +368,21 @@ __device__ inline bool csr_point_mixed(const DfcsrHistory &H, const HistAxes &A,
       }
     }
   } else {
-    // one corner-packed record per time plane, one 16-byte chunk per field
-    const float4 *q0 = kTI ? hist_rec32t(H, T.i0, xi0, Z.i0) : hist_rec32(H, T.i0, xi0, Z.i0);
-    const float4 *q1 = kTI ? hist_rec32t(H, T.i1, xi0, Z.i0) : hist_rec32(H, T.i1, xi0, Z.i0);
-    const int64_t cs = kTI ? 1 : H.Zi;
+    // the corner-packed values of node (x0, z0) at both planes: from the
+    // slot-innermost ring (neighbouring slots) or the chunked one (one
+    // 16-byte chunk per field and plane)
+    const float *p = kTI ? H.data32t + hist_node_index(H, T.i0, xi0, Z.i0) : nullptr;
+    const float4 *q0 = kTI ? nullptr : hist_rec32(H, T.i0, xi0, Z.i0);
+    const float4 *q1 = kTI ? nullptr : hist_rec32(H, T.i1, xi0, Z.i0);
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
-      const float4 a = q0[k * cs], b = q1[k * cs];   // (x0,z0), (x0,z1), (x1,z0), (x1,z1)
+      float4 a, b;   // (x0,z0), (x0,z1), (x1,z0), (x1,z1)
+      if (kTI) {
+        ring32_corners(p, (int64_t)H.cap + 1, k, T.i1 == T.i0, a, b);
+      } else {
+        a = q0[k * (int64_t)H.Zi];
+        b = q1[k * (int64_t)H.Zi];
+      }
       const float c00 = fmaf(td, b.x - a.x, a.x);
       const float c01 = fmaf(td, b.y - a.y, a.y);
       const float c10 = fmaf(td, b.z - a.z, a.z);
@@ -589,6 +602,7 @@ __device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
   const int ngroup = (sa.n + 63) >> 6;
   double acc_z = 0.0, acc_x = 0.0;
+  unsigned evaluated = 0;   // points this lane evaluated (for the optional counter)
   for (int g = wave; g < ngroup; g += nwave) {
     // a group of m < 64 columns (the last one) spreads its rows over 64 / m
     // lane sets, so the wave still does a full wave's work
@@ -608,6 +622,7 @@ __device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int
         if (!csr_point_mixed<kTI, kCells>(H, A, t, col, rt.xp[r], rt.i0[r], rt.i1[r],
                                            rt.rr[r], (float)rt.frac[r], Iz, Ix))
           continue;
+        ++evaluated;
         const float wx = (float)rt.w[r];
         cz.add(wx * Iz);
         cx.add(wx * Ix);
@@ -622,6 +637,7 @@ __device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int
         const TruncIdx X = {rt.i0[r], rt.i1[r], rt.frac[r], true};
         double Iz, Ix;
         if (!csr_point<true, kCells>(H, A, o, col, rt.xp[r], Iz, Ix, &X, &rt.rr[r])) continue;
+        ++evaluated;
         const double wx = rt.w[r];
         cz += wx * Iz;
         cx += wx * Ix;
@@ -629,6 +645,12 @@ __device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int
       acc_z += wz * cz;
       acc_x += wz * cx;
     }
+  }
+  if (T.P.points_evaluated) {   // uniform: a diagnostic call
+    unsigned long long c = evaluated;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
+    if (lane == 0 && c) atomicAdd(T.P.points_evaluated, c);
   }
   out_z = block_sum(acc_z, red);
   out_x = block_sum(acc_x, red);
@@ -1006,3 +1028,4 @@ extern "C" int dfcsr_wake_mesh_batched(int k, const DfcsrHistory *hist, const Df
   return dfcsr_wake_batched_launch(ws, &batch, precision, static_cast<char *>(ws) + tb,
                                    ws_bytes - tb, stream);
 }
+

@@ -180,10 +180,9 @@ int dfcsr_sgolay2d_kernels(int window_size, int order, int derivative, double *k
  * src: raw slices, src_slot_stride doubles apart, each holding the five
  * fields as [field][nbx][nbz] (field order DFCSR_F_*).
  * jobs: HOST array of njobs descriptors; hist64 (FP64), hist32 (FP32
- * chunked) and hist32t (FP32 slot-innermost) are rings of cap slots in the
- * layouts of DfcsrHistory (dst_slot < cap; writing slot 0 also writes its
- * copy, slot cap, of the slot-innermost rings), cells and cells32 the
- * cell-major rings; each may be NULL, but not all five. */
+ * chunked) and hist32t (FP32 time-innermost) are rings of cap slots in the
+ * layouts above (dst_slot < cap), cells and cells32 the cell-major rings of
+ * DfcsrHistory; each may be NULL, but not all five. */
 typedef struct {
   int32_t src_slot;   /* raw slice index in src */
   int32_t dst_slot;   /* ring slot in the history */
@@ -203,14 +202,7 @@ int dfcsr_regrid(const double *src, int64_t src_slot_stride, const DfcsrRegridJo
  * dx = (xi1 - xi0)/(Xi - 1), same for z.  The FP64 ring is needed by the
  * lookup primitives, the integrand and FP64 wakes; a mixed-precision wake
  * needs at least one FP32 ring (with both, its region-0 phases read the
- * slot-innermost one and the other phases the chunked one).
- * The slot-innermost rings (data, data32t) hold, per grid node (x, z), the
- * corner-packed values v = 4 f + c (field f at (x, z), (x, z+1), (x+1, z),
- * (x+1, z+1), clamped to the last row and column) with the slot fastest:
- * value v of slot s at ((x Zi + z) DFCSR_HIST_REC64 + v) (cap + 1) + s, and
- * slot cap a copy of slot 0, so the two planes of a lookup are neighbouring
- * values and lookups of neighbouring planes by neighbouring lanes are
- * neighbouring addresses.
+ * time-innermost one and the other phases the chunked one).
  * The cell-major rings (optional) hold the same slices plane by plane:
  * cell (slot, x, z) at ((slot Xi + x) Zi + z) DFCSR_CELL, the five fields
  * of that grid node (40 B in float64, 20 B in float32, against 160 B / 80 B
@@ -220,9 +212,9 @@ int dfcsr_regrid(const double *src, int64_t src_slot_stride, const DfcsrRegridJo
  * One plane must span at most 4 GiB (the wakes address cells within a plane
  * with 32-bit offsets). */
 typedef struct {
-  const double *data;    /* [Xi][Zi][DFCSR_HIST_REC64][cap + 1] or NULL */
+  const double *data;    /* [Xi][Zi][cap][DFCSR_HIST_REC64] or NULL */
   const float *data32;   /* [Xi][cap][DFCSR_HIST_CHUNKS32][Zi][4] or NULL */
-  const float *data32t;  /* [Xi][Zi][DFCSR_HIST_REC32][cap + 1] or NULL */
+  const float *data32t;  /* [Xi][Zi][cap][DFCSR_HIST_REC32] or NULL */
   const double *cells;   /* [cap][Xi][Zi][DFCSR_CELL] or NULL */
   const float *cells32;  /* [cap][Xi][Zi][DFCSR_CELL] or NULL */
   int32_t cap, head, nt, Xi, Zi;
@@ -294,8 +286,12 @@ int dfcsr_wake_mesh(const DfcsrHistory *hist, const DfcsrLattice *lat, const Dfc
                     void *ws, int64_t ws_bytes, void *stream);
 
 /* Batched form for serving: k independent simulations (each its own history,
- * lattice and parameters) in one launch.  All descriptor arrays are HOST
- * arrays of length k; output pointers per simulation. */
+ * lattice and parameters) in one launch, 1 <= k <= DFCSR_WAKE_MAX_TASKS.  All
+ * descriptor arrays are HOST arrays of length k; output pointers per
+ * simulation.  The task table is copied into the workspace on the stream and
+ * the call waits for that copy before it returns (the kernels stay
+ * asynchronous). */
+#define DFCSR_WAKE_MAX_TASKS 4096
 int dfcsr_wake_mesh_batched(int k, const DfcsrHistory *hist, const DfcsrLattice *lat,
                             const DfcsrWakeParams *p, int precision, double *const *dE_dct,
                             double *const *x_kick, void *ws, int64_t ws_bytes, void *stream);

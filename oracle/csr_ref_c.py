@@ -76,13 +76,13 @@ def hist_from_tracker(T, keep: _Keep):
 
 
 def hist_from_records(rec: np.ndarray, head, nt, axes, keep: _Keep):
-    """The device's float64 slot-innermost ring rec[Xi][Zi][20][cap + 1] of
-    corner-packed values; field f of node (x, z) is value 4 f."""
+    """The device's float64 ring rec[Xi][Zi][cap][20] of corner-packed
+    records; field f of cell (x, z) is element 4 f of its record."""
     rec = np.ascontiguousarray(rec, dtype=np.float64)
     keep.refs.append(rec)
-    Xi, Zi, R, cap1 = rec.shape
-    base = (c_dp * 5)(*[rec.ctypes.data + 8 * 4 * f * cap1 for f in range(5)])
-    return RefHist(base, 1, Zi * R * cap1, R * cap1, nt, Xi, Zi, cap1 - 1, head, *axes)
+    Xi, Zi, cap, R = rec.shape
+    base = (c_dp * 5)(*[rec.ctypes.data + 8 * 4 * f for f in range(5)])
+    return RefHist(base, R, Zi * cap * R, cap * R, nt, Xi, Zi, cap, head, *axes)
 
 
 def lat_from_tables(lat, keep: _Keep):

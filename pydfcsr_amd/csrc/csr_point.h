@@ -75,12 +75,9 @@ __device__ inline int hist_slot(const DfcsrHistory &H, int ti) {
   return slot >= H.cap ? slot - H.cap : slot;
 }
 
-// Slot-innermost rings (FP64 and FP32): value v (= 4 f + corner) of node
-// (x, z) at slot s lies at ((x Zi + z) 20 + v) (cap + 1) + s, and slot cap
-// repeats slot 0, so the two planes of a lookup are always neighbouring
-// values.  First value (v = 0) of node (x, z) at the slot of logical slice ti:
-__device__ inline int64_t hist_node_index(const DfcsrHistory &H, int ti, int xi, int zi) {
-  return ((int64_t)xi * H.Zi + zi) * DFCSR_HIST_REC64 * (int64_t)(H.cap + 1) + hist_slot(H, ti);
+// FP64 record (x, z, slot of ti): [x][z][slot][20] layout.
+__device__ inline const double *hist_rec(const DfcsrHistory &H, int ti, int xi, int zi) {
+  return H.data + (((int64_t)xi * H.Zi + zi) * H.cap + hist_slot(H, ti)) * DFCSR_HIST_REC64;
 }
 
 // Byte offsets of an x row within a plane of a cell-major ring: row i0, and
@@ -185,25 +182,25 @@ __device__ inline bool hist_lookup5(const DfcsrHistory &H, const HistAxes &A, do
                                v1[2][f], v1[3][f], td, xd, zd);
     return true;
   }
-  // corner-packed values of node (x0, z0): value f*4 + {0,1,2,3} = field f
-  // at (x0,z0), (x0,z1), (x1,z0), (x1,z1); plane t1 is the next slot (i1 =
-  // i0 + 1), or plane t0 itself at the last plane (i1 == i0), read once.
-  const double *p = H.data + hist_node_index(H, T.i0, X.i0, Z.i0);
-  const int64_t vs = (int64_t)H.cap + 1;
-  double a[DFCSR_HIST_REC64], b[DFCSR_HIST_REC64];
+  // corner-packed records: p0 at (t0, x0, z0), p1 at (t1, x0, z0); element
+  // f*4 + {0,1,2,3} = (x0,z0), (x0,z1), (x1,z0), (x1,z1).  At the last plane
+  // (i1 == i0) the record is read once.
+  const double2 *p0 = reinterpret_cast<const double2 *>(hist_rec(H, T.i0, X.i0, Z.i0));
+  double2 a[DFCSR_HIST_REC64 / 2], b[DFCSR_HIST_REC64 / 2];
 #pragma unroll
-  for (int v = 0; v < DFCSR_HIST_REC64; ++v) a[v] = p[v * vs];
+  for (int k = 0; k < DFCSR_HIST_REC64 / 2; ++k) a[k] = p0[k];
   if (T.i1 != T.i0) {
+    const double2 *p1 = reinterpret_cast<const double2 *>(hist_rec(H, T.i1, X.i0, Z.i0));
 #pragma unroll
-    for (int v = 0; v < DFCSR_HIST_REC64; ++v) b[v] = p[v * vs + 1];
+    for (int k = 0; k < DFCSR_HIST_REC64 / 2; ++k) b[k] = p1[k];
   } else {
 #pragma unroll
-    for (int v = 0; v < DFCSR_HIST_REC64; ++v) b[v] = a[v];
+    for (int k = 0; k < DFCSR_HIST_REC64 / 2; ++k) b[k] = a[k];
   }
 #pragma unroll
   for (int f = 0; f < 5; ++f)
-    out[f] = trilerp<kRecip>(a[4 * f], a[4 * f + 1], a[4 * f + 2], a[4 * f + 3], b[4 * f],
-                             b[4 * f + 1], b[4 * f + 2], b[4 * f + 3], td, xd, zd);
+    out[f] = trilerp<kRecip>(a[2 * f].x, a[2 * f].y, a[2 * f + 1].x, a[2 * f + 1].y, b[2 * f].x,
+                             b[2 * f].y, b[2 * f + 1].x, b[2 * f + 1].y, td, xd, zd);
   return true;
 }
 

@@ -9,14 +9,13 @@
 //   v00(1-y0)(1-y1) + v01(1-y0)y1 + v10 y0(1-y1) + v11 y0 y1,
 // with fill 0 outside the grid (mean(vx_x) of the slice for vx_x).
 //
-// History layout: corner-packed values (node (x, z) holds field f at
-// (x, z), (x, z1), (x1, z), (x1, z1) as values f*4 + 0..3, x1 and z1
-// clamped to the last plane).  float64 and float32 slot-innermost rings:
-// [x][z][value][cap + 1], the slot fastest and slot cap a copy of slot 0
-// (so the two planes of any lookup are neighbours); float32 chunked: 16-byte
-// chunks (one field's four corners each) stored [x][slot][f][z], where
-// threads of consecutive z write consecutive chunks; cell-major rings
-// [slot][x][z][5].  Any ring may be absent.
+// History layout: corner-packed records (record (x, z) holds field f at
+// (x, z), (x, z1), (x1, z), (x1, z1) in elements f*4 + 0..3, x1 and z1
+// clamped to the last plane).  float64: whole records [x][z][slot][20];
+// float32 chunked: 16-byte chunks (one field's four corners each) stored
+// [x][slot][f][z], where threads of consecutive z write consecutive chunks;
+// float32 time-innermost: whole records [x][z][slot][20].  Any of the three
+// rings may be absent.
 #include "common.h"
 
 namespace dfcsr {
@@ -90,28 +89,19 @@ __global__ __launch_bounds__(kThreads) void regrid_kernel(
     rgi5(J, base, fill_vxx, x0, z1, v[1]);
     rgi5(J, base, fill_vxx, x1, z0, v[2]);
     rgi5(J, base, fill_vxx, x1, z1, v[3]);
-    // slot-innermost rings: value 4 f + q of node c at (c 20 + 4 f + q)(cap + 1)
-    // + slot; slot cap repeats slot 0
-    const int64_t vs = (int64_t)cap + 1;
-    const int64_t node = c * DFCSR_HIST_REC64 * vs + J.dst_slot;
-    const bool mirror = J.dst_slot == 0;
     if (h64) {
+      double2 *d2 = reinterpret_cast<double2 *>(h64 + (c * cap + J.dst_slot) * DFCSR_HIST_REC64);
 #pragma unroll
-      for (int f = 0; f < 5; ++f)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          h64[node + (4 * f + q) * vs] = v[q][f];
-          if (mirror) h64[node + (4 * f + q) * vs + cap] = v[q][f];
-        }
+      for (int f = 0; f < 5; ++f) {
+        d2[2 * f] = make_double2(v[0][f], v[1][f]);
+        d2[2 * f + 1] = make_double2(v[2][f], v[3][f]);
+      }
     }
     if (h32t) {
+      float4 *t4 = reinterpret_cast<float4 *>(h32t + (c * cap + J.dst_slot) * DFCSR_HIST_REC32);
 #pragma unroll
       for (int f = 0; f < 5; ++f)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          h32t[node + (4 * f + q) * vs] = (float)v[q][f];
-          if (mirror) h32t[node + (4 * f + q) * vs + cap] = (float)v[q][f];
-        }
+        t4[f] = make_float4((float)v[0][f], (float)v[1][f], (float)v[2][f], (float)v[3][f]);
     }
     if (h32) {
       const int64_t blk = (int64_t)ix * cap + J.dst_slot;

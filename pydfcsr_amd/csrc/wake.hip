@@ -26,6 +26,8 @@
 // ~130 MB, more than an L2 but within the 256 MB last-level cache: all XCDs
 // sweep the same z line together, each taking groups of 16 neighbouring
 // workgroups.
+#include <vector>
+
 #include "csr_point.h"
 
 namespace dfcsr {
@@ -64,6 +66,10 @@ struct RowTable {
   // byte offsets of row i0, and from row i0 to row i1, within a plane of the
   // cell-major ring (a plane spans at most 4 GiB, so 32 bits hold both)
   CellRow rr[kMaxRows];
+  // the mixed form's float32 copies of x', the weight and the x fraction
+  float xpf[kMaxRows];
+  float wf[kMaxRows];
+  float fracf[kMaxRows];
 };
 constexpr int kNumXcd = 8;
 constexpr int kNearGroup = 16;   // consecutive near-phase workgroups per XCD
@@ -200,20 +206,24 @@ __device__ inline int xcd_remap(int b, int nb) {
 // and q = (r-r').(n-n')/R - n.tau' (whose two terms cancel to leading order)
 // are float64; the gathered fields, the coefficients' products and the
 // integrand are float32.  1/R comes from an FP32 reciprocal square root with
-// one FP64 Newton step.  The rows of one column within a phase are summed
-// in FP32 with Neumaier's compensated summation (Neumaier 1974, the
-// Kahan-Babuska improvement); those sums are weighted and accumulated in
-// float64, and the block and phase reductions are float64.
+// one FP64 Newton step; 1/scale is the FP32 hardware reciprocal.  The
+// retarded indices come from one FMA each: (t_ret - min_t)/dt = ut0 - R/dt
+// and (z' - min_z)/dz = uz0 + R/dz with ut0 per wake and uz0 per column.
+// Each point's weighted FP32 integrand is added to float64 row sums of its
+// column, which are weighted and accumulated in float64; the block and phase
+// reductions are float64.
 struct ColumnMixed {
   double sp, ax, ay, npx, npy, P0, P1, p2;
+  double uz0;   // ((s' - t) - min_z) / dz: z' index of the column before + R / dz
   float rho, D0, D1, D2, G0, G1, G2, e, a, b, A, B;
 };
 
 __device__ inline ColumnMixed make_column_mixed(const DfcsrLattice &L, const Observer &o,
-                                                double sp, double inv_ds) {
+                                                const HistAxes &HA, double sp, double inv_ds) {
   const Column c = make_column<true>(L, o, sp, inv_ds);
   ColumnMixed m;
   m.sp = sp;
+  m.uz0 = ((sp - o.t) - HA.min_z) * HA.iz;
   m.ax = c.ax;
   m.ay = c.ay;
   m.npx = c.L.nx;
@@ -249,15 +259,10 @@ __device__ inline const float4 *hist_rec32(const DfcsrHistory &H, int ti, int xi
   return reinterpret_cast<const float4 *>(H.data32) + blk * H.Zi + zi;
 }
 
-// Field k's four corners at planes t0 (a) and t1 (b) from the FP32
-// slot-innermost ring (the same layout as the FP64 one: value 4 k + c of a
-// node at stride cap + 1, the next plane at the next slot).  p: value 0 of
-// the node at the slot of t0; same_plane: i1 == i0 (b = a).
-__device__ inline void ring32_corners(const float *p, int64_t vs, int k, bool same_plane,
-                                      float4 &a, float4 &b) {
-  const float *q = p + (int64_t)(4 * k) * vs;
-  a = make_float4(q[0], q[vs], q[2 * vs], q[3 * vs]);
-  b = same_plane ? a : make_float4(q[1], q[vs + 1], q[2 * vs + 1], q[3 * vs + 1]);
+// The same record in the time-innermost ring: five consecutive chunks.
+__device__ inline const float4 *hist_rec32t(const DfcsrHistory &H, int ti, int xi, int zi) {
+  const int64_t rec = ((int64_t)xi * H.Zi + zi) * H.cap + hist_slot(H, ti);
+  return reinterpret_cast<const float4 *>(H.data32t) + rec * DFCSR_HIST_CHUNKS32;
 }
 
 // vx at (tq, xq, zq) from an FP32 ring (the time-innermost one when present),
@@ -270,8 +275,8 @@ __device__ inline double lookup_vx32(const DfcsrHistory &H, const HistAxes &A, d
   if (!(T.ok && X.ok && Z.ok)) return 0.0;
   float4 a, b;
   if (H.data32t) {
-    ring32_corners(H.data32t + hist_node_index(H, T.i0, X.i0, Z.i0), (int64_t)H.cap + 1,
-                   DFCSR_F_VX, T.i1 == T.i0, a, b);
+    a = hist_rec32t(H, T.i0, X.i0, Z.i0)[DFCSR_F_VX];
+    b = hist_rec32t(H, T.i1, X.i0, Z.i0)[DFCSR_F_VX];
   } else {
     a = hist_rec32(H, T.i0, X.i0, Z.i0)[(int64_t)DFCSR_F_VX * H.Zi];
     b = hist_rec32(H, T.i1, X.i0, Z.i0)[(int64_t)DFCSR_F_VX * H.Zi];
@@ -285,16 +290,6 @@ __device__ inline double lookup_vx32(const DfcsrHistory &H, const HistAxes &A, d
   const double c1 = c01 * (1 - xd) + c11 * xd;
   return c0 * (1 - zd) + c1 * zd;
 }
-
-struct Neumaier {
-  float s = 0.f, c = 0.f;
-  __device__ void add(float x) {
-    float t = s + x;
-    c += (fabsf(s) >= fabsf(x)) ? ((s - t) + x) : ((x - t) + s);
-    s = t;
-  }
-  __device__ float value() const { return s + c; }
-};
 
 // The five fields of the four corner cells at byte offsets off[q] of a
 // float32 cell plane (20-byte cells).
@@ -312,9 +307,9 @@ __device__ inline void load_cells32(const char *plane, const uint32_t off[4], fl
 // read the float32 cell-major ring H.cells32 instead (same values, same
 // arithmetic).
 template <bool kTI, bool kCells = false>
-__device__ inline bool csr_point_mixed(const DfcsrHistory &H, const HistAxes &A, double t,
-                                       const ColumnMixed &c, double xp, int xi0, int xi1,
-                                       CellRow rr, float xd, float &Iz, float &Ix) {
+__device__ inline bool csr_point_mixed(const DfcsrHistory &H, const HistAxes &A, double ut0,
+                                       const ColumnMixed &c, double xp, float xpf, int xi0,
+                                       int xi1, CellRow rr, float xd, float &Iz, float &Ix) {
   const double rx = c.ax - xp * c.npx;
   const double ry = c.ay - xp * c.npy;
   const double R2 = rx * rx + ry * ry;
@@ -325,9 +320,8 @@ __device__ inline bool csr_point_mixed(const DfcsrHistory &H, const HistAxes &A,
   double y = (double)rsqrtf((float)R2);
   y = y * (1.5 - 0.5 * R2 * y * y);               // one Newton step: ~1e-14 relative
   const double R = R2 * y;
-  const double t_ret = t - R;
-  const double ut = (t_ret - A.min_t) * A.it;
-  const double uz = ((c.sp - t_ret) - A.min_z) * A.iz;
+  const double ut = ut0 - R * A.it;               // (t - R - min_t) / dt
+  const double uz = c.uz0 + R * A.iz;             // (s' - (t - R) - min_z) / dz
   const TruncIdx T = trunc_index(ut, H.nt), Z = trunc_index(uz, H.Zi);
   if (!(T.ok && Z.ok)) return false;
   const float td = (float)T.frac, zd = (float)Z.frac;
@@ -368,21 +362,13 @@ __device__ inline bool csr_point_mixed(const DfcsrHistory &H, const HistAxes &A,
       }
     }
   } else {
-    // the corner-packed values of node (x0, z0) at both planes: from the
-    // slot-innermost ring (neighbouring slots) or the chunked one (one
-    // 16-byte chunk per field and plane)
-    const float *p = kTI ? H.data32t + hist_node_index(H, T.i0, xi0, Z.i0) : nullptr;
-    const float4 *q0 = kTI ? nullptr : hist_rec32(H, T.i0, xi0, Z.i0);
-    const float4 *q1 = kTI ? nullptr : hist_rec32(H, T.i1, xi0, Z.i0);
+    // one corner-packed record per time plane, one 16-byte chunk per field
+    const float4 *q0 = kTI ? hist_rec32t(H, T.i0, xi0, Z.i0) : hist_rec32(H, T.i0, xi0, Z.i0);
+    const float4 *q1 = kTI ? hist_rec32t(H, T.i1, xi0, Z.i0) : hist_rec32(H, T.i1, xi0, Z.i0);
+    const int64_t cs = kTI ? 1 : H.Zi;
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
-      float4 a, b;   // (x0,z0), (x0,z1), (x1,z0), (x1,z1)
-      if (kTI) {
-        ring32_corners(p, (int64_t)H.cap + 1, k, T.i1 == T.i0, a, b);
-      } else {
-        a = q0[k * (int64_t)H.Zi];
-        b = q1[k * (int64_t)H.Zi];
-      }
+      const float4 a = q0[k * cs], b = q1[k * cs];   // (x0,z0), (x0,z1), (x1,z0), (x1,z1)
       const float c00 = fmaf(td, b.x - a.x, a.x);
       const float c01 = fmaf(td, b.y - a.y, a.y);
       const float c10 = fmaf(td, b.z - a.z, a.z);
@@ -394,8 +380,8 @@ __device__ inline bool csr_point_mixed(const DfcsrHistory &H, const HistAxes &A,
   }
   const float dens = f[DFCSR_F_DENSITY], dx = f[DFCSR_F_DENSITY_X], dz = f[DFCSR_F_DENSITY_Z];
   const float vr = f[DFCSR_F_VX], dv = f[DFCSR_F_VX_X];
-  const float scale = fmaf((float)xp, c.rho, 1.0f);
-  const float g = dz / scale;
+  const float scale = fmaf(xpf, c.rho, 1.0f);
+  const float g = dz * __builtin_amdgcn_rcpf(scale);
   const float Cd = fmaf(vr, fmaf(vr, c.D2, c.D1), c.D0);
   const float Cg = fmaf(vr, fmaf(vr, c.G2, c.G1), c.G0);
   const float part1 = fmaf(vr, c.B, c.A);
@@ -417,12 +403,30 @@ __device__ inline bool csr_point_mixed(const DfcsrHistory &H, const HistAxes &A,
 // Region 0 has 2 Nx rows, the others Nx (make_regions); each region's rows
 // are split into ceil(rows / kChunk*) chunks of equal length.  Phase order of
 // a mesh point: the chunks of region 0, then those of regions 1, 2 (, 3).
+//
+// Region 0 of a mesh (observers from the mesh formula) is integrated
+// observer-major instead: every observer of a z line has the same region-0
+// x' rows (x0 depends on s only) and, off the chirp branch, the same s'
+// columns, so a wavefront's lanes take neighbouring observers of one z line
+// and evaluate the same source point together -- their retarded lookups fall
+// on the same or neighbouring records.  A workgroup covers one z line, a
+// block of W observers along x (W = 64, or the mesh's x bins rounded up to a
+// power of two) and 4 S column blocks (S = 64 / W lane groups per wave); its
+// row table holds the region's rows inside the history, 64 at a time.  The
+// partial sums of region 0 are then per (observer, column block): nA = nCB.
 struct PhasePlan {
   int nreg;                 // 3 without chirp, 4 with
-  int rows0, ch0, per0;     // region 0: rows, chunks, rows per chunk
+  int rows0, ch0, per0;     // region 0 by row chunks: rows, chunks, rows per chunk
   int rows1, ch1, per1;     // each later region
-  int nA, nB;               // phases of region 0 / of the later regions
+  int nA, nB;               // partial sums of region 0 / phases of the later regions
+  bool obs_major;           // region 0 observer-major (mesh observers)
+  int W, S, nob, q;         // observer-major: lanes per observer block, column blocks
+                            // per wave, observer blocks per z line, workgroups per block
 };
+
+// Workgroups of an observer-major region-0 launch wanted per simulation (at
+// most): enough to fill the chip when the mesh is small.
+constexpr int kFarWaveTarget = 8192;
 
 __host__ __device__ inline PhasePlan make_plan(const DfcsrWakeParams &P) {
   PhasePlan pl;
@@ -433,19 +437,45 @@ __host__ __device__ inline PhasePlan make_plan(const DfcsrWakeParams &P) {
   pl.rows1 = P.Nx;
   pl.ch1 = (pl.rows1 + kChunkNear - 1) / kChunkNear;
   pl.per1 = (pl.rows1 + pl.ch1 - 1) / pl.ch1;
-  pl.nA = pl.ch0;
   pl.nB = (pl.nreg - 1) * pl.ch1;
+  pl.obs_major = P.observers == nullptr;
+  if (pl.obs_major) {
+    const int xb = P.mesh_xbins, zb = P.mesh_zbins;
+    int W = 1;
+    while (W < xb && W < 64) W <<= 1;
+    pl.W = W;
+    pl.S = 64 / W;
+    pl.nob = (xb + W - 1) / W;
+    const int64_t lines = (int64_t)zb * pl.nob;
+    int q = (int)((kFarWaveTarget / 4 + lines - 1) / lines);
+    const int qmax = (P.Nz + 4 * pl.S - 1) / (4 * pl.S);   // no more blocks than columns
+    q = q < 1 ? 1 : q;
+    q = q > 16 ? 16 : q;
+    q = q > qmax ? (qmax < 1 ? 1 : qmax) : q;
+    pl.q = q;
+    pl.nA = 4 * pl.S * q;
+  } else {
+    pl.W = pl.S = pl.nob = pl.q = 0;
+    pl.nA = pl.ch0;
+  }
   return pl;
 }
 
-// Region and row range [j0, j1) of phase ph.
+// Workgroups of the region-0 launch of one simulation.
+__host__ __device__ inline int64_t far_blocks(const PhasePlan &pl, const DfcsrWakeParams &P) {
+  return pl.obs_major ? (int64_t)P.mesh_zbins * pl.nob * pl.q
+                      : (int64_t)P.mesh_xbins * P.mesh_zbins * pl.nA;
+}
+
+// Region and row range [j0, j1) of phase ph (row-chunk phases: region 0 when
+// it is not observer-major, and the later regions).
 __host__ __device__ inline void phase_rows(const PhasePlan &pl, int ph, int &r, int &j0, int &j1) {
-  if (ph < pl.ch0) {
+  if (ph < pl.nA) {
     r = 0;
     j0 = ph * pl.per0;
     j1 = min(j0 + pl.per0, pl.rows0);
   } else {
-    const int q = ph - pl.ch0;
+    const int q = ph - pl.nA;
     r = 1 + q / pl.ch1;
     j0 = (q % pl.ch1) * pl.per1;
     j1 = min(j0 + pl.per1, pl.rows1);
@@ -591,6 +621,9 @@ __device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int
     rt.i0[r] = X.i0;
     rt.i1[r] = X.i1;
     rt.ok[r] = X.ok;
+    rt.xpf[r] = (float)xp;
+    rt.wf[r] = (float)rt.w[r];
+    rt.fracf[r] = (float)X.frac;
     if (kCells) {
       const uint32_t rowB = (uint32_t)H.Zi * (uint32_t)(DFCSR_CELL *
                                                         (kMixed ? sizeof(float) : sizeof(double)));
@@ -614,21 +647,22 @@ __device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int
     const double sp = sa.at(k);
     const double wz = sa.w(k);
     if (kMixed) {
-      const ColumnMixed col = make_column_mixed(L, o, sp, inv_ds);
-      Neumaier cz, cx;
+      const ColumnMixed col = make_column_mixed(L, o, A, sp, inv_ds);
+      const double ut0 = (t - A.min_t) * A.it;
+      double cz = 0.0, cx = 0.0;
       for (int r = q; r < v1 - v0; r += nq) {
         if (!rt.ok[r]) continue;
         float Iz, Ix;
-        if (!csr_point_mixed<kTI, kCells>(H, A, t, col, rt.xp[r], rt.i0[r], rt.i1[r],
-                                           rt.rr[r], (float)rt.frac[r], Iz, Ix))
+        if (!csr_point_mixed<kTI, kCells>(H, A, ut0, col, rt.xp[r], rt.xpf[r], rt.i0[r],
+                                           rt.i1[r], rt.rr[r], rt.fracf[r], Iz, Ix))
           continue;
         ++evaluated;
-        const float wx = (float)rt.w[r];
-        cz.add(wx * Iz);
-        cx.add(wx * Ix);
+        const float wx = rt.wf[r];
+        cz += (double)(wx * Iz);
+        cx += (double)(wx * Ix);
       }
-      acc_z += wz * (double)cz.value();
-      acc_x += wz * (double)cx.value();
+      acc_z += wz * cz;
+      acc_x += wz * cx;
     } else {
       const Column col = make_column<true>(L, o, sp, inv_ds);
       double cz = 0.0, cx = 0.0;
@@ -684,6 +718,151 @@ __device__ inline void run_phase_block(const WakeTask &T, int l, double *__restr
   }
 }
 
+// Workgroup l of a task's observer-major region-0 launch: z line, block of
+// observers, group of column blocks (see PhasePlan).  Lane (sub, ol) of wave
+// w integrates, for observer ol of the block, column block (4 g + w) S + sub
+// over every row of region 0 inside the history, and writes that partial sum
+// (NaN where the reference's integrand is 0/0 over the region, as
+// wake_phase).  Summation: per column over the rows in order, weighted sums
+// over the block's columns in order.
+template <bool kMixed, bool kTI>
+__device__ void far_obs_block(const WakeTask &T, int l, double *__restrict__ partials,
+                              RowTable &rt) {
+  const DfcsrHistory &H = T.H;
+  const DfcsrLattice &L = T.L;
+  const PhasePlan pl = make_plan(T.P);
+  const int xb = T.P.mesh_xbins, zb = T.P.mesh_zbins;
+  const int per_line = pl.nob * pl.q;
+  const int zl = l / per_line, rem = l - zl * per_line;
+  const int ob = rem / pl.q, g = rem - ob * pl.q;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int sub = lane / pl.W, ol = lane - sub * pl.W;
+  const int ix = ob * pl.W + ol;
+  const bool valid = ix < xb;
+  const int i = (valid ? ix : ob * pl.W) * zb + zl;   // padding lanes follow the block's first
+  const int cb = (4 * g + wave) * pl.S + sub;
+  const int Nz = T.P.Nz;
+  const int k0 = (int)((int64_t)cb * Nz / pl.nA), k1 = (int)((int64_t)(cb + 1) * Nz / pl.nA);
+  const double *__restrict__ od = partials + T.obs + (int64_t)i * kObsStride;
+  const double t = T.P.t;
+  const HistAxes A = make_axes(H);
+  const double inv_ds = 1.0 / L.ds;
+  // the z line's x' rows (the same for every observer of the line) and this
+  // observer's s' columns (the same too, off the chirp branch)
+  const LinAxis xa = load_axis(partials + T.obs + (int64_t)(ob * pl.W * zb + zl) * kObsStride + 16);
+  const LinAxis sa = load_axis(od + 32);
+  double *out = partials + T.part + ((int64_t)i * (pl.nA + pl.nB) + cb) * 2;
+  if (xa.n < 2 || sa.n < 2) {   // trapz over a single sample (uniform)
+    if (valid) out[0] = out[1] = 0.0;
+    return;
+  }
+  bool nan_region = false;
+  if (od[9] != 0.0) {   // as wake_phase: observer and an end column outside the table
+    const bool in0 = trunc_index((sa.at(0) - L.min_s) * inv_ds, L.ns).ok;
+    const bool in1 = trunc_index((sa.at(sa.n - 1) - L.min_s) * inv_ds, L.ns).ok;
+    nan_region = !in0 || !in1;
+  }
+  // hull of the rows inside the history (x' is monotone in j)
+  __shared__ int hull[2];
+  if (threadIdx.x == 0) {
+    hull[0] = xa.n;
+    hull[1] = -1;
+  }
+  __syncthreads();
+  {
+    int lo = xa.n, hi = -1;
+    for (int j = threadIdx.x; j < xa.n; j += blockDim.x) {
+      const double ux = (xa.at(j) - A.min_x) * A.ix;
+      if (ux > -1.0 && ux < (double)H.Xi) {
+        lo = min(lo, j);
+        hi = max(hi, j);
+      }
+    }
+    if (lo <= hi) {
+      atomicMin(&hull[0], lo);
+      atomicMax(&hull[1], hi);
+    }
+  }
+  __syncthreads();
+  const int v0 = hull[0], v1 = hull[1] + 1;
+  Observer o;
+  o.s = od[0];
+  o.x = od[1];
+  o.t = t;
+  o.L.X0 = od[2];
+  o.L.Y0 = od[3];
+  o.L.nx = od[4];
+  o.L.ny = od[5];
+  o.L.tx = od[6];
+  o.L.ty = od[7];
+  o.vx = od[8];
+  const double ut0 = (t - A.min_t) * A.it;
+  double acc_z = 0.0, acc_x = 0.0;
+  unsigned evaluated = 0;
+  for (int b0 = v0; b0 < v1; b0 += kMaxRows) {   // uniform over the workgroup
+    const int nr = min(kMaxRows, v1 - b0);
+    if (threadIdx.x < nr) {
+      const int r = threadIdx.x;
+      const double xp = xa.at(b0 + r);
+      const TruncIdx X = trunc_index((xp - A.min_x) * A.ix, H.Xi);
+      rt.xp[r] = xp;
+      rt.w[r] = xa.w(b0 + r);
+      rt.frac[r] = X.frac;
+      rt.i0[r] = X.i0;
+      rt.i1[r] = X.i1;
+      rt.ok[r] = X.ok;
+      rt.xpf[r] = (float)xp;
+      rt.wf[r] = (float)rt.w[r];
+      rt.fracf[r] = (float)X.frac;
+    }
+    __syncthreads();
+    for (int k = k0; k < k1; ++k) {
+      const double sp = sa.at(k);
+      const double wz = sa.w(k);
+      double cz = 0.0, cx = 0.0;
+      if (kMixed) {
+        const ColumnMixed col = make_column_mixed(L, o, A, sp, inv_ds);
+        for (int r = 0; r < nr; ++r) {
+          if (!rt.ok[r]) continue;
+          float Iz, Ix;
+          if (!csr_point_mixed<kTI, false>(H, A, ut0, col, rt.xp[r], rt.xpf[r], rt.i0[r],
+                                            rt.i1[r], CellRow{0u, 0u}, rt.fracf[r], Iz, Ix))
+            continue;
+          ++evaluated;
+          const float wx = rt.wf[r];
+          cz += (double)(wx * Iz);
+          cx += (double)(wx * Ix);
+        }
+      } else {
+        const Column col = make_column<true>(L, o, sp, inv_ds);
+        for (int r = 0; r < nr; ++r) {
+          if (!rt.ok[r]) continue;
+          const TruncIdx X = {rt.i0[r], rt.i1[r], rt.frac[r], true};
+          double Iz, Ix;
+          if (!csr_point<true, false>(H, A, o, col, rt.xp[r], Iz, Ix, &X)) continue;
+          ++evaluated;
+          const double wx = rt.w[r];
+          cz += wx * Iz;
+          cx += wx * Ix;
+        }
+      }
+      acc_z += wz * cz;
+      acc_x += wz * cx;
+    }
+    __syncthreads();   // before the next rows overwrite the table
+  }
+  if (T.P.points_evaluated) {   // uniform: a diagnostic call
+    unsigned long long c = valid ? evaluated : 0u;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
+    if (lane == 0 && c) atomicAdd(T.P.points_evaluated, c);
+  }
+  if (valid) {
+    out[0] = nan_region ? NAN : acc_z;
+    out[1] = nan_region ? NAN : acc_x;
+  }
+}
+
 // The FP32 ring a mixed launch reads: region 0 (many retarded planes per
 // wavefront) the time-innermost one, the later regions (one plane,
 // consecutive z') the chunked one, each when present.
@@ -700,6 +879,14 @@ __global__ __launch_bounds__(kThreads, kCells ? (kMixed ? 4 : 3) : DFCSR_WAKE_MI
   const int b = blockIdx.x, nb = gridDim.x;
   run_phase_block<kMixed, kNear, kTI, kCells>(
       T, kNear ? xcd_group(b, nb, kNearGroup) : xcd_remap(b, nb), partials, red);
+}
+
+// The observer-major region-0 launch: each XCD a contiguous range of z lines.
+template <bool kMixed, bool kTI>
+__global__ __launch_bounds__(kThreads, DFCSR_WAKE_MINBLOCKS) void wake_far_kernel(
+    WakeTask T, double *__restrict__ partials) {
+  __shared__ RowTable rt;
+  far_obs_block<kMixed, kTI>(T, xcd_remap(blockIdx.x, gridDim.x), partials, rt);
 }
 
 // Last task whose first index (workgroup or observer) is <= b.
@@ -724,6 +911,18 @@ __global__ __launch_bounds__(kThreads, kMixed && kNear ? 4 : DFCSR_WAKE_MINBLOCK
                           [](const WakeTask &T) { return kNear ? T.blockB0 : T.blockA0; });
   const WakeTask &T = tasks[j];
   const int l = b - (kNear ? T.blockB0 : T.blockA0);
+  if (!kNear && T.P.observers == nullptr) {   // observer-major region 0
+    __shared__ RowTable rt;
+    if constexpr (kMixed) {
+      if (mixed_reads_ti(T.H, false))
+        far_obs_block<true, true>(T, l, partials, rt);
+      else
+        far_obs_block<true, false>(T, l, partials, rt);
+    } else {
+      far_obs_block<false, false>(T, l, partials, rt);
+    }
+    return;
+  }
   if constexpr (kMixed) {   // per task: its FP32 rings, cell-major or not
     if (kNear && T.H.cells32)
       run_phase_block<true, kNear, false, true>(T, l, partials, red);
@@ -845,13 +1044,14 @@ int make_task(const DfcsrHistory &H, const DfcsrLattice &L, const DfcsrWakeParam
   T.nA = pl.nA;
   T.nB = pl.nB;
   T.blockA0 = T.blockB0 = T.obs0 = 0;
-  if ((int64_t)T.M * (pl.nA + pl.nB) > ((int64_t)1 << 30))
+  if ((int64_t)T.M * (pl.nA + pl.nB) > ((int64_t)1 << 30) || far_blocks(pl, P) > ((int64_t)1 << 30))
     return set_error(DFCSR_ERANGE, "%s: %d mesh points x %d phases is too many workgroups", who,
                      T.M, pl.nA + pl.nB);
   return DFCSR_OK;
 }
 
 constexpr int64_t kTableAlign = 256;
+constexpr int kMaxTasks = DFCSR_WAKE_MAX_TASKS;
 
 int64_t table_bytes(int k) {
   return (((int64_t)k * (int64_t)sizeof(WakeTask) + kTableAlign - 1) / kTableAlign) * kTableAlign;
@@ -888,10 +1088,16 @@ extern "C" int dfcsr_wake_mesh(const DfcsrHistory *hist, const DfcsrLattice *lat
   if ((rc = make_task(*hist, *lat, *p, dE_dct, x_kick, xmesh, zmesh, T, "wake_mesh"))) return rc;
   double *part = static_cast<double *>(ws);
   hipStream_t s = as_stream(stream);
-  const dim3 ga(T.M * T.nA), gb(T.M * T.nB), gf((T.M + kThreads - 1) / kThreads);
+  const PhasePlan pl = make_plan(*p);
+  const dim3 ga((unsigned)far_blocks(pl, *p)), gb(T.M * T.nB), gf((T.M + kThreads - 1) / kThreads);
   hipLaunchKernelGGL(wake_observer_kernel, gf, dim3(kThreads), 0, s, T, part);
   if (precision == DFCSR_PREC_MIXED) {
-    if (mixed_reads_ti(*hist, false))
+    const bool ti_far = mixed_reads_ti(*hist, false);
+    if (pl.obs_major && ti_far)
+      hipLaunchKernelGGL((wake_far_kernel<true, true>), ga, dim3(kThreads), 0, s, T, part);
+    else if (pl.obs_major)
+      hipLaunchKernelGGL((wake_far_kernel<true, false>), ga, dim3(kThreads), 0, s, T, part);
+    else if (ti_far)
       hipLaunchKernelGGL((wake_phase_kernel<true, false, true>), ga, dim3(kThreads), 0, s, T, part);
     else
       hipLaunchKernelGGL((wake_phase_kernel<true, false, false>), ga, dim3(kThreads), 0, s, T, part);
@@ -905,7 +1111,10 @@ extern "C" int dfcsr_wake_mesh(const DfcsrHistory *hist, const DfcsrLattice *lat
       hipLaunchKernelGGL((wake_phase_kernel<true, true, false>), gb, dim3(kThreads), 0, s, T, part);
     }
   } else {
-    hipLaunchKernelGGL((wake_phase_kernel<false, false, false>), ga, dim3(kThreads), 0, s, T, part);
+    if (pl.obs_major)
+      hipLaunchKernelGGL((wake_far_kernel<false, false>), ga, dim3(kThreads), 0, s, T, part);
+    else
+      hipLaunchKernelGGL((wake_phase_kernel<false, false, false>), ga, dim3(kThreads), 0, s, T, part);
     if (hist->cells) {
       hipLaunchKernelGGL((wake_phase_kernel<false, true, false, true>), gb, dim3(kThreads), 0, s, T,
                          part);
@@ -932,7 +1141,8 @@ extern "C" int dfcsr_wake_tasks_pack(int k, const DfcsrHistory *hist, const Dfcs
                                      const DfcsrWakeParams *p, int precision,
                                      double *const *dE_dct, double *const *x_kick, void *tasks,
                                      int64_t tasks_bytes, DfcsrWakeBatch *batch) {
-  if (k < 1 || k > 4096) return set_error(DFCSR_ERANGE, "wake_tasks_pack: %d tasks", k);
+  if (k < 1 || k > kMaxTasks)
+    return set_error(DFCSR_ERANGE, "wake_tasks_pack: %d tasks (1 to %d)", k, kMaxTasks);
   if (!hist || !lat || !p || !dE_dct || !x_kick || !tasks || !batch)
     return set_error(DFCSR_EINVAL, "wake_tasks_pack: NULL pointer");
   if (tasks_bytes < (int64_t)k * (int64_t)sizeof(WakeTask))
@@ -958,7 +1168,7 @@ extern "C" int dfcsr_wake_tasks_pack(int k, const DfcsrHistory *hist, const Dfcs
     T.blockB0 = (int)nb;
     T.obs0 = (int)nobs;
     part += partial_doubles(p[j]);
-    na += (int64_t)T.M * T.nA;
+    na += far_blocks(make_plan(p[j]), p[j]);
     nb += (int64_t)T.M * T.nB;
     nobs += T.M;
     if (na > ((int64_t)1 << 30) || nb > ((int64_t)1 << 30))
@@ -1012,18 +1222,23 @@ extern "C" int dfcsr_wake_mesh_batched(int k, const DfcsrHistory *hist, const Df
                                        double *const *dE_dct, double *const *x_kick, void *ws,
                                        int64_t ws_bytes, void *stream) {
   if (k < 1) return DFCSR_OK;
-  if (k > 256) return set_error(DFCSR_ERANGE, "wake_mesh_batched: %d tasks > 256", k);
+  if (k > kMaxTasks)
+    return set_error(DFCSR_ERANGE, "wake_mesh_batched: %d tasks > %d", k, kMaxTasks);
   if (!ws) return set_error(DFCSR_EINVAL, "wake_mesh_batched: NULL workspace");
   if (!p) return set_error(DFCSR_EINVAL, "wake_mesh_batched: NULL params");
   if (ws_bytes < dfcsr_wake_batched_workspace_bytes(k, p))
     return set_error(DFCSR_ENOWS, "wake_mesh_batched: workspace too small");
-  WakeTask tasks_h[256];
+  std::vector<WakeTask> tasks_h((size_t)k);
   DfcsrWakeBatch batch;
-  int rc = dfcsr_wake_tasks_pack(k, hist, lat, p, precision, dE_dct, x_kick, tasks_h,
-                                 (int64_t)sizeof(tasks_h), &batch);
+  int rc = dfcsr_wake_tasks_pack(k, hist, lat, p, precision, dE_dct, x_kick, tasks_h.data(),
+                                 (int64_t)(tasks_h.size() * sizeof(WakeTask)), &batch);
   if (rc) return rc;
   hipStream_t s = as_stream(stream);
-  DFCSR_CHECK_HIP(hipMemcpyAsync(ws, tasks_h, (size_t)k * sizeof(WakeTask), hipMemcpyHostToDevice, s));
+  // ordered on the stream after whatever still reads the workspace's table;
+  // the host table must outlive the copy, so wait for it before returning
+  DFCSR_CHECK_HIP(hipMemcpyAsync(ws, tasks_h.data(), (size_t)k * sizeof(WakeTask),
+                                 hipMemcpyHostToDevice, s));
+  DFCSR_CHECK_HIP(hipStreamSynchronize(s));
   const int64_t tb = table_bytes(k);
   return dfcsr_wake_batched_launch(ws, &batch, precision, static_cast<char *>(ws) + tb,
                                    ws_bytes - tb, stream);

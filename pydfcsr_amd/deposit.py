@@ -8,18 +8,17 @@ interp grid; build_interpolant fixes the axes the wake kernel reads.
 
 Storage is rings with one slot per logged slice, on the device for the
 whole run:
-  raw    [cap][5][nbmax*nbmax]   float64  fields on the slice's own grid
-  hist   [Xi][Zi][20][cap + 1]   float64  re-gridded corner-packed values
-  hist32t[Xi][Zi][20][cap + 1]   float32  the same values (mixed precision)
-  hist32 [Xi][cap][5][Zi][4]     float32  the same, chunked (optional)
-  cells  [cap][Xi][Zi][5]        float64  cell-major: the five fields at (x, z)
-  cells32[cap][Xi][Zi][5]        float32  the same in float32
-(node (x, z) holds every field at (x, z), (x, z+1), (x+1, z), (x+1, z+1),
-clamped to the last row and column: one node per lookup.  hist and hist32t
-keep the slot fastest, with slot cap a copy of slot 0: the two planes of a
-lookup are neighbouring values, and the far region's lanes, which look up
-neighbouring retarded planes at one node, read neighbouring addresses; the
-cell-major rings keep a plane's cells together, neighbouring z in
+  raw    [cap][5][nbmax*nbmax] float64  fields on the slice's own grid
+  hist   [Xi][Zi][cap][20]     float64  re-gridded corner-packed records
+  hist32t[Xi][Zi][cap][20]     float32  the same records (mixed precision)
+  hist32 [Xi][cap][5][Zi][4]   float32  the same, chunked (optional)
+  cells  [cap][Xi][Zi][5]      float64  cell-major: the five fields at (x, z)
+  cells32[cap][Xi][Zi][5]      float32  the same in float32
+(record (x, z) holds every field at (x, z), (x, z+1), (x+1, z), (x+1, z+1),
+clamped to the last plane: one record per time plane per lookup.  hist and
+hist32t keep the slot innermost (the two planes of a lookup are neighbours,
+one lookup reads one contiguous span: what the far region's lookups want);
+the cell-major rings keep a plane's cells together, neighbouring z in
 neighbouring cells: what the near regions' lookups want.  hist32 stores each field's four corners as one
 16-byte chunk, z fastest; it serves the near regions when the cell-major
 rings are not kept.  keep_fp64 chooses hist (and cells), keep_fp32 hist32t
@@ -90,13 +89,16 @@ class DF_tracker:
         self.Xi = self.Zi = 0
         self.xi = self.zi = None          # (start, stop) of the interp grid
         self.raw = None
-        self.hist = None      # float64 (Xi, Zi, 20, cap + 1)
+        self.hist = None      # float64 (Xi, Zi, cap, 20)
         self.hist32 = None    # float32 chunked (Xi, cap, 5, Zi, 4)
-        self.hist32t = None   # float32 slot-innermost (Xi, Zi, 20, cap + 1)
+        self.hist32t = None   # float32 time-innermost (Xi, Zi, cap, 20)
         self.cells = None     # float64 cell-major (cap, Xi, Zi, 5)
         self.cells32 = None   # float32 cell-major (cap, Xi, Zi, 5)
         self.cur = None
         self.n_regrid = 0
+        # largest interp grid side the rings may take (None: unbounded, as the
+        # reference); a re-grid past it raises (serving admits by it)
+        self.max_grid = None
         self.info = torch.zeros(4, dtype=torch.float64, device=self.device)
         self._ensure_capacity(capacity)
 
@@ -126,17 +128,14 @@ class DF_tracker:
                 s = self._slot(i)
                 raw[i].copy_(self.raw[s])
                 if hist is not None and self.hist is not None:
-                    hist[..., i].copy_(self.hist[..., s])
+                    hist[:, :, i].copy_(self.hist[:, :, s])
                 if hist32t is not None and self.hist32t is not None:
-                    hist32t[..., i].copy_(self.hist32t[..., s])
+                    hist32t[:, :, i].copy_(self.hist32t[:, :, s])
                 if hist32 is not None and self.hist32 is not None:
                     hist32[:, i].copy_(self.hist32[:, s])
                 for new, old in ((cells, self.cells), (cells32, self.cells32)):
                     if new is not None and old is not None:
                         new[i].copy_(old[s])
-        for r in (hist, hist32t):   # slot new_cap repeats slot 0
-            if r is not None and self.cap:
-                r[..., new_cap].copy_(r[..., 0])
         self.raw, self.hist, self.hist32, self.hist32t = raw, hist, hist32, hist32t
         self.cells, self.cells32 = cells, cells32
         self.cap = new_cap
@@ -149,13 +148,11 @@ class DF_tracker:
         hist = hist32 = hist32t = cells = cells32 = None
         dev = self.device
         if self.keep_fp64:
-            hist = torch.empty((Xi, Zi, _lib.HIST_REC64, cap + 1), dtype=torch.float64,
-                               device=dev)
+            hist = torch.empty((Xi, Zi, cap, _lib.HIST_REC64), dtype=torch.float64, device=dev)
             if self.keep_cells and Xi * Zi * _lib.CELL * 8 <= (1 << 32):
                 cells = torch.empty((cap, Xi, Zi, _lib.CELL), dtype=torch.float64, device=dev)
         if self.keep_fp32:
-            hist32t = torch.empty((Xi, Zi, _lib.HIST_REC32, cap + 1), dtype=torch.float32,
-                                  device=dev)
+            hist32t = torch.empty((Xi, Zi, cap, _lib.HIST_REC32), dtype=torch.float32, device=dev)
             if self.keep_cells and Xi * Zi * _lib.CELL * 4 <= (1 << 32):
                 cells32 = torch.empty((cap, Xi, Zi, _lib.CELL), dtype=torch.float32, device=dev)
             if self.keep_chunked:
@@ -249,6 +246,9 @@ class DF_tracker:
         if isinstance(self.upper_limit, int):
             xbins = min(xbins, self.upper_limit)
             zbins = min(zbins, self.upper_limit)
+        if self.max_grid is not None and max(xbins, zbins) > self.max_grid:
+            raise DfcsrError(f"re-grid to {xbins} x {zbins} exceeds the {self.max_grid} per side "
+                             "this history was admitted for")
         self.sigma_x_interp = max_sx
         self.sigma_z_interp = max_sz
         self.xi = (c.xmean - 5 * max_sx, c.xmean + 5 * max_sx)
@@ -294,8 +294,8 @@ class DF_tracker:
         (the reference's data_*_interp arrays; for inspection and tests)."""
         idx = torch.tensor([self._slot(i) for i in range(self.nt)], device=self.device)
         ring = self.hist if self.hist is not None else self.hist32t
-        h = ring[:, :, 0::4].index_select(3, idx).double()   # corner (x, z) of each field
-        return {f: h[:, :, k].permute(2, 0, 1).contiguous() for k, f in enumerate(_lib.FIELDS)}
+        h = ring[..., 0::4].index_select(2, idx).double()   # corner (x, z) of each field
+        return {f: h[..., k].permute(2, 0, 1).contiguous() for k, f in enumerate(_lib.FIELDS)}
 
     def footprint_bytes(self):
         b = self.raw.numel() * 8 if self.raw is not None else 0

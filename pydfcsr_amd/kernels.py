@@ -212,10 +212,9 @@ def sgolay2d_kernels(window_size: int, order: int, derivative=None):
 def regrid(src: torch.Tensor, slot_stride: int, jobs, hist64, hist32, Xi, Zi, xi0, xi1, zi0, zi1,
            ws: _lib.Workspace, hist32t=None, cells=None, cells32=None):
     """jobs: list of (src_slot, dst_slot, nbx, nbz, x0, x1, z0, z1).  Rings (any
-    may be None, not all): hist64 float64 slot-innermost (Xi, Zi, 20, cap + 1),
-    hist32 float32 chunked (Xi, cap, 5, Zi, 4), hist32t float32 slot-innermost
-    (Xi, Zi, 20, cap + 1), cells / cells32 cell-major float64 / float32
-    (cap, Xi, Zi, 5)."""
+    may be None, not all): hist64 float64 (Xi, Zi, cap, 20), hist32 float32
+    chunked (Xi, cap, 5, Zi, 4), hist32t float32 time-innermost (Xi, Zi, cap, 20),
+    cells / cells32 cell-major float64 / float32 (cap, Xi, Zi, 5)."""
     lib = _lib.load()
     n = len(jobs)
     if n == 0:
@@ -224,15 +223,15 @@ def regrid(src: torch.Tensor, slot_stride: int, jobs, hist64, hist32, Xi, Zi, xi
     b = ws.get("regrid", lib.dfcsr_regrid_workspace_bytes(n))
     rings = [r for r in (hist64, hist32t) if r is not None]
     if rings:
-        cap = int(rings[0].shape[3]) - 1
+        cap = int(rings[0].shape[2])
     elif hist32 is not None:
         cap = int(hist32.shape[1])
     elif cells is not None or cells32 is not None:
         cap = int((cells if cells is not None else cells32).shape[0])
     else:
         raise DfcsrError("regrid: no history ring")
-    want = {id(hist64): ((Xi, Zi, _lib.HIST_REC64, cap + 1), torch.float64),
-            id(hist32t): ((Xi, Zi, _lib.HIST_REC32, cap + 1), torch.float32),
+    want = {id(hist64): ((Xi, Zi, cap, _lib.HIST_REC64), torch.float64),
+            id(hist32t): ((Xi, Zi, cap, _lib.HIST_REC32), torch.float32),
             id(hist32): ((Xi, cap, _lib.HIST_CHUNKS32, Zi, 4), torch.float32),
             id(cells): ((cap, Xi, Zi, _lib.CELL), torch.float64),
             id(cells32): ((cap, Xi, Zi, _lib.CELL), torch.float32)}

@@ -56,17 +56,31 @@ class _Slot:
     wake_results: list = field(default_factory=list)
 
 
+# interp-grid side a request without an integer upper_limit is admitted for
+# (the reference's chicane example caps it at 2000); a re-grid past it fails
+# the request instead of overrunning its admitted memory
+DEFAULT_GRID_BOUND = 2000
+
+
+def grid_bound(config: dict) -> int:
+    """Largest interp grid side the request's history may reach: its
+    upper_limit when that is an int (deposit.py:361-363), else
+    DEFAULT_GRID_BOUND."""
+    ul = (config.get("particle_deposition") or {}).get("upper_limit")
+    return int(ul) if isinstance(ul, int) else DEFAULT_GRID_BOUND
+
+
 def request_footprint(config: dict, steps: int, precision: str = "mixed") -> int:
-    """Upper bound of the device bytes one simulation holds (admission)."""
+    """Upper bound of the device bytes one simulation holds (admission); the
+    simulation's history is held to grid_bound(config) per side."""
     beam = config["input_beam"]
     n = int(beam.get("n_particle", 0)) or 1_000_000
     dep = config.get("particle_deposition") or {}
     nb = max(int(dep.get("xbins", 100)), int(dep.get("zbins", 100)), 100)
-    ul = dep.get("upper_limit")
-    grid = int(ul) if isinstance(ul, int) else 2000
-    slots = steps + 2
+    grid = grid_bound(config)
+    slots = steps + 2   # window and pending slice
     # per slot and grid node: the corner-packed record (160 B in float64, 80 B
-    # in the time-innermost float32 ring) and the cell-major cell (48 B / 24 B)
+    # in float32) and the cell-major cell (40 B / 20 B)
     rec = (80 + 4 * _lib.CELL) if precision == "mixed" else (160 + 8 * _lib.CELL)
     hist = slots * grid * grid * rec
     raw = slots * 5 * nb * nb * 8
@@ -94,7 +108,11 @@ class BatchedWake:
         self.lib = _lib.load()
         self.max_tasks = max_tasks
         nbytes = int(self.lib.dfcsr_wake_task_table_bytes(max_tasks))
-        self.host = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True)
+        # two pinned host tables used in turn: a table is packed again only
+        # after the copy that last read it has completed (its event)
+        self.hosts = [torch.empty(nbytes, dtype=torch.uint8, pin_memory=True) for _ in range(2)]
+        self.copied = [None, None]
+        self.turn = 0
         self.dev = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
         self.part = torch.empty(0, dtype=torch.uint8, device=self.device)
         self.graphs = collections.OrderedDict()
@@ -117,8 +135,13 @@ class BatchedWake:
         dE = (_lib.c_dp * k)(*[it[3]["dE_dct"].data_ptr() for it in items])
         xk = (_lib.c_dp * k)(*[it[3]["x_kick"].data_ptr() for it in items])
         batch = _lib.WakeBatch()
+        i = self.turn
+        self.turn ^= 1
+        host = self.hosts[i]
+        if self.copied[i] is not None:
+            self.copied[i].synchronize()
         _lib.check(self.lib.dfcsr_wake_tasks_pack(k, H, L, P, self.prec, dE, xk,
-                                                  self.host.data_ptr(), self.host.numel(),
+                                                  host.data_ptr(), host.numel(),
                                                   ctypes.byref(batch)), "wake_tasks_pack")
         if batch.partial_bytes > self.part.numel():
             # freeing the old buffer is stream-ordered: torch's allocator hands it
@@ -130,9 +153,12 @@ class BatchedWake:
         nbytes = int(self.lib.dfcsr_wake_task_table_bytes(k))
         # the previous launch may still read the device task buffer: the copy
         # is ordered after it on the same stream
-        self.dev[:nbytes].copy_(self.host[:nbytes], non_blocking=True)
-        key = (k, batch.blocks_far, batch.blocks_near, batch.observers)
         stream = torch.cuda.current_stream(self.device)
+        self.dev[:nbytes].copy_(host[:nbytes], non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record(stream)
+        self.copied[i] = ev
+        key = (k, batch.blocks_far, batch.blocks_near, batch.observers)
 
         def launch(s):
             _lib.check(self.lib.dfcsr_wake_batched_launch(self.dev.data_ptr(), ctypes.byref(batch),
@@ -187,6 +213,7 @@ class Server:
         self.steps = 0
         self.ticks = 0
         self.finished = {}
+        self.failed = {}
         self.rejected = []
         self.active_samples = []
 
@@ -199,8 +226,10 @@ class Server:
         comp = dict(cfg.get("CSR_computation") or {})
         comp.setdefault("write_wakes", False)
         cfg["CSR_computation"] = comp
-        return CSR2D(config=cfg, device=self.device, precision=self.precision,
-                     keep_fp32=(self.precision == "mixed"), history_capacity=req.steps + 2)
+        sim = CSR2D(config=cfg, device=self.device, precision=self.precision,
+                    keep_fp32=(self.precision == "mixed"), history_capacity=req.steps + 2)
+        sim.DF_tracker.max_grid = grid_bound(cfg)
+        return sim
 
     def submit(self, req: Request):
         if not req.footprint:
@@ -226,7 +255,8 @@ class Server:
     def _retire(self, slot: _Slot):
         self.active.remove(slot)
         self.used -= slot.req.footprint
-        self.finished[slot.req.rid] = slot
+        if slot.req.rid not in self.failed:
+            self.finished[slot.req.rid] = slot
         if not self.record_wakes:
             slot.sim = None
             slot.gen = None
@@ -244,6 +274,10 @@ class Server:
             except StopIteration:
                 ended.append(slot)
                 continue
+            except _lib.DfcsrError as e:   # e.g. its history outgrew the admitted grid
+                self.failed[slot.req.rid] = str(e)
+                self._retire(slot)
+                continue
             (need if kind == "wake" else done).append(slot)
         if need:
             items = [s.sim.prepare_wake() for s in need]
@@ -252,9 +286,14 @@ class Server:
             else:
                 for s in need:
                     s.sim.external_wake(items[need.index(s)])
-            for s in need:
+            for s in list(need):
                 self.points += s.sim.integrand_points()
-                kind, _step = next(s.gen)          # kicks, end of step
+                try:
+                    kind, _step = next(s.gen)      # kicks, end of step
+                except _lib.DfcsrError as e:
+                    self.failed[s.req.rid] = str(e)
+                    self._retire(s)
+                    continue
                 if self.record_wakes:
                     s.wake_results.append((s.sim.dE_dct.detach().clone(),
                                            s.sim.x_kick.detach().clone()))
@@ -313,6 +352,7 @@ class Server:
             "queue_wait_s_p50": round(float(np.percentile(qw, 50)), 3),
             "queue_wait_s_p99": round(float(np.percentile(qw, 99)), 3),
             "finished": len(self.finished),
+            "failed": len(self.failed),
             "rejected": len(self.rejected),
             "still_waiting": len(self.waiting),
             "still_active": len(self.active),

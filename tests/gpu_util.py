@@ -27,14 +27,13 @@ def corner_records(data, rec_len=20):
 
 
 def ring64(records, cap, head):
-    """Corner-packed records (nt, Xi, Zi, 20) -> a slot-innermost device ring
-    (Xi, Zi, 20, cap + 1) (the float64 ring, or the float32 one), logical
-    slice i in slot (head + i) % cap, slot cap a copy of slot 0."""
+    """Corner-packed records (nt, Xi, Zi, 20) -> a time-innermost device ring
+    (Xi, Zi, cap, 20) (the float64 ring, or the float32 time-innermost one),
+    logical slice i in slot (head + i) % cap."""
     nt, Xi, Zi, n = records.shape
-    ring = np.zeros((Xi, Zi, n, cap + 1), dtype=records.dtype)
+    ring = np.zeros((Xi, Zi, cap, n), dtype=records.dtype)
     for i in range(nt):
-        ring[..., (head + i) % cap] = records[i]
-    ring[..., cap] = ring[..., 0]
+        ring[:, :, (head + i) % cap] = records[i]
     return ring
 
 

@@ -86,16 +86,39 @@ def _params(slope0, mesh=64, nxz=200):
                            -1e-4, 1e-4, -1e-4, 1e-4, None)
 
 
-@pytest.mark.parametrize("slope0,phases", [(-1.25, 7 + 3 * 4), (0.5, 7 + 2 * 4), (2.0, 19)])
-def test_wake_workspace_follows_the_phase_plan(slope0, phases):
-    """Region 0 has 2 Nx rows in ceil(400/64) = 7 chunks, every later region
-    Nx rows in 4 chunks; 3 regions without chirp (|slope| <= 1), 4 with.  Per
-    mesh point: 2 partial sums per phase + 48 doubles of observer data."""
+def far_plan(xbins, zbins, Nz):
+    """Region 0 of a mesh, observer-major: (partial sums per mesh point,
+    workgroups).  W lanes per observer block (64, or xbins rounded up to a
+    power of two), S = 64 / W column blocks per wave, q workgroups of 4 waves
+    per (z line, observer block): enough for 8192 waves, at most 16, no more
+    column blocks than columns."""
+    W = 1
+    while W < xbins and W < 64:
+        W *= 2
+    S = 64 // W
+    nob = -(-xbins // W)
+    lines = zbins * nob
+    q = -(-(8192 // 4) // lines)
+    q = max(1, min(q, 16, max(1, -(-Nz // (4 * S)))))
+    return 4 * S * q, zbins * nob * q
+
+
+@pytest.mark.parametrize("slope0,near", [(-1.25, 3 * 4), (0.5, 2 * 4), (2.0, 3 * 4)])
+def test_wake_workspace_follows_the_phase_plan(slope0, near):
+    """Every later region has Nx rows in 4 chunks; 3 regions without chirp
+    (|slope| <= 1), 4 with.  Region 0 of a mesh: one partial sum per column
+    block (64 for a 64 x 64 mesh); of explicit observers: 2 Nx rows in
+    ceil(400/64) = 7 chunks.  Per mesh point: 2 partial sums per phase or
+    column block + 48 doubles of observer data."""
     import ctypes
     lib = _lib.load()
     P = _params(slope0)
     M = 64 * 64
-    assert lib.dfcsr_wake_workspace_bytes(ctypes.byref(P)) == M * (2 * phases + 48) * 8 + 256
+    far, _ = far_plan(64, 64, 200)
+    assert far == 64
+    assert lib.dfcsr_wake_workspace_bytes(ctypes.byref(P)) == M * (2 * (far + near) + 48) * 8 + 256
+    P.observers = 8
+    assert lib.dfcsr_wake_workspace_bytes(ctypes.byref(P)) == M * (2 * (7 + near) + 48) * 8 + 256
 
 
 def test_wake_tasks_pack_host_logic():
@@ -116,9 +139,10 @@ def test_wake_tasks_pack_host_logic():
                                    ctypes.byref(batch))
     assert rc == 0, lib.dfcsr_last_error()
     M = [64, 256, 16]
-    ph = [(7, 12), (7, 8), (7, 12)]
+    plans = [far_plan(b, b, 200) for b in (8, 16, 4)]
+    ph = [(plans[0][0], 12), (plans[1][0], 8), (plans[2][0], 12)]
     assert batch.tasks == k and batch.observers == sum(M)
-    assert batch.blocks_far == sum(m * a for m, (a, _) in zip(M, ph))
+    assert batch.blocks_far == sum(wg for _, wg in plans)
     assert batch.blocks_near == sum(m * b for m, (_, b) in zip(M, ph))
     assert batch.partial_bytes == sum(m * (2 * (a + b) + 48) * 8 for m, (a, b) in zip(M, ph))
     assert lib.dfcsr_wake_batched_workspace_bytes(k, P) == nb + batch.partial_bytes

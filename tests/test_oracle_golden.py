@@ -138,16 +138,14 @@ def test_c_oracle_matches_golden_wake(golden_dir):
         dE, xk = C.wake(H, Lr, 0.3, b, 0.05, 1.0, 12, 16, float(w["csr_scaling"]), obs[rows], 2)
         res = np.stack([dE, xk], axis=1)
         assert np.abs(res - w[key]).max() <= 1e-12 * np.abs(w[key]).max()
-    # the device ring layout ([Xi][Zi][20][slot] corner-packed values, field f
-    # at value 4 f, the slot fastest, the ring wrapping around) reads the same
-    # history
+    # the device ring layout ([Xi][Zi][slot][20] corner-packed records, field
+    # f at element 4 f, the ring wrapping around) reads the same history
     data = np.stack([T.data[f] for f in O.FIELDS], axis=-1)
     nt, Xi, Zi = data.shape[:3]
     cap, head = nt + 3, 5
-    rec = np.full((Xi, Zi, 20, cap + 1), np.nan)
+    rec = np.zeros((Xi, Zi, cap, 20))
     for i in range(nt):
-        rec[:, :, 0::4, (head + i) % cap] = data[i]
-    rec[..., cap] = rec[..., 0]
+        rec[:, :, (head + i) % cap, 0::4] = data[i]
     a = T.axes()
     H2 = C.hist_from_records(rec, head, nt, (a["min_t"], a["min_x"], a["min_z"], a["dt"],
                                              a["dx"], a["dz"]), keep)

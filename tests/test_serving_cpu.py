@@ -89,3 +89,54 @@ def test_mix_requests_ranges():
         assert 100 <= r.steps <= 800
         fp = request_footprint(r.config, r.steps)
         assert fp > r.steps * 600 * 600 * 80
+
+
+def test_admission_on_computed_footprints():
+    """Requests admitted on the footprints request_footprint computes (not
+    given ones): against a budget that holds the larger of two requests but
+    not both, the second waits until the first retires."""
+    reqs = mix_requests(n=2, rate=1000.0, seed=11, steps_range=(20, 30))
+    fps = [request_footprint(r.config, r.steps) for r in reqs]
+    log = []
+    srv = _server(budget=max(fps) + min(fps) - 1, max_active=8, log=log)
+    for r in reqs:
+        assert srv.submit(Request(r.rid, r.config, 0.0, r.steps))
+    assert [q.footprint for q in srv.waiting] == fps
+    srv.tick(0.0)
+    assert [s.req.rid for s in srv.active] == [reqs[0].rid] and srv.used == fps[0]
+    rep = srv.run(simulated_clock=True)
+    assert rep["finished"] == 2 and srv.used == 0
+
+
+def test_regrid_past_the_admitted_grid_fails_the_request():
+    """A history whose re-grid (int(500 sigma_max / sigma_min) per side,
+    deposit.py:355-363) would pass the grid it was admitted for raises, and
+    the server retires that request as failed, freeing its memory."""
+    from pydfcsr_amd._lib import DfcsrError
+    from pydfcsr_amd.deposit import DF_tracker, SliceMeta
+    from pydfcsr_amd.serving import grid_bound
+    tr = DF_tracker(dict(xbins=32, zbins=32), device="cpu")
+    tr.max_grid = grid_bound({"particle_deposition": {}})
+    assert tr.max_grid == 2000
+    tr.max_grid = 600
+    for t, sx in ((0.0, 10e-6), (0.1, 20e-6)):
+        tr.cur = SliceMeta(t, sx, 50e-6, 0.0, 0.0, 32, 32, -5 * sx, 5 * sx, -2.5e-4, 2.5e-4,
+                           10.0, 7)
+        tr.append_DF()
+    with pytest.raises(DfcsrError, match="admitted"):
+        tr.append_interpolant(float("inf"), 4)
+
+    class FailingSim(StubSim):
+        def iter_phases(self, external_wake=True):
+            yield ("wake", 1)
+            yield ("done", 1)
+            raise DfcsrError("re-grid to 1000 x 500 exceeds the 600 per side this history "
+                             "was admitted for")
+
+    srv = Server(device="cpu", hbm_budget=100, max_active=4, clock=Clock(),
+                 sim_factory=lambda r: (FailingSim(r, []) if r.rid == 0 else StubSim(r, [])))
+    srv.submit(Request(0, {}, 0.0, 3, footprint=60))
+    srv.submit(Request(1, {}, 0.0, 2, footprint=60))
+    rep = srv.run(simulated_clock=True)
+    assert rep["failed"] == 1 and rep["finished"] == 1 and srv.used == 0
+    assert "admitted" in srv.failed[0]

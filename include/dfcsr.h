@@ -285,6 +285,21 @@ int dfcsr_wake_mesh(const DfcsrHistory *hist, const DfcsrLattice *lat, const Dfc
                     int precision, double *xmesh, double *zmesh, double *dE_dct, double *x_kick,
                     void *ws, int64_t ws_bytes, void *stream);
 
+/* dfcsr_wake_mesh without a workspace argument, for callers that cannot
+ * allocate (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  The partial sums and
+ * observer data live in a static device array of DFCSR_WAKE_SCRATCH_BYTES
+ * per GPU, held until the process ends; a mesh whose workspace
+ * (dfcsr_wake_workspace_bytes) exceeds it is rejected with DFCSR_ERANGE
+ * (32 MiB: up to a 128 x 128 mesh on either branch at 200 x 200
+ * integration).  Calls on one GPU share that array: calls that overlap on
+ * one GPU (concurrent streams or threads) race.  Calls no allocating
+ * routine. */
+#define DFCSR_WAKE_SCRATCH_BYTES ((int64_t)32 << 20)
+int dfcsr_wake_mesh_noalloc(const DfcsrHistory *hist, const DfcsrLattice *lat,
+                            const DfcsrWakeParams *p, int precision, double *xmesh,
+                            double *zmesh, double *dE_dct, double *x_kick, void *stream);
+
 /* Batched form for serving: k independent simulations (each its own history,
  * lattice and parameters) in one launch, 1 <= k <= DFCSR_WAKE_MAX_TASKS.  All
  * descriptor arrays are HOST arrays of length k; output pointers per

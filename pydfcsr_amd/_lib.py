@@ -118,6 +118,9 @@ _SIGNATURES = [
     ("dfcsr_wake_mesh", c_i32, [ctypes.POINTER(History), ctypes.POINTER(Lattice),
                                 ctypes.POINTER(WakeParams), c_i32, c_dp, c_dp, c_dp, c_dp, c_dp,
                                 c_i64, c_dp]),
+    ("dfcsr_wake_mesh_noalloc", c_i32, [ctypes.POINTER(History), ctypes.POINTER(Lattice),
+                                        ctypes.POINTER(WakeParams), c_i32, c_dp, c_dp, c_dp,
+                                        c_dp, c_dp]),
     ("dfcsr_wake_batched_workspace_bytes", c_i64, [c_i32, ctypes.POINTER(WakeParams)]),
     ("dfcsr_wake_task_table_bytes", c_i64, [c_i32]),
     ("dfcsr_wake_mesh_batched", c_i32, [c_i32, ctypes.POINTER(History), ctypes.POINTER(Lattice),

@@ -26,6 +26,7 @@
 // ~130 MB, more than an L2 but within the 256 MB last-level cache: all XCDs
 // sweep the same z line together, each taking groups of 16 neighbouring
 // workgroups.
+#include <mutex>
 #include <vector>
 
 #include "csr_point.h"
@@ -1244,3 +1245,62 @@ extern "C" int dfcsr_wake_mesh_batched(int k, const DfcsrHistory *hist, const Df
                                    ws_bytes - tb, stream);
 }
 
+// ---- the entry point that allocates nothing --------------------------------
+//
+// dfcsr_wake_mesh_noalloc keeps its partial sums and observer data in a
+// file-scope device array instead of a caller's workspace.  Each GPU the
+// library runs on holds its own copy of the array until the process ends;
+// host code reaches it only through the per-device address hipGetSymbolAddress
+// returns (the array's name in host code is not a device address).  Every
+// phase workgroup writes its partial sums and the observer launch its
+// observer data before they are read, so nothing is cleared between calls;
+// calls that overlap on one GPU share the array and race.
+namespace dfcsr {
+namespace {
+
+static __device__ __align__(256) double scratch[DFCSR_WAKE_SCRATCH_BYTES / sizeof(double)];
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+void *scratch_dev[kMaxDevices] = {};
+
+// Address of the current device's copy of scratch, looked up once per device.
+int scratch_address(void **out, const char *who) {
+  int dev = -1;
+  DFCSR_CHECK_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices)
+    return set_error(DFCSR_ERANGE, "%s: device %d (at most %d devices)", who, dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_dev[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_error(DFCSR_EHIP, "%s: hipGetSymbolAddress(scratch) on device %d: %s", who, dev,
+                       hipGetErrorString(e));
+    scratch_dev[dev] = p;
+  }
+  *out = scratch_dev[dev];
+  return DFCSR_OK;
+}
+
+}  // namespace
+}  // namespace dfcsr
+
+extern "C" int dfcsr_wake_mesh_noalloc(const DfcsrHistory *hist, const DfcsrLattice *lat,
+                                       const DfcsrWakeParams *p, int precision, double *xmesh,
+                                       double *zmesh, double *dE_dct, double *x_kick,
+                                       void *stream) {
+  int rc;
+  if ((rc = check_params(p, "wake_mesh_noalloc"))) return rc;
+  const int64_t need = dfcsr_wake_workspace_bytes(p);
+  if (need > (int64_t)DFCSR_WAKE_SCRATCH_BYTES)
+    return set_error(DFCSR_ERANGE,
+                     "wake_mesh_noalloc: %d x %d mesh points need %lld bytes of scratch, more "
+                     "than the %lld the library holds (use dfcsr_wake_mesh with a workspace)",
+                     p->mesh_xbins, p->mesh_zbins, (long long)need,
+                     (long long)DFCSR_WAKE_SCRATCH_BYTES);
+  void *ws = nullptr;
+  if ((rc = scratch_address(&ws, "wake_mesh_noalloc"))) return rc;
+  return dfcsr_wake_mesh(hist, lat, p, precision, xmesh, zmesh, dE_dct, x_kick, ws,
+                         (int64_t)DFCSR_WAKE_SCRATCH_BYTES, stream);
+}

@@ -17,11 +17,19 @@ in a FIFO queue.  At every tick the server
   4. resumes each simulation to apply its kicks, retires finished ones and
      frees their memory, so waiting requests get in at the next tick.
 
+With overlap (the default for real simulations) each active simulation runs
+its host steps in a worker thread on a HIP stream of its own, so one
+simulation's host work and device readbacks overlap the others' tracking,
+deposition and re-grid kernels; the batched wake waits for every
+simulation's stream by events and each simulation's kicks wait for the
+wake.
+
 Each simulation's wake is computed by the same per-workgroup code whether it
 runs alone or in a mix, so its results do not depend on its neighbours.
-Step latency is the wall time of the tick that completed the step; the
-report gives p50/p99 of it with throughput (simulation steps/s and integrand
-points/s).
+Step latency is the wall time of the tick that completed the step; request
+latency runs from a request's arrival to the end of its last step; the
+report gives p50/p99 of both with throughput (simulation steps/s and
+integrand points/s).
 """
 from __future__ import annotations
 
@@ -54,6 +62,9 @@ class _Slot:
     admitted_at: float
     steps_done: int = 0
     wake_results: list = field(default_factory=list)
+    last_wake: tuple = None
+    stream: object = None
+    finished_at: float = None
 
 
 # interp-grid side a request without an integer upper_limit is admitted for
@@ -190,11 +201,21 @@ class Server:
     """Continuous-batching request loop (module docstring)."""
 
     def __init__(self, device="cuda", hbm_budget=None, max_active=64, precision="mixed",
-                 use_graph=True, sim_factory=None, clock=None, record_wakes=False):
+                 use_graph=True, sim_factory=None, clock=None, record_wakes=False,
+                 overlap=None, workers=8):
+        """overlap: each active simulation runs its host step in a worker
+        thread on a stream of its own (default: on for real simulations), so
+        one simulation's host work and readbacks overlap other simulations'
+        kernels; the batched wake waits for all of them by events."""
         self.device = device
         self.precision = precision
         self.max_active = max_active
         self.record_wakes = record_wakes
+        self.overlap = (sim_factory is None) if overlap is None else bool(overlap)
+        self.pool = None
+        if self.overlap:
+            from concurrent.futures import ThreadPoolExecutor
+            self.pool = ThreadPoolExecutor(max_workers=max(1, min(int(workers), 16)))
         self.waiting = collections.deque()
         self.active: list[_Slot] = []
         self.used = 0
@@ -248,7 +269,12 @@ class Server:
             self.waiting.popleft()
             sim = self.sim_factory(req)
             gen = sim.iter_phases(external_wake=True)
-            self.active.append(_Slot(req, sim, gen, now))
+            slot = _Slot(req, sim, gen, now)
+            if self.overlap:
+                import torch
+                slot.stream = torch.cuda.Stream(torch.device(self.device))
+                slot.stream.wait_stream(torch.cuda.current_stream(torch.device(self.device)))
+            self.active.append(slot)
             self.used += req.footprint
             self.queue_waits.append(now - req.arrival)
 
@@ -261,46 +287,106 @@ class Server:
             slot.sim = None
             slot.gen = None
 
+    def _on_stream(self, slot, fn):
+        if slot.stream is None:
+            return fn()
+        import torch
+        with torch.cuda.stream(slot.stream):
+            return fn()
+
+    def _advance(self, slot):
+        """The host side of one step up to its wake (or the whole step when it
+        has none): ("wake", (H, L, P, out), event), ("done",), ("end",) or
+        ("fail", message)."""
+        def step():
+            try:
+                kind, _step = next(slot.gen)
+            except StopIteration:
+                return ("end",)
+            except _lib.DfcsrError as e:   # e.g. its history outgrew the admitted grid
+                return ("fail", str(e))
+            if kind != "wake":
+                return ("done",)
+            item = slot.sim.prepare_wake()
+            ev = None
+            if slot.stream is not None:
+                import torch
+                ev = torch.cuda.Event()
+                ev.record(slot.stream)
+            return ("wake", item, ev)
+        return self._on_stream(slot, step)
+
+    def _resume(self, slot, wake_done):
+        """The rest of the step after its wake (kicks, statistics)."""
+        def step():
+            if wake_done is not None:
+                slot.stream.wait_event(wake_done)
+            try:
+                next(slot.gen)
+            except _lib.DfcsrError as e:
+                return ("fail", str(e))
+            if self.record_wakes:
+                slot.wake_results.append((slot.sim.dE_dct.detach().clone(),
+                                          slot.sim.x_kick.detach().clone()))
+            return ("done",)
+        return self._on_stream(slot, step)
+
+    def _map(self, fn, items):
+        if self.pool is None or len(items) < 2:
+            return [fn(x) for x in items]
+        return list(self.pool.map(fn, items))
+
     def tick(self, now):
         """One scheduling round; returns the number of steps completed."""
         self._admit(now)
         if not self.active:
             return 0
         t0 = self.clock()
+        slots = list(self.active)
         need, done, ended = [], [], []
-        for slot in list(self.active):
-            try:
-                kind, _step = next(slot.gen)
-            except StopIteration:
+        for slot, r in zip(slots, self._map(self._advance, slots)):
+            if r[0] == "end":
                 ended.append(slot)
-                continue
-            except _lib.DfcsrError as e:   # e.g. its history outgrew the admitted grid
-                self.failed[slot.req.rid] = str(e)
+            elif r[0] == "fail":
+                self.failed[slot.req.rid] = r[1]
                 self._retire(slot)
-                continue
-            (need if kind == "wake" else done).append(slot)
-        if need:
-            items = [s.sim.prepare_wake() for s in need]
-            if self.batcher is not None:
-                self.batcher.run(items)
+            elif r[0] == "wake":
+                need.append((slot, r[1], r[2]))
             else:
-                for s in need:
-                    s.sim.external_wake(items[need.index(s)])
-            for s in list(need):
+                done.append(slot)
+        if need:
+            items = [it for _, it, _ in need]
+            if self.record_wakes:   # the inputs of each one's latest wake
+                for (s, _, _), it in zip(need, items):
+                    s.last_wake = (_lib.History.from_buffer_copy(it[0]),
+                                   _lib.Lattice.from_buffer_copy(it[1]),
+                                   _lib.WakeParams.from_buffer_copy(it[2]))
+            wake_done = None
+            if self.batcher is not None:
+                torch = self.batcher.torch
+                main = torch.cuda.current_stream(torch.device(self.device))
+                for _, _, ev in need:
+                    if ev is not None:
+                        main.wait_event(ev)
+                self.batcher.run(items)
+                if self.overlap:
+                    wake_done = torch.cuda.Event()
+                    wake_done.record(main)
+            else:
+                for (s, _, _), it in zip(need, items):
+                    s.sim.external_wake(it)
+            resumed = [s for s, _, _ in need]
+            for s, r in zip(resumed, self._map(lambda sl: self._resume(sl, wake_done), resumed)):
                 self.points += s.sim.integrand_points()
-                try:
-                    kind, _step = next(s.gen)      # kicks, end of step
-                except _lib.DfcsrError as e:
-                    self.failed[s.req.rid] = str(e)
+                if r[0] == "fail":
+                    self.failed[s.req.rid] = r[1]
                     self._retire(s)
-                    continue
-                if self.record_wakes:
-                    s.wake_results.append((s.sim.dE_dct.detach().clone(),
-                                           s.sim.x_kick.detach().clone()))
-                done.append(s)
+                else:
+                    done.append(s)
         if self.batcher is not None:
             self.batcher.torch.cuda.synchronize()
-        dt = self.clock() - t0
+        t1 = self.clock()
+        dt = t1 - t0
         for s in done:
             s.steps_done += 1
             self.latencies.append(dt)
@@ -310,6 +396,7 @@ class Server:
         self.active_samples.append(len(self.active))
         for s in ended:
             if s in self.active:
+                s.finished_at = now + dt   # in the run's time frame, as arrivals
                 self._retire(s)
         self.ticks += 1
         return len(done)
@@ -341,6 +428,8 @@ class Server:
     def report(self, elapsed):
         lat = np.array(self.latencies) if self.latencies else np.zeros(1)
         qw = np.array(self.queue_waits) if self.queue_waits else np.zeros(1)
+        done = [sl for sl in self.finished.values() if sl.finished_at is not None]
+        rl = np.array([sl.finished_at - sl.req.arrival for sl in done]) if done else np.zeros(1)
         return {
             "elapsed_s": round(elapsed, 3),
             "ticks": self.ticks,
@@ -351,6 +440,8 @@ class Server:
             "step_latency_ms_p99": round(float(np.percentile(lat, 99)) * 1e3, 3),
             "queue_wait_s_p50": round(float(np.percentile(qw, 50)), 3),
             "queue_wait_s_p99": round(float(np.percentile(qw, 99)), 3),
+            "request_latency_s_p50": round(float(np.percentile(rl, 50)), 3),
+            "request_latency_s_p99": round(float(np.percentile(rl, 99)), 3),
             "finished": len(self.finished),
             "failed": len(self.failed),
             "rejected": len(self.rejected),
@@ -364,15 +455,54 @@ class Server:
         }
 
 
-# -- the C4 request mix -------------------------------------------------------------------
+# -- the request mix ------------------------------------------------------------------------
+def make_request(rid, arrival, steps, n_particle, grid, mesh, lattice, strength, sigma_x,
+                 sigma_z, chirp, seed, upper_limit=600, Nxz=200, precision="mixed"):
+    """One request: a Gaussian beam (1 nC, 1 GeV, energy spread 1e-4) through
+    a single 0.5 m dipole of radius `strength` [m] (lattice 'dipole') or a
+    4-bend chicane (0.3 m bends at +, -, -, + `strength` rad, drifts 2/1/2 m;
+    lattice 'chicane'), in `steps` steps; deposition grid `grid` per side
+    (SG 7/2), wake mesh `mesh` per side, integration Nxz x Nxz, history grid
+    capped at upper_limit per side."""
+    from collections import OrderedDict
+    lat = OrderedDict()
+    if lattice == "dipole":
+        L = 0.5
+        lat["step_size"] = L / steps
+        lat["B1"] = dict(type="dipole", L=L, angle=L / strength, nsep=1)
+    else:
+        ang = strength
+        total = 4 * 0.3 + 5.0
+        lat["step_size"] = total / steps
+        lat["B1"] = dict(type="dipole", L=0.3, angle=ang, E1=0.0, E2=ang, nsep=1)
+        lat["D1"] = dict(type="drift", L=2.0, nsep=1)
+        lat["B2"] = dict(type="dipole", L=0.3, angle=-ang, E1=-ang, E2=0.0, nsep=1)
+        lat["D2"] = dict(type="drift", L=1.0, nsep=1)
+        lat["B3"] = dict(type="dipole", L=0.3, angle=-ang, E1=0.0, E2=-ang, nsep=1)
+        lat["D3"] = dict(type="drift", L=2.0, nsep=1)
+        lat["B4"] = dict(type="dipole", L=0.3, angle=ang, E1=ang, E2=0.0, nsep=1)
+    cfg = {
+        "input_beam": dict(style="gaussian", n_particle=int(n_particle), charge=1e-9,
+                           energy=1e9, sigma_x=sigma_x, sigma_z=sigma_z, chirp=chirp,
+                           sigma_delta=1e-4, seed=int(seed)),
+        "input_lattice": dict(lattice_input_file=f"<request {rid}>", _config=lat),
+        "particle_deposition": dict(xbins=int(grid), zbins=int(grid), xlim=5, zlim=5,
+                                    filter_order=2, filter_window=7, velocity_threhold=1000,
+                                    upper_limit=upper_limit),
+        "CSR_integration": dict(n_formation_length=4, zbins=Nxz, xbins=Nxz),
+        "CSR_computation": dict(xbins=int(mesh), zbins=int(mesh), xlim=5, zlim=5, apply_CSR=1,
+                                transverse_on=1, write_wakes=False),
+    }
+    return Request(rid, cfg, arrival, int(steps), precision)
+
+
 def mix_requests(n=64, rate=2.0, seed=0, steps_range=(100, 800), particles_range=(1e5, 1e7),
                  grids=(64, 128, 256), meshes=(32, 64, 128), upper_limit=600, Nxz=200,
                  precision="mixed"):
     """Seeded request stream: Poisson arrivals at `rate` per second; particle
     count log-uniform; deposition grid, wake mesh, lattice (single dipole with
     rho 5-20 m, or a 4-bend chicane at 0.02-0.08 rad), history length, beam
-    sizes (20-100 um) and chirp (-20..0 /m) drawn per request."""
-    from collections import OrderedDict
+    sizes (20-100 um) and chirp (-20..0 /m) drawn per request (make_request)."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
@@ -383,36 +513,14 @@ def mix_requests(n=64, rate=2.0, seed=0, steps_range=(100, 800), particles_range
                                       math.log10(particles_range[1])))
         nb = int(rng.choice(grids))
         mesh = int(rng.choice(meshes))
-        lat = OrderedDict()
         if rng.random() < 0.5:
-            rho = rng.uniform(5.0, 20.0)
-            L = 0.5
-            lat["step_size"] = L / steps
-            lat["B1"] = dict(type="dipole", L=L, angle=L / rho, nsep=1)
+            kind, strength = "dipole", rng.uniform(5.0, 20.0)
         else:
-            ang = rng.uniform(0.02, 0.08)
-            total = 4 * 0.3 + 5.0
-            lat["step_size"] = total / steps
-            lat["B1"] = dict(type="dipole", L=0.3, angle=ang, E1=0.0, E2=ang, nsep=1)
-            lat["D1"] = dict(type="drift", L=2.0, nsep=1)
-            lat["B2"] = dict(type="dipole", L=0.3, angle=-ang, E1=-ang, E2=0.0, nsep=1)
-            lat["D2"] = dict(type="drift", L=1.0, nsep=1)
-            lat["B3"] = dict(type="dipole", L=0.3, angle=-ang, E1=0.0, E2=-ang, nsep=1)
-            lat["D3"] = dict(type="drift", L=2.0, nsep=1)
-            lat["B4"] = dict(type="dipole", L=0.3, angle=ang, E1=ang, E2=0.0, nsep=1)
-        cfg = {
-            "input_beam": dict(style="gaussian", n_particle=npart, charge=1e-9, energy=1e9,
-                               sigma_x=rng.uniform(20e-6, 100e-6),
-                               sigma_z=rng.uniform(20e-6, 100e-6),
-                               chirp=rng.uniform(-20.0, 0.0), sigma_delta=1e-4,
-                               seed=int(rng.integers(1 << 30))),
-            "input_lattice": dict(lattice_input_file=f"<request {rid}>", _config=lat),
-            "particle_deposition": dict(xbins=nb, zbins=nb, xlim=5, zlim=5, filter_order=2,
-                                        filter_window=7, velocity_threhold=1000,
-                                        upper_limit=upper_limit),
-            "CSR_integration": dict(n_formation_length=4, zbins=Nxz, xbins=Nxz),
-            "CSR_computation": dict(xbins=mesh, zbins=mesh, xlim=5, zlim=5, apply_CSR=1,
-                                    transverse_on=1, write_wakes=False),
-        }
-        out.append(Request(rid, cfg, t, steps, precision))
+            kind, strength = "chicane", rng.uniform(0.02, 0.08)
+        sx = rng.uniform(20e-6, 100e-6)
+        sz = rng.uniform(20e-6, 100e-6)
+        chirp = rng.uniform(-20.0, 0.0)
+        bseed = int(rng.integers(1 << 30))
+        out.append(make_request(rid, t, steps, npart, nb, mesh, kind, strength, sx, sz, chirp,
+                                bseed, upper_limit, Nxz, precision))
     return out

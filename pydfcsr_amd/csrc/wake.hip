@@ -883,8 +883,10 @@ __global__ __launch_bounds__(kThreads, kCells ? (kMixed ? 4 : 3) : DFCSR_WAKE_MI
 }
 
 // The observer-major region-0 launch: each XCD a contiguous range of z lines.
+// Three workgroups per CU (FP64: 168 VGPRs and a few spilled, against 191 at
+// two; large run FP64 far phases -10 %).
 template <bool kMixed, bool kTI>
-__global__ __launch_bounds__(kThreads, DFCSR_WAKE_MINBLOCKS) void wake_far_kernel(
+__global__ __launch_bounds__(kThreads, 3) void wake_far_kernel(
     WakeTask T, double *__restrict__ partials) {
   __shared__ RowTable rt;
   far_obs_block<kMixed, kTI>(T, xcd_remap(blockIdx.x, gridDim.x), partials, rt);

@@ -553,8 +553,9 @@ def main():
                      "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
                      # bytes per wake from the committed counter runs of this state
                      "traffic": traffic,
-                     "kernel": f"{rkernel} (wake_observer_kernel, wake_phase_kernel far and "
-                               "near, wake_finalize_kernel; one stream, timed together)",
+                     "kernel": f"{rkernel} (wake_observer_kernel, wake_far_kernel for region 0, "
+                               "wake_phase_kernel for the near regions, wake_finalize_kernel; "
+                               "one stream, timed together)",
                      "flops_per_point": FLOPS_PER_POINT,
                      "points": "evaluated",
                      "nominal_equivalent_frac": round(achieved / frac_eval / peak, 4)},

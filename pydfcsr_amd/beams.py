@@ -158,13 +158,28 @@ class Beam:
 
     # -- statistics (beams.py:88-98, 137-215) ---------------------------------
     def update_status(self):
-        m = kernels.beam_moments(self.x, self.z, self.ws)
-        self.moments = m
-        self._sigma_x = m["sigma_x"]
-        self._sigma_z = m["sigma_z"]
-        self._slope = m["slope"]
-        self._mean_x = m["mean_x"]
-        self._mean_z = m["mean_z"]
+        """Launches the moments of the current coordinates; they stay on the
+        device (moments_device) until a host value is read."""
+        self._moments_dev = kernels.beam_moments(self.x, self.z, self.ws, host=False)
+        self._moments_host = None
+
+    @property
+    def moments_device(self):
+        """Device float64 vector of the DFCSR_MOMENTS_* values."""
+        return self._moments_dev
+
+    @property
+    def moments(self):
+        """The moments as a dict of floats (copied from the device on first read)."""
+        if self._moments_host is None:
+            self._moments_host = kernels.moments_dict(self._moments_dev)
+        return self._moments_host
+
+    _sigma_x = property(lambda self: self.moments["sigma_x"])
+    _sigma_z = property(lambda self: self.moments["sigma_z"])
+    _slope = property(lambda self: self.moments["slope"])
+    _mean_x = property(lambda self: self.moments["mean_x"])
+    _mean_z = property(lambda self: self.moments["mean_z"])
 
     def track(self, element, step_size, update_step=True):   # beams.py:101-106
         """Tracks through one element piece (an Element from bmadx_element)

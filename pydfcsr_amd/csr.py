@@ -104,9 +104,11 @@ class CSR2D:
         for k in ("sigma_x", "sigma_z", "sigma_energy", "mean_x", "mean_z", "mean_energy"):
             st[k] = np.zeros(n)
         self._statistics = st
-        # Twiss and energy rows stay on the device until the dict is read
+        # moments, Twiss and energy rows stay on the device until the dict is read
         self._stats_dev = torch.zeros((n, _lib.STATS_COUNT), dtype=torch.float64,
                                       device=self.device)
+        self._moments_rows = torch.zeros((n, _lib.MOMENTS_COUNT), dtype=torch.float64,
+                                         device=self.device)
         self._stats_dirty = False
         self.update_statistics(step=0)
         self.inbend = False
@@ -118,13 +120,8 @@ class CSR2D:
         if step >= len(self._statistics["sigma_x"]):
             return
         b = self.beam
-        st = self._statistics
-        st["slope"][step, :] = b._slope
-        st["sigma_x"][step] = b._sigma_x
-        st["sigma_z"][step] = b._sigma_z
-        st["mean_x"][step] = b._mean_x
-        st["mean_z"][step] = b._mean_z
-        b.statistics(out=self._stats_dev[step])   # Twiss, energy: device row, no sync
+        self._moments_rows[step].copy_(b.moments_device)   # device rows, no sync
+        b.statistics(out=self._stats_dev[step])
         self._stats_dirty = True
 
     @property
@@ -134,6 +131,13 @@ class CSR2D:
         sigma_z, sigma_energy, mean_x, mean_z, mean_energy.  Reading it copies
         the device-side rows once."""
         if self._stats_dirty:
+            m = self._moments_rows.cpu().numpy()
+            st = self._statistics
+            col = kernels.MOMENT_KEYS.index
+            st["slope"][:, 0] = m[:, col("slope0")]
+            st["slope"][:, 1] = m[:, col("slope1")]
+            for k in ("sigma_x", "sigma_z", "mean_x", "mean_z"):
+                st[k][:] = m[:, col(k)]
             v = self._stats_dev.cpu().numpy()
             tw = self._statistics["twiss"]
             for plane, off in (("x", _lib.STAT_TWISS_X), ("y", _lib.STAT_TWISS_Y)):

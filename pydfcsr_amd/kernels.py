@@ -25,7 +25,7 @@ import torch
 from . import _lib
 from ._lib import DfcsrError, check, ptr, require_device_tensor, stream_handle
 
-_MOMENT_KEYS = ("mean_x", "mean_z", "sigma_x", "sigma_z", "slope0", "slope1", "mean_xt",
+MOMENT_KEYS = ("mean_x", "mean_z", "sigma_x", "sigma_z", "slope0", "slope1", "mean_xt",
                 "sigma_xt", "slice_sigma_x", "n_slice")
 
 
@@ -47,8 +47,13 @@ def beam_moments(x, z, ws: _lib.Workspace, out: torch.Tensor | None = None, host
                                  stream_handle(x.device)), "beam_moments")
     if not host:
         return out
+    return moments_dict(out)
+
+
+def moments_dict(out: torch.Tensor):
+    """The moments vector as a dict of floats, plus "slope" = [slope0, slope1]."""
     v = out.cpu().numpy()
-    d = dict(zip(_MOMENT_KEYS, (float(a) for a in v)))
+    d = dict(zip(MOMENT_KEYS, (float(a) for a in v)))
     d["slope"] = np.array([d["slope0"], d["slope1"]])
     return d
 

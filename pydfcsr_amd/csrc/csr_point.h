@@ -5,8 +5,10 @@
 //
 // The floating-point contraction mode is the including file's: the parity
 // entry points compile this without fused multiply-add and with true
-// divisions (kRecip = false) so they reproduce the reference's rounding; the
-// wake kernels multiply by precomputed reciprocals (kRecip = true).
+// divisions and square roots (kRecip = false) so they reproduce the
+// reference's rounding; the wake kernels multiply by precomputed reciprocals
+// and take |r - r'|, 1/|r - r'| and 1/scale from the hardware estimates
+// refined by Newton steps (kRecip = true, a few ulp).
 #pragma once
 
 #include "common.h"
@@ -244,6 +246,25 @@ __device__ inline Column make_column(const DfcsrLattice &L, const Observer &o, d
   return c;
 }
 
+// 1/sqrt(a) for a normal a > 0: the hardware estimate and two Newton steps
+// y += y (1 - a y^2) / 2, each squaring the relative error (a few ulp).
+__device__ inline double rsqrt_newton(double a) {
+  double y = __builtin_amdgcn_rsq(a);
+  double e = fma(-(a * y), y, 1.0);
+  y = fma(0.5 * y, e, y);
+  e = fma(-(a * y), y, 1.0);
+  return fma(0.5 * y, e, y);
+}
+
+// 1/a for a normal a: the hardware estimate and two Newton steps r += r (1 - a r).
+__device__ inline double recip_newton(double a) {
+  double r = __builtin_amdgcn_rcp(a);
+  double e = fma(-a, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-a, r, 1.0);
+  return fma(r, e, r);
+}
+
 // One integrand point (I_z, I_x) at source (x', s'), CSR.py:645-775 in the
 // reference's operation order.  Returns false (I = 0) when the retarded
 // lookup falls outside the history, where every gathered field is zero and
@@ -256,7 +277,14 @@ __device__ inline bool csr_point(const DfcsrHistory &H, const HistAxes &A, const
                                  const CellRow *rr = nullptr) {
   const double rx = c.ax - xp * c.L.nx;
   const double ry = c.ay - xp * c.L.ny;
-  const double R = sqrt(rx * rx + ry * ry);
+  double R, iR = 0.0;
+  if (kRecip) {
+    const double R2 = rx * rx + ry * ry;
+    iR = R2 != 0.0 ? rsqrt_newton(R2) : 0.0;
+    R = R2 * iR;
+  } else {
+    R = sqrt(rx * rx + ry * ry);
+  }
   const double t_ret = o.t - R;
   double f[5];
   if (!hist_lookup5<kRecip, kCells>(H, A, t_ret, xp, c.sp - t_ret, f, Xpre, rr) && R != 0.0) {
@@ -267,12 +295,13 @@ __device__ inline bool csr_point(const DfcsrHistory &H, const HistAxes &A, const
   const double dens = f[DFCSR_F_DENSITY], dens_x = f[DFCSR_F_DENSITY_X],
                dens_z = f[DFCSR_F_DENSITY_Z], vx_ret = f[DFCSR_F_VX], div_v = f[DFCSR_F_VX_X];
   const double scale = 1 + xp * c.rho;
+  const double dzs = kRecip ? dens_z * recip_newton(scale) : dens_z / scale;
   const double vel_x = o.L.tx + o.vx * o.L.nx;
   const double vel_y = o.L.ty + o.vx * o.L.ny;
   const double vr_x = c.L.tx + vx_ret * c.L.nx;
   const double vr_y = c.L.ty + vx_ret * c.L.ny;
-  const double nab_x = dens_x * c.L.nx + dens_z / scale * c.L.tx;
-  const double nab_y = dens_x * c.L.ny + dens_z / scale * c.L.ty;
+  const double nab_x = dens_x * c.L.nx + dzs * c.L.tx;
+  const double nab_y = dens_x * c.L.ny + dzs * c.L.ty;
   const double part1 = vel_x * vr_x + vel_y * vr_y;
   const double num1 = scale * ((vel_x - part1 * vr_x) * nab_x + (vel_y - part1 * vr_y) * nab_y);
   const double num2 = -scale * part1 * dens * div_v;
@@ -281,7 +310,6 @@ __device__ inline bool csr_point(const DfcsrHistory &H, const HistAxes &A, const
   const double p2 = o.L.nx * c.L.tx + o.L.ny * c.L.ty;
   const double pd = -(vr_x * nab_x + vr_y * nab_y) - dens * div_v;
   if (kRecip && R != 0.0) {
-    const double iR = 1.0 / R;
     const double iR2 = iR * iR;
     Iz = (num1 + num2) * iR;
     Ix = scale * p1 * (iR2 * iR) * dens + scale * p1 * iR2 * pd + -scale * p2 * iR * pd;

@@ -206,8 +206,10 @@ __device__ inline int xcd_remap(int b, int nb) {
 // Those coefficients, |r - r'|, t_ret, z' = s' - t_ret, the history indices
 // and q = (r-r').(n-n')/R - n.tau' (whose two terms cancel to leading order)
 // are float64; the gathered fields, the coefficients' products and the
-// integrand are float32.  1/R comes from an FP32 reciprocal square root with
-// one FP64 Newton step; 1/scale is the FP32 hardware reciprocal.  The
+// integrand are float32.  1/R comes from the FP32 hardware reciprocal square
+// root with one FP64 Newton step (a nonzero |r - r'| between positions of
+// metres is at least ~1e-16 m, so R^2 is never an FP32 denormal); 1/scale is
+// the FP32 hardware reciprocal.  The
 // retarded indices come from one FMA each: (t_ret - min_t)/dt = ut0 - R/dt
 // and (z' - min_z)/dz = uz0 + R/dz with ut0 per wake and uz0 per column.
 // Each point's weighted FP32 integrand is added to float64 row sums of its
@@ -292,22 +294,31 @@ __device__ inline double lookup_vx32(const DfcsrHistory &H, const HistAxes &A, d
   return c0 * (1 - zd) + c1 * zd;
 }
 
-// The five fields of the four corner cells at byte offsets off[q] of a
-// float32 cell plane (20-byte cells).
-__device__ inline void load_cells32(const char *plane, const uint32_t off[4], float v[4][5]) {
+// The corners (x0,z0), (x0,z1), (x1,z0), (x1,z1) of a float32 cell plane
+// from the cell at p (x0, z0) and the one dxb bytes on (x1, z0): each x
+// row's cells z0 and z1 = z0 + 1 read as 40 contiguous bytes.
+__device__ inline void load_pairs32(const char *p, uint32_t dxb, float v[4][5]) {
+  const float *a = reinterpret_cast<const float *>(p);
+  const float *b = reinterpret_cast<const float *>(p + dxb);
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const float *p = reinterpret_cast<const float *>(plane + off[q]);
-#pragma unroll
-    for (int k = 0; k < 5; ++k) v[q][k] = p[k];
+  for (int k = 0; k < 5; ++k) {
+    v[0][k] = a[k];
+    v[1][k] = a[5 + k];
+    v[2][k] = b[k];
+    v[3][k] = b[5 + k];
   }
 }
 
+// Ways a lookup finds the history: the corner-packed rings, the cell-major
+// ring with a 64-bit plane address per lookup, or (mixed precision, a
+// float32 cell-major ring of at most 4 GiB) 32-bit byte offsets from the
+// ring's start, which the loads add to the ring's scalar address.
+enum CellMode { kRings = 0, kCellsPlane = 1, kCellsRing32 = 2 };
+
 // One source point in mixed precision; false when outside the history.
-// kTI: read the time-innermost FP32 ring, else the chunked one.  kCells:
-// read the float32 cell-major ring H.cells32 instead (same values, same
-// arithmetic).
-template <bool kTI, bool kCells = false>
+// kTI: read the time-innermost FP32 ring, else the chunked one.  kCells
+// (a CellMode): read the float32 cell-major ring H.cells32 instead.
+template <bool kTI, int kCells = kRings>
 __device__ inline bool csr_point_mixed(const DfcsrHistory &H, const HistAxes &A, double ut0,
                                        const ColumnMixed &c, double xp, float xpf, int xi0,
                                        int xi1, CellRow rr, float xd, float &Iz, float &Ix) {
@@ -318,29 +329,48 @@ __device__ inline bool csr_point_mixed(const DfcsrHistory &H, const HistAxes &A,
     Iz = Ix = NAN;
     return true;
   }
-  double y = (double)rsqrtf((float)R2);
+  double y = (double)__builtin_amdgcn_rsqf((float)R2);
   y = y * (1.5 - 0.5 * R2 * y * y);               // one Newton step: ~1e-14 relative
   const double R = R2 * y;
   const double ut = ut0 - R * A.it;               // (t - R - min_t) / dt
   const double uz = c.uz0 + R * A.iz;             // (s' - (t - R) - min_z) / dz
   const TruncIdx T = trunc_index(ut, H.nt), Z = trunc_index(uz, H.Zi);
   if (!(T.ok && Z.ok)) return false;
-  const float td = (float)T.frac, zd = (float)Z.frac;
+  const float td = (float)T.frac;
+  float zd = (float)Z.frac;
   float f[5];
-  if (kCells) {
-    // float32 cell (slot, x, z) at byte ((slot Xi + x) Zi + z) * 20 of the cell-major
-    // ring: corners (x0,z0), (x0,z1), (x1,z0), (x1,z1) of each plane, addressed by
-    // 32-bit offsets within the plane
+  if (kCells != kRings) {
+    // float32 cell (slot, x, z) at byte ((slot Xi + x) Zi + z) * 20 of the
+    // cell-major ring; the plane of the next slot follows (slot 0 after the
+    // ring's last).  Each x row's two z cells are one 40-byte pair; at the
+    // last z (interpolate3D clamps z1 to z0) the pair is the one before,
+    // (z - 1, z), with weight 1 on its second cell, so the lerp gives that
+    // cell's value within one FP32 rounding and the loads need no branch.
     constexpr uint32_t cb = DFCSR_CELL * sizeof(float);
-    const int64_t planeB = (int64_t)H.Xi * H.Zi * cb;
+    const uint32_t planeB = (uint32_t)H.Xi * (uint32_t)H.Zi * cb;   // < 4 GiB (checked)
     const char *ring = reinterpret_cast<const char *>(H.cells32);
-    const uint32_t o0 = rr.rb + (uint32_t)Z.i0 * cb, oz = Z.i1 != Z.i0 ? cb : 0u;
-    const uint32_t off[4] = {o0, o0 + oz, o0 + rr.dxb, o0 + rr.dxb + oz};
+    int z0 = Z.i0;
+    if (Z.i1 == Z.i0) {   // Zi >= 2
+      z0 -= 1;
+      zd = 1.0f;
+    }
+    const uint32_t o = rr.rb + (uint32_t)z0 * cb;
+    const int s0 = hist_slot(H, T.i0);
+    const bool wrap = s0 + 1 == H.cap;
     float v0[4][5];
-    load_cells32(ring + hist_slot(H, T.i0) * planeB, off, v0);
+    const char *p1;
+    if (kCells == kCellsRing32) {
+      const uint32_t q0 = (uint32_t)s0 * planeB + o;
+      load_pairs32(ring + (uint64_t)q0, rr.dxb, v0);
+      p1 = ring + (uint64_t)(wrap ? o : q0 + planeB);
+    } else {
+      const char *pl0 = ring + (uint64_t)(uint32_t)s0 * planeB;
+      load_pairs32(pl0 + o, rr.dxb, v0);
+      p1 = (wrap ? ring : pl0 + planeB) + o;
+    }
     if (T.i1 != T.i0) {
       float v1[4][5];
-      load_cells32(ring + hist_slot(H, T.i1) * planeB, off, v1);
+      load_pairs32(p1, rr.dxb, v1);
 #pragma unroll
       for (int k = 0; k < 5; ++k) {
         const float c00 = fmaf(td, v1[0][k] - v0[0][k], v0[0][k]);
@@ -557,9 +587,10 @@ __device__ inline void observer_point(const WakeTask &T, int i, double *__restri
 
 // Integrates phase ph of the mesh point whose observer data is od; every
 // thread returns the block sums of wz_k wx_j (I_z, I_x) over the phase.
-// kTI (mixed only): read the time-innermost FP32 ring.  kCells: read the
-// cell-major ring (H.cells for FP64, H.cells32 for mixed).
-template <bool kMixed, bool kTI, bool kCells = false>
+// kTI (mixed only): read the time-innermost FP32 ring.  kCells (a
+// CellMode): read the cell-major ring (H.cells for FP64, H.cells32 for
+// mixed).
+template <bool kMixed, bool kTI, int kCells = kRings>
 __device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int ph, double *red,
                            RowTable &rt, double &out_z, double &out_x) {
   const DfcsrHistory &H = T.H;
@@ -625,7 +656,7 @@ __device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int
     rt.xpf[r] = (float)xp;
     rt.wf[r] = (float)rt.w[r];
     rt.fracf[r] = (float)X.frac;
-    if (kCells) {
+    if (kCells != kRings) {
       const uint32_t rowB = (uint32_t)H.Zi * (uint32_t)(DFCSR_CELL *
                                                         (kMixed ? sizeof(float) : sizeof(double)));
       rt.rr[r].rb = (uint32_t)X.i0 * rowB;
@@ -671,7 +702,8 @@ __device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int
         if (!rt.ok[r]) continue;
         const TruncIdx X = {rt.i0[r], rt.i1[r], rt.frac[r], true};
         double Iz, Ix;
-        if (!csr_point<true, kCells>(H, A, o, col, rt.xp[r], Iz, Ix, &X, &rt.rr[r])) continue;
+        if (!csr_point<true, kCells != kRings>(H, A, o, col, rt.xp[r], Iz, Ix, &X, &rt.rr[r]))
+          continue;
         ++evaluated;
         const double wx = rt.w[r];
         cz += wx * Iz;
@@ -693,9 +725,9 @@ __device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int
 
 // Workgroup l of a task's far (kNear = false) or near launch -> (mesh
 // point, phase); writes the phase's partial sums.  kTI (mixed only): the
-// phases read the time-innermost FP32 ring, else the chunked one; kCells:
-// the cell-major ring.
-template <bool kMixed, bool kNear, bool kTI, bool kCells = false>
+// phases read the time-innermost FP32 ring, else the chunked one; kCells (a
+// CellMode): the cell-major ring.
+template <bool kMixed, bool kNear, bool kTI, int kCells = kRings>
 __device__ inline void run_phase_block(const WakeTask &T, int l, double *__restrict__ partials,
                                        double *red) {
   const int xb = T.P.mesh_xbins, zb = T.P.mesh_zbins;
@@ -826,7 +858,7 @@ __device__ void far_obs_block(const WakeTask &T, int l, double *__restrict__ par
         for (int r = 0; r < nr; ++r) {
           if (!rt.ok[r]) continue;
           float Iz, Ix;
-          if (!csr_point_mixed<kTI, false>(H, A, ut0, col, rt.xp[r], rt.xpf[r], rt.i0[r],
+          if (!csr_point_mixed<kTI, kRings>(H, A, ut0, col, rt.xp[r], rt.xpf[r], rt.i0[r],
                                             rt.i1[r], CellRow{0u, 0u}, rt.fracf[r], Iz, Ix))
             continue;
           ++evaluated;
@@ -871,9 +903,14 @@ __host__ __device__ inline bool mixed_reads_ti(const DfcsrHistory &H, bool near)
   return near ? (H.data32 == nullptr) : (H.data32t != nullptr);
 }
 
+// Whether every byte of the float32 cell-major ring has a 32-bit offset.
+__host__ __device__ inline bool cells32_ring32(const DfcsrHistory &H) {
+  return (int64_t)H.cap * H.Xi * H.Zi * (int64_t)(DFCSR_CELL * sizeof(float)) <= 0xffffffffLL;
+}
+
 // Workgroups per CU the register allocation must allow: the near phases on
 // the cell-major ring 3 (FP64) or 4 (mixed: 128 registers), the others 2.
-template <bool kMixed, bool kNear, bool kTI, bool kCells = false>
+template <bool kMixed, bool kNear, bool kTI, int kCells = kRings>
 __global__ __launch_bounds__(kThreads, kCells ? (kMixed ? 4 : 3) : DFCSR_WAKE_MINBLOCKS) void wake_phase_kernel(
     WakeTask T, double *__restrict__ partials) {
   __shared__ double red[kThreads / 64];
@@ -927,14 +964,16 @@ __global__ __launch_bounds__(kThreads, kMixed && kNear ? 4 : DFCSR_WAKE_MINBLOCK
     return;
   }
   if constexpr (kMixed) {   // per task: its FP32 rings, cell-major or not
-    if (kNear && T.H.cells32)
-      run_phase_block<true, kNear, false, true>(T, l, partials, red);
+    if (kNear && T.H.cells32 && cells32_ring32(T.H))
+      run_phase_block<true, kNear, false, kCellsRing32>(T, l, partials, red);
+    else if (kNear && T.H.cells32)
+      run_phase_block<true, kNear, false, kCellsPlane>(T, l, partials, red);
     else if (mixed_reads_ti(T.H, kNear))
       run_phase_block<true, kNear, true>(T, l, partials, red);
     else
       run_phase_block<true, kNear, false>(T, l, partials, red);
   } else if (kNear && T.H.cells) {   // per task: the cell-major ring or not
-    run_phase_block<false, kNear, false, true>(T, l, partials, red);
+    run_phase_block<false, kNear, false, kCellsPlane>(T, l, partials, red);
   } else {
     run_phase_block<false, kNear, false>(T, l, partials, red);
   }
@@ -1105,9 +1144,12 @@ extern "C" int dfcsr_wake_mesh(const DfcsrHistory *hist, const DfcsrLattice *lat
     else
       hipLaunchKernelGGL((wake_phase_kernel<true, false, false>), ga, dim3(kThreads), 0, s, T, part);
     const bool ti = mixed_reads_ti(*hist, true);
-    if (hist->cells32) {
-      hipLaunchKernelGGL((wake_phase_kernel<true, true, false, true>), gb, dim3(kThreads), 0, s,
-                         T, part);
+    if (hist->cells32 && cells32_ring32(*hist)) {
+      hipLaunchKernelGGL((wake_phase_kernel<true, true, false, kCellsRing32>), gb, dim3(kThreads),
+                         0, s, T, part);
+    } else if (hist->cells32) {
+      hipLaunchKernelGGL((wake_phase_kernel<true, true, false, kCellsPlane>), gb, dim3(kThreads),
+                         0, s, T, part);
     } else if (ti) {
       hipLaunchKernelGGL((wake_phase_kernel<true, true, true>), gb, dim3(kThreads), 0, s, T, part);
     } else {
@@ -1119,8 +1161,8 @@ extern "C" int dfcsr_wake_mesh(const DfcsrHistory *hist, const DfcsrLattice *lat
     else
       hipLaunchKernelGGL((wake_phase_kernel<false, false, false>), ga, dim3(kThreads), 0, s, T, part);
     if (hist->cells) {
-      hipLaunchKernelGGL((wake_phase_kernel<false, true, false, true>), gb, dim3(kThreads), 0, s, T,
-                         part);
+      hipLaunchKernelGGL((wake_phase_kernel<false, true, false, kCellsPlane>), gb, dim3(kThreads),
+                         0, s, T, part);
     } else {
       hipLaunchKernelGGL((wake_phase_kernel<false, true, false>), gb, dim3(kThreads), 0, s, T, part);
     }

@@ -1,12 +1,13 @@
 // a4-a6: density-function post-processing of one deposited slice.
 //
-// Replaces deposit.py:183-235 as nine launches on one stream: the
-// element-wise steps (smoothing stencils, gradients, thresholds) run over
-// the whole grid, and the global steps between them (the maxima, the
-// trapezoid normalisation, the masked mean of vx_x) read partials or run as
-// one fixed-order workgroup, so results are the same every launch.  The
-// velocity division and the normalisation are applied where their inputs
-// are read, in the reference's operation order.
+// Replaces deposit.py:183-235 as seven launches on one stream: the two
+// separable smoothings (density and vx, then the three derivatives) run as
+// LDS-tiled stencils, one workgroup per output tile; the gradients and the
+// element-wise steps run over the whole grid; the global steps between them
+// (the maxima, the trapezoid normalisation, the masked mean of vx_x) read
+// partials or run as one fixed-order workgroup, so results are the same
+// every launch.  The velocity division and the normalisation are applied
+// where their inputs are read, in the reference's operation order.
 //
 // Savitzky-Golay: scipy.signal.savgol_filter(mode='interp') is linear: the
 // interior is the symmetric smoothing stencil c[] (the centre row of the
@@ -104,21 +105,6 @@ __host__ __device__ inline int grid_of(int64_t N) {
   return (int)(g < 1 ? 1 : (g > kMaxGrid ? kMaxGrid : g));
 }
 
-// vx after the velocity division (deposit.py:183-184).
-struct VelDiv {
-  const double *vx, *dens;
-  double thr;
-  __device__ double operator()(int64_t e) const {
-    const double d = dens[e];
-    return d > thr ? vx[e] / d : vx[e];
-  }
-};
-
-struct Plain {
-  const double *a;
-  __device__ double operator()(int64_t e) const { return a[e]; }
-};
-
 // Savitzky-Golay output at element e of a [nx][nz] array along `axis`
 // (0: along x rows, 1: along z); in(e) gives the input at element e.
 template <typename In>
@@ -200,36 +186,78 @@ __global__ __launch_bounds__(kT) void df_max_kernel(const double *__restrict__ d
   max_partial(m, part);
 }
 
-// 2. velocity division fused into the axis-0 smoothing of density and vx
-// (deposit.py:183-184, 195-199)
-__global__ __launch_bounds__(kT) void df_sg0_kernel(SGOp op, const double *__restrict__ dens,
-                                                    const double *__restrict__ vx, int nx, int nz,
-                                                    double vt, const double *__restrict__ part,
-                                                    int npart, double *__restrict__ T0,
-                                                    double *__restrict__ T1,
-                                                    double *__restrict__ scal) {
-  const double thr = max_of(part, npart) / vt;
-  if (blockIdx.x == 0 && threadIdx.x == 0) scal[S_THR] = thr;
-  const int64_t N = (int64_t)nx * nz;
-  const VelDiv v{vx, dens, thr};
-  const Plain d{dens};
-  for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < N; e += (int64_t)gridDim.x * kT) {
-    T0[e] = sg_at(op, d, nx, nz, 0, e);
-    T1[e] = sg_at(op, v, nx, nz, 0, e);
+// Separable Savitzky-Golay smoothing of a tile: axis 0 (along x) into LDS,
+// then axis 1 (along z) from LDS.  A workgroup owns kTileX x kTileZ outputs;
+// its LDS rows hold the axis-0 results of its rows over every column the
+// axis-1 windows of its outputs read (interior window, or the first / last
+// `W` columns for the edge rows of the fit).  Each output is the same sum in
+// the same order as two grid-wide passes would give.
+constexpr int kTileX = 8, kTileZ = 32;
+constexpr int kTileW = kTileZ + 2 * DFCSR_MAX_SG_WINDOW;
+constexpr int kTileFields = 3;
+
+// Input field of a tile pass: a grid, optionally divided by the density
+// where that exceeds thr (the velocity division of deposit.py:183-184).
+struct TileIn {
+  const double *a, *dens;
+  double thr;
+  __device__ double operator()(int64_t e) const {
+    if (!dens) return a[e];
+    const double d = dens[e];
+    return d > thr ? a[e] / d : a[e];
+  }
+};
+
+// One LDS row of axis-0 results: element e of grid row i is at row[e - off].
+struct LdsRow {
+  const double *row;
+  int64_t off;
+  __device__ double operator()(int64_t e) const { return row[e - off]; }
+};
+
+template <int NF>
+__device__ void sg_tile(const SGOp &op, const TileIn (&in)[NF], double *const (&out)[NF], int nx,
+                        int nz) {
+  __shared__ double T[kTileFields][kTileX][kTileW];
+  static_assert(NF <= kTileFields, "tile fields");
+  const int x0 = blockIdx.y * kTileX, z0 = blockIdx.x * kTileZ;
+  const int h = op.h, W = op.W;
+  const int lo = max(0, min(z0 - h, nz - W));
+  const int hi = min(nz, max(z0 + kTileZ + h, W));
+  const int w = hi - lo;   // <= kTileZ + 2 W
+  const int rows = min(kTileX, nx - x0), cols = min(kTileZ, nz - z0);
+  for (int t = threadIdx.x; t < rows * w; t += blockDim.x) {
+    const int r = t / w, c = t - r * w;
+    const int64_t e = (int64_t)(x0 + r) * nz + (lo + c);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) T[f][r][c] = sg_at(op, in[f], nx, nz, 0, e);
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < rows * cols; t += blockDim.x) {
+    const int r = t / cols, c = t - r * cols;
+    const int64_t i = x0 + r;
+    const int64_t e = i * nz + (z0 + c);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      const LdsRow row{&T[f][r][0], i * nz + lo};
+      out[f][e] = sg_at(op, row, nx, nz, 1, e);
+    }
   }
 }
 
-// 3. axis-1 smoothing of both into density and vx (deposit.py:195-199)
-__global__ __launch_bounds__(kT) void df_sg1_kernel(SGOp op, const double *__restrict__ T0,
-                                                    const double *__restrict__ T1, int nx, int nz,
-                                                    double *__restrict__ dens,
-                                                    double *__restrict__ vx) {
-  const int64_t N = (int64_t)nx * nz;
-  const Plain a{T0}, b{T1};
-  for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < N; e += (int64_t)gridDim.x * kT) {
-    dens[e] = sg_at(op, a, nx, nz, 1, e);
-    vx[e] = sg_at(op, b, nx, nz, 1, e);
-  }
+// 2. the velocity division and both smoothing axes of density and vx
+// (deposit.py:183-184, 195-199), into S0 (density) and S1 (vx)
+__global__ __launch_bounds__(kT) void df_sg_kernel(SGOp op, const double *__restrict__ dens,
+                                                   const double *__restrict__ vx, int nx, int nz,
+                                                   double vt, const double *__restrict__ part,
+                                                   int npart, double *__restrict__ S0,
+                                                   double *__restrict__ S1,
+                                                   double *__restrict__ scal) {
+  const double thr = max_of(part, npart) / vt;
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) scal[S_THR] = thr;
+  const TileIn in[2] = {{dens, nullptr, 0.0}, {vx, dens, thr}};
+  double *const out[2] = {S0, S1};
+  sg_tile<2>(op, in, out, nx, nz);
 }
 
 // 4. dsum = trapz(trapz(density, x_grids, axis=0), z_grids) (deposit.py:201):
@@ -275,73 +303,50 @@ struct VxCut {
   __device__ double operator()(int64_t e) const { return (dens[e] / dsum <= thr) ? 0.0 : vx[e]; }
 };
 
-// 5. gradients of the normalised density and of vx (deposit.py:212-213)
-__global__ __launch_bounds__(kT) void df_grad_kernel(const double *__restrict__ dens,
-                                                     const double *__restrict__ vx, int nx, int nz,
+// 5. gradients of the normalised density and of vx (deposit.py:212-213);
+// the density and vx outputs take their normalised / thresholded values
+// (deposit.py:202, 204), with max partials of the normalised density
+__global__ __launch_bounds__(kT) void df_grad_kernel(const double *__restrict__ S0,
+                                                     const double *__restrict__ S1, int nx, int nz,
                                                      double x0, double x1, double z0, double z1,
                                                      int ux, int uz,
                                                      const double *__restrict__ scal,
                                                      double *__restrict__ G0,
                                                      double *__restrict__ G1,
-                                                     double *__restrict__ G2) {
+                                                     double *__restrict__ G2,
+                                                     double *__restrict__ dens,
+                                                     double *__restrict__ vx,
+                                                     double *__restrict__ part) {
   const int64_t N = (int64_t)nx * nz;
   const double dsum = scal[S_DSUM], thr = scal[S_THR];
-  const Normed d{dens, dsum};
-  const VxCut v{vx, dens, dsum, thr};
+  const Normed d{S0, dsum};
+  const VxCut v{S1, S0, dsum, thr};
+  double m = -INFINITY;
   for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < N; e += (int64_t)gridDim.x * kT) {
     G0[e] = gradient_at(d, nx, nz, 0, x0, x1, ux, e);
     G1[e] = gradient_at(d, nx, nz, 1, z0, z1, uz, e);
     G2[e] = gradient_at(v, nx, nz, 0, x0, x1, ux, e);
-  }
-}
-
-// 6. axis-0 smoothing of the three derivatives (deposit.py:215-224); the
-// density and vx take their normalised / thresholded values in place.
-__global__ __launch_bounds__(kT) void df_sg0_grad_kernel(SGOp op, const double *__restrict__ G0,
-                                                         const double *__restrict__ G1,
-                                                         const double *__restrict__ G2, int nx,
-                                                         int nz, const double *__restrict__ scal,
-                                                         double *__restrict__ dens,
-                                                         double *__restrict__ vx,
-                                                         double *__restrict__ o0,
-                                                         double *__restrict__ o1,
-                                                         double *__restrict__ o2) {
-#pragma clang fp contract(off)
-  const int64_t N = (int64_t)nx * nz;
-  const double dsum = scal[S_DSUM], thr = scal[S_THR];
-  const Plain a{G0}, b{G1}, c{G2};
-  for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < N; e += (int64_t)gridDim.x * kT) {
-    o0[e] = sg_at(op, a, nx, nz, 0, e);
-    o1[e] = sg_at(op, b, nx, nz, 0, e);
-    o2[e] = sg_at(op, c, nx, nz, 0, e);
-    const double d = dens[e] / dsum;
-    dens[e] = d;
-    if (d <= thr) vx[e] = 0.0;   // raw-count threshold on the normalised density (deposit.py:204)
-  }
-}
-
-// 7. axis-1 smoothing of the derivatives; max partials of the density
-__global__ __launch_bounds__(kT) void df_sg1_grad_kernel(SGOp op, const double *__restrict__ i0,
-                                                         const double *__restrict__ i1,
-                                                         const double *__restrict__ i2, int nx,
-                                                         int nz, const double *__restrict__ dens,
-                                                         double *__restrict__ G0,
-                                                         double *__restrict__ G1,
-                                                         double *__restrict__ G2,
-                                                         double *__restrict__ part) {
-  const int64_t N = (int64_t)nx * nz;
-  const Plain a{i0}, b{i1}, c{i2};
-  double m = -INFINITY;
-  for (int64_t e = (int64_t)blockIdx.x * kT + threadIdx.x; e < N; e += (int64_t)gridDim.x * kT) {
-    G0[e] = sg_at(op, a, nx, nz, 1, e);
-    G1[e] = sg_at(op, b, nx, nz, 1, e);
-    G2[e] = sg_at(op, c, nx, nz, 1, e);
-    m = fmax(m, dens[e]);
+    const double dn = d(e);
+    dens[e] = dn;
+    vx[e] = v(e);
+    m = fmax(m, dn);
   }
   max_partial(m, part);
 }
 
-// 8. vx_x below 8*max/vt takes the mean of vx_x above it (deposit.py:233-235):
+// 6. both smoothing axes of the three derivatives (deposit.py:215-224)
+__global__ __launch_bounds__(kT) void df_sg_grad_kernel(SGOp op, const double *__restrict__ G0,
+                                                        const double *__restrict__ G1,
+                                                        const double *__restrict__ G2, int nx,
+                                                        int nz, double *__restrict__ o0,
+                                                        double *__restrict__ o1,
+                                                        double *__restrict__ o2) {
+  const TileIn in[3] = {{G0, nullptr, 0.0}, {G1, nullptr, 0.0}, {G2, nullptr, 0.0}};
+  double *const out[3] = {o0, o1, o2};
+  sg_tile<3>(op, in, out, nx, nz);
+}
+
+// 7. vx_x below 8*max/vt takes the mean of vx_x above it (deposit.py:233-235):
 // fixed-order sums in one workgroup.
 __global__ __launch_bounds__(kBlock) void df_fill_kernel(const double *__restrict__ dens,
                                                          const double *__restrict__ G2, int64_t N,
@@ -371,7 +376,7 @@ __global__ __launch_bounds__(kBlock) void df_fill_kernel(const double *__restric
   }
 }
 
-// 9. outputs
+// 8. outputs
 __global__ __launch_bounds__(kT) void df_out_kernel(const double *__restrict__ dens,
                                                     const double *__restrict__ G0,
                                                     const double *__restrict__ G1,
@@ -430,22 +435,20 @@ extern "C" int dfcsr_df_process(double *density, double *vx, double *density_x,
   double *T1 = T0 + N, *T2 = T1 + N, *col = T2 + N;
   double *partA = col + nz, *partB = partA + kMaxGrid, *scal = partB + kMaxGrid;
   const int g = grid_of(N);
+  const dim3 gt((nz + kTileZ - 1) / kTileZ, (nx + kTileX - 1) / kTileX);
   hipStream_t s = as_stream(stream);
-  // T0/T1: smoothed along x; density/vx: smoothed; T0,T1,T2: gradients;
-  // density_x/z, vx_x: gradients smoothed along x; T0,T1,T2: along z.
+  // T0/T1: density and vx smoothed; density_x/z, vx_x: the gradients; then
+  // density/vx: normalised and thresholded; T0,T1,T2: the gradients smoothed.
   hipLaunchKernelGGL(df_max_kernel, dim3(g), dim3(kT), 0, s, density, N, partA);
-  hipLaunchKernelGGL(df_sg0_kernel, dim3(g), dim3(kT), 0, s, op, density, vx, nx, nz,
+  hipLaunchKernelGGL(df_sg_kernel, gt, dim3(kT), 0, s, op, density, vx, nx, nz,
                      velocity_threshold, partA, g, T0, T1, scal);
-  hipLaunchKernelGGL(df_sg1_kernel, dim3(g), dim3(kT), 0, s, op, T0, T1, nx, nz, density, vx);
-  hipLaunchKernelGGL(df_dsum_kernel, dim3(1), dim3(kBlock), 0, s, density, nx, nz, x_start,
-                     x_end, z_start, z_end, col, scal);
+  hipLaunchKernelGGL(df_dsum_kernel, dim3(1), dim3(kBlock), 0, s, T0, nx, nz, x_start, x_end,
+                     z_start, z_end, col, scal);
   const int ux = axis_uniform(x_start, x_end, nx), uz = axis_uniform(z_start, z_end, nz);
-  hipLaunchKernelGGL(df_grad_kernel, dim3(g), dim3(kT), 0, s, density, vx, nx, nz, x_start, x_end,
-                     z_start, z_end, ux, uz, scal, T0, T1, T2);
-  hipLaunchKernelGGL(df_sg0_grad_kernel, dim3(g), dim3(kT), 0, s, op, T0, T1, T2, nx, nz, scal,
-                     density, vx, density_x, density_z, vx_x);
-  hipLaunchKernelGGL(df_sg1_grad_kernel, dim3(g), dim3(kT), 0, s, op, density_x, density_z, vx_x,
-                     nx, nz, density, T0, T1, T2, partB);
+  hipLaunchKernelGGL(df_grad_kernel, dim3(g), dim3(kT), 0, s, T0, T1, nx, nz, x_start, x_end,
+                     z_start, z_end, ux, uz, scal, density_x, density_z, vx_x, density, vx, partB);
+  hipLaunchKernelGGL(df_sg_grad_kernel, gt, dim3(kT), 0, s, op, density_x, density_z, vx_x, nx,
+                     nz, T0, T1, T2);
   hipLaunchKernelGGL(df_fill_kernel, dim3(1), dim3(kBlock), 0, s, density, T2, N,
                      velocity_threshold, partB, g, scal, info);
   hipLaunchKernelGGL(df_out_kernel, dim3(g), dim3(kT), 0, s, density, T0, T1, T2, N, scal,

@@ -455,8 +455,11 @@ struct PhasePlan {
                             // per wave, observer blocks per z line, workgroups per block
 };
 
-// Workgroups of an observer-major region-0 launch wanted per simulation (at
-// most): enough to fill the chip when the mesh is small.
+// Waves of an observer-major region-0 launch wanted per simulation: about
+// 2048 workgroups of 4 waves, 8 per CU, at most 32 per (z line, observer
+// block).  Measured: the large run (128 x 128, 8 per block) is fastest there
+// and slows with 16 or 32 per block (FP64 12.43 -> 12.51 / 12.65 ms); the
+// chicane (64 x 64) gains from 16 to 32 per block (FP64 6.29 -> 5.97 ms).
 constexpr int kFarWaveTarget = 8192;
 
 __host__ __device__ inline PhasePlan make_plan(const DfcsrWakeParams &P) {
@@ -481,7 +484,7 @@ __host__ __device__ inline PhasePlan make_plan(const DfcsrWakeParams &P) {
     int q = (int)((kFarWaveTarget / 4 + lines - 1) / lines);
     const int qmax = (P.Nz + 4 * pl.S - 1) / (4 * pl.S);   // no more blocks than columns
     q = q < 1 ? 1 : q;
-    q = q > 16 ? 16 : q;
+    q = q > 32 ? 32 : q;
     q = q > qmax ? (qmax < 1 ? 1 : qmax) : q;
     pl.q = q;
     pl.nA = 4 * pl.S * q;

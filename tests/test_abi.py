@@ -90,7 +90,7 @@ def far_plan(xbins, zbins, Nz):
     """Region 0 of a mesh, observer-major: (partial sums per mesh point,
     workgroups).  W lanes per observer block (64, or xbins rounded up to a
     power of two), S = 64 / W column blocks per wave, q workgroups of 4 waves
-    per (z line, observer block): enough for 8192 waves, at most 16, no more
+    per (z line, observer block): enough for 8192 waves, at most 32, no more
     column blocks than columns."""
     W = 1
     while W < xbins and W < 64:
@@ -99,7 +99,7 @@ def far_plan(xbins, zbins, Nz):
     nob = -(-xbins // W)
     lines = zbins * nob
     q = -(-(8192 // 4) // lines)
-    q = max(1, min(q, 16, max(1, -(-Nz // (4 * S)))))
+    q = max(1, min(q, 32, max(1, -(-Nz // (4 * S)))))
     return 4 * S * q, zbins * nob * q
 
 
@@ -107,7 +107,8 @@ def far_plan(xbins, zbins, Nz):
 def test_wake_workspace_follows_the_phase_plan(slope0, near):
     """Every later region has Nx rows in 4 chunks; 3 regions without chirp
     (|slope| <= 1), 4 with.  Region 0 of a mesh: one partial sum per column
-    block (64 for a 64 x 64 mesh); of explicit observers: 2 Nx rows in
+    block (128 for a 64 x 64 mesh: 32 workgroups of 4 per z line); of
+    explicit observers: 2 Nx rows in
     ceil(400/64) = 7 chunks.  Per mesh point: 2 partial sums per phase or
     column block + 48 doubles of observer data."""
     import ctypes
@@ -115,7 +116,7 @@ def test_wake_workspace_follows_the_phase_plan(slope0, near):
     P = _params(slope0)
     M = 64 * 64
     far, _ = far_plan(64, 64, 200)
-    assert far == 64
+    assert far == 128
     assert lib.dfcsr_wake_workspace_bytes(ctypes.byref(P)) == M * (2 * (far + near) + 48) * 8 + 256
     P.observers = 8
     assert lib.dfcsr_wake_workspace_bytes(ctypes.byref(P)) == M * (2 * (7 + near) + 48) * 8 + 256

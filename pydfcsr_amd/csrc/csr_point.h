@@ -100,13 +100,6 @@ __device__ inline void load_cells5(const char *plane, const uint32_t off[4], dou
   }
 }
 
-__device__ inline void copy_cells5(const double v[4][5], double w[4][5]) {
-#pragma unroll
-  for (int c = 0; c < 4; ++c)
-#pragma unroll
-    for (int f = 0; f < 5; ++f) w[c][f] = v[c][f];
-}
-
 // interpolate3D's lerp a (1 - w) + b w (interp3D.py:53-61).  kFused (the
 // wake kernels): b w added to the rounded a (1 - w) by one fused
 // multiply-add; else the reference's two rounded products and a sum.
@@ -170,35 +163,43 @@ __device__ inline bool hist_lookup5(const DfcsrHistory &H, const HistAxes &A, do
     const char *ring = reinterpret_cast<const char *>(H.cells);
     const uint32_t o0 = r.rb + (uint32_t)Z.i0 * cb, oz = Z.i1 != Z.i0 ? cb : 0u;
     const uint32_t off[4] = {o0, o0 + oz, o0 + r.dxb, o0 + r.dxb + oz};
-    double v0[4][5], v1[4][5];
+    double v0[4][5];
     load_cells5(ring + hist_slot(H, T.i0) * planeB, off, v0);
     // at the last plane interpolate3D clamps i1 to i0: both ends of the time
-    // lerp are the same cell, read once
-    if (T.i1 != T.i0)
+    // lerp are the same cell, read once.  Each branch does its own lerps, so
+    // the second plane's loads need not wait for the first plane's values.
+    if (T.i1 != T.i0) {
+      double v1[4][5];
       load_cells5(ring + hist_slot(H, T.i1) * planeB, off, v1);
-    else
-      copy_cells5(v0, v1);
 #pragma unroll
-    for (int f = 0; f < 5; ++f)
-      out[f] = trilerp<kRecip>(v0[0][f], v0[1][f], v0[2][f], v0[3][f], v1[0][f], v1[1][f],
-                               v1[2][f], v1[3][f], td, xd, zd);
+      for (int f = 0; f < 5; ++f)
+        out[f] = trilerp<kRecip>(v0[0][f], v0[1][f], v0[2][f], v0[3][f], v1[0][f], v1[1][f],
+                                 v1[2][f], v1[3][f], td, xd, zd);
+    } else {
+#pragma unroll
+      for (int f = 0; f < 5; ++f)
+        out[f] = trilerp<kRecip>(v0[0][f], v0[1][f], v0[2][f], v0[3][f], v0[0][f], v0[1][f],
+                                 v0[2][f], v0[3][f], td, xd, zd);
+    }
     return true;
   }
   // corner-packed records: p0 at (t0, x0, z0), p1 at (t1, x0, z0); element
-  // f*4 + {0,1,2,3} = (x0,z0), (x0,z1), (x1,z0), (x1,z1).  At the last plane
-  // (i1 == i0) the record is read once.
-  const double2 *p0 = reinterpret_cast<const double2 *>(hist_rec(H, T.i0, X.i0, Z.i0));
+  // f*4 + {0,1,2,3} = (x0,z0), (x0,z1), (x1,z0), (x1,z1).  A node's records
+  // of consecutive slots are consecutive.  At the last plane (i1 == i0) the
+  // second read is the first record again, a cache hit: selecting its
+  // address, not copying its values, lets both reads be issued before
+  // either is waited for.
+  const int s0 = hist_slot(H, T.i0);
+  const double *node = H.data + ((int64_t)X.i0 * H.Zi + Z.i0) * H.cap * DFCSR_HIST_REC64;
+  const double *r0 = node + (int64_t)s0 * DFCSR_HIST_REC64;
+  const double *r1 = T.i1 == T.i0 ? r0 : (s0 + 1 == H.cap ? node : r0 + DFCSR_HIST_REC64);
+  const double2 *p0 = reinterpret_cast<const double2 *>(r0);
+  const double2 *p1 = reinterpret_cast<const double2 *>(r1);
   double2 a[DFCSR_HIST_REC64 / 2], b[DFCSR_HIST_REC64 / 2];
 #pragma unroll
   for (int k = 0; k < DFCSR_HIST_REC64 / 2; ++k) a[k] = p0[k];
-  if (T.i1 != T.i0) {
-    const double2 *p1 = reinterpret_cast<const double2 *>(hist_rec(H, T.i1, X.i0, Z.i0));
 #pragma unroll
-    for (int k = 0; k < DFCSR_HIST_REC64 / 2; ++k) b[k] = p1[k];
-  } else {
-#pragma unroll
-    for (int k = 0; k < DFCSR_HIST_REC64 / 2; ++k) b[k] = a[k];
-  }
+  for (int k = 0; k < DFCSR_HIST_REC64 / 2; ++k) b[k] = p1[k];
 #pragma unroll
   for (int f = 0; f < 5; ++f)
     out[f] = trilerp<kRecip>(a[2 * f].x, a[2 * f].y, a[2 * f + 1].x, a[2 * f + 1].y, b[2 * f].x,

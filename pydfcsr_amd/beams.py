@@ -203,7 +203,9 @@ class Beam:
         kernels.apply_wakes(self.coords[1], self.coords[5], self.coords[0], self.coords[4], dE, xk,
                             xr[0], xr[1], zr[0], zr[1], m["slope0"], m["slope1"], step_size,
                             self.init_energy, transverse_on, xr[2], zr[2])
-        self.update_status()
+        # the reference recomputes the status here (beams.py:131); the kick
+        # changes px and pz only, and the status is moments of x and z, so
+        # the current ones stand
 
     # -- properties --------------------------------------------------------------
     x = property(lambda self: self.coords[0])

@@ -105,11 +105,15 @@ __global__ __launch_bounds__(kThreads) void moments_final(int64_t n, int pass, i
 // ---- per-step beam statistics (CSR.py:837-859, twiss.py:2-70) -------------
 //
 // Means and the sample covariance (np.cov, ddof = 1) of the six coordinates
-// (x, px, y, py, z, pz), two passes like np.cov (mean, then centred products),
-// each a fixed reduction tree; the final single-workgroup pass derives the
+// (x, px, y, py, z, pz) in one pass over the particles: sums of d = v - c
+// and of the products d_p d_q, each a fixed reduction tree, with c the mean
+// of the first (up to) 256 particles -- within a small fraction of a sigma
+// of the bunch's mean, so the shifted sums lose nothing to cancellation
+// beyond a few ulps.  The final single-workgroup pass forms
+// mean = c + S_d / n and cov = (S_pq - S_p S_q / n) / (n - 1), then the
 // Twiss and dispersion parameters of (x, px, pz) and (y, py, pz) and the
-// energy statistics from them.
-constexpr int kNCov = 27;          // 6 sums, then the 21 centred products (upper triangle)
+// energy statistics.
+constexpr int kNCov = 27;          // 6 shifted sums, then the 21 products (upper triangle)
 constexpr int kStatOff = 32;       // block partials start here (doubles)
 
 __device__ inline int tri_index(int i, int j) {   // i <= j < 6, row-major upper triangle
@@ -119,34 +123,32 @@ __device__ inline int tri_index(int i, int j) {   // i <= j < 6, row-major upper
 __global__ __launch_bounds__(kThreads) void stats_partial(
     const double *__restrict__ x, const double *__restrict__ px, const double *__restrict__ y,
     const double *__restrict__ py, const double *__restrict__ z, const double *__restrict__ pz,
-    int64_t n, int pass, double *__restrict__ st) {
+    int64_t n, double *__restrict__ st) {
   __shared__ double red[kThreads / 64];
   const double *c[6] = {x, px, y, py, z, pz};
-  double a[21];
+  double a[kNCov];
 #pragma unroll
-  for (int k = 0; k < 21; ++k) a[k] = 0.0;
+  for (int k = 0; k < kNCov; ++k) a[k] = 0.0;
+  // the shift: every workgroup forms the same sum in the same order
   double m[6];
-#pragma unroll
-  for (int k = 0; k < 6; ++k) m[k] = st[k];
+  const double m0 = (double)(n < kThreads ? n : kThreads);
+  for (int k = 0; k < 6; ++k)
+    m[k] = block_sum((int64_t)threadIdx.x < n ? c[k][threadIdx.x] : 0.0, red) / m0;
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    for (int k = 0; k < 6; ++k) st[k] = m[k];
   const int64_t stride = (int64_t)gridDim.x * kThreads;
   for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) {
     double v[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) v[k] = c[k][i];
-    if (pass == 0) {
+    for (int k = 0; k < 6; ++k) v[k] = c[k][i] - m[k];
 #pragma unroll
-      for (int k = 0; k < 6; ++k) a[k] += v[k];
-    } else {
+    for (int k = 0; k < 6; ++k) a[k] += v[k];
 #pragma unroll
-      for (int k = 0; k < 6; ++k) v[k] = v[k] - m[k];
+    for (int p = 0; p < 6; ++p)
 #pragma unroll
-      for (int p = 0; p < 6; ++p)
-#pragma unroll
-        for (int q = p; q < 6; ++q) a[tri_index(p, q)] += v[p] * v[q];
-    }
+      for (int q = p; q < 6; ++q) a[6 + tri_index(p, q)] += v[p] * v[q];
   }
-  const int nv = pass == 0 ? 6 : 21;
-  for (int k = 0; k < nv; ++k) {
+  for (int k = 0; k < kNCov; ++k) {
     const double v = block_sum(a[k], red);
     if (threadIdx.x == 0) st[kStatOff + blockIdx.x * kNCov + k] = v;
   }
@@ -170,27 +172,26 @@ __device__ inline void twiss_plane(double s00, double s01, double s02, double s1
   out[6] = emit * p0c / mc2;
 }
 
-__global__ __launch_bounds__(kThreads) void stats_final(int64_t n, int pass, int nblk,
-                                                        double p0c, double mc2,
-                                                        double *__restrict__ st,
+__global__ __launch_bounds__(kThreads) void stats_final(int64_t n, int nblk, double p0c,
+                                                        double mc2, const double *__restrict__ st,
                                                         double *__restrict__ out) {
   __shared__ double red[kThreads / 64];
-  const int nv = pass == 0 ? 6 : 21;
-  double tot[21];
-  for (int k = 0; k < nv; ++k) {
+  double tot[kNCov];
+  for (int k = 0; k < kNCov; ++k) {
     double v = 0.0;
     for (int b = threadIdx.x; b < nblk; b += kThreads) v += st[kStatOff + b * kNCov + k];
     tot[k] = block_sum(v, red);
   }
   if (threadIdx.x != 0) return;
   const double dn = (double)n;
-  if (pass == 0) {
-    for (int k = 0; k < 6; ++k) st[k] = tot[k] / dn;
-    return;
-  }
+  const double c[6] = {st[0], st[1], st[2], st[3], st[4], st[5]};
   double *cov = out + DFCSR_STAT_COV;
-  for (int k = 0; k < 6; ++k) out[DFCSR_STAT_MEAN + k] = st[k];
-  for (int k = 0; k < 21; ++k) cov[k] = tot[k] / (dn - 1.0);
+  for (int k = 0; k < 6; ++k) out[DFCSR_STAT_MEAN + k] = c[k] + tot[k] / dn;
+  for (int p = 0; p < 6; ++p)
+    for (int q = p; q < 6; ++q) {
+      const int t = tri_index(p, q);
+      cov[t] = (tot[6 + t] - tot[p] * tot[q] / dn) / (dn - 1.0);
+    }
   // planes (x, px, pz) and (y, py, pz): coordinates 0, 1, 5 and 2, 3, 5
   twiss_plane(cov[tri_index(0, 0)], cov[tri_index(0, 1)], cov[tri_index(0, 5)],
               cov[tri_index(1, 1)], cov[tri_index(1, 5)], cov[tri_index(5, 5)], p0c, mc2,
@@ -199,8 +200,8 @@ __global__ __launch_bounds__(kThreads) void stats_final(int64_t n, int pass, int
               cov[tri_index(3, 3)], cov[tri_index(3, 5)], cov[tri_index(5, 5)], p0c, mc2,
               out + DFCSR_STAT_TWISS_Y);
   // energy = (pz + 1) p0c (beams.py:166-181): its mean and population std
-  out[DFCSR_STAT_MEAN_ENERGY] = (st[5] + 1.0) * p0c;
-  out[DFCSR_STAT_SIGMA_ENERGY] = p0c * sqrt(tot[tri_index(5, 5)] / dn);
+  out[DFCSR_STAT_MEAN_ENERGY] = (out[DFCSR_STAT_MEAN + 5] + 1.0) * p0c;
+  out[DFCSR_STAT_SIGMA_ENERGY] = p0c * sqrt(cov[tri_index(5, 5)] * ((dn - 1.0) / dn));
 }
 
 }  // namespace
@@ -228,12 +229,9 @@ extern "C" int dfcsr_beam_statistics(const double *x, const double *px, const do
   hipStream_t s = as_stream(stream);
   double *st = static_cast<double *>(ws);
   const int nb = nblocks_for(n);
-  for (int pass = 0; pass < 2; ++pass) {
-    hipLaunchKernelGGL(stats_partial, dim3(nb), dim3(kThreads), 0, s, x, px, y, py, z, pz, n,
-                       pass, st);
-    hipLaunchKernelGGL(stats_final, dim3(1), dim3(kThreads), 0, s, n, pass, nb, p0c, mc2, st,
-                       out);
-  }
+  hipLaunchKernelGGL(stats_partial, dim3(nb), dim3(kThreads), 0, s, x, px, y, py, z, pz, n, st);
+  hipLaunchKernelGGL(stats_final, dim3(1), dim3(kThreads), 0, s, n, nb, p0c, mc2,
+                     (const double *)st, out);
   DFCSR_LAUNCH_CHECK("beam_statistics");
   return DFCSR_OK;
 }

@@ -183,6 +183,33 @@ def test_get_df_golden(golden_dir, tag, bins):
         assert rel_max(f[k].cpu(), d[k]) <= 1e-9, k
 
 
+@pytest.mark.parametrize("bins,window,order", [((7, 9), 7, 2), ((33, 65), 9, 3),
+                                                ((65, 33), 5, 2), ((17, 40), 13, 4)])
+def test_get_df_ragged_grids_vs_oracle(bins, window, order):
+    """The smoothing runs as tiles of 8 x 32 outputs; grids that end inside a
+    tile, are narrower than a tile, or whose edge windows (the first / last
+    `window` samples) reach past a partial tile's halo give the oracle's
+    get_DF (deposit.py:145-245) on a correlated beam (configured grid)."""
+    from pydfcsr_amd.deposit import DF_tracker
+    rng = np.random.default_rng(sum(bins) + window)
+    n = 20000
+    z = rng.normal(0.0, 60e-6, n)
+    x = rng.normal(0.0, 20e-6, n) + 4.0 * z
+    px = rng.normal(0.0, 1e-4, n) + 0.3 * z / 60e-6 * 1e-4
+    cfg = dict(xbins=bins[0], zbins=bins[1], filter_order=order, filter_window=window,
+               velocity_threhold=1000.0)
+    ref = O.get_DF(x, z, px, 0.1, **cfg)
+    assert ref["nbins"] == bins
+    tr = DF_tracker(cfg, device=DEV)
+    m = tr.get_DF(_t(x), _t(z), _t(px), 0.1)
+    assert (m.nbx, m.nbz) == bins
+    f = tr.current_fields()
+    for k in ("density", "vx", "density_x", "density_z", "vx_x"):
+        a, b = f[k].cpu().numpy(), ref[k]
+        assert np.array_equal(np.isnan(a), np.isnan(b)), k
+        assert rel_max(np.nan_to_num(a), np.nan_to_num(b)) <= 1e-9, k
+
+
 # -- a8-a10 --------------------------------------------------------------------------
 def test_history_golden(golden_dir):
     from pydfcsr_amd.deposit import DF_tracker

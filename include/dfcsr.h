@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define DFCSR_ABI_VERSION 5
+#define DFCSR_ABI_VERSION 6
 
 enum {
   DFCSR_OK = 0,
@@ -210,7 +210,14 @@ int dfcsr_regrid(const double *src, int64_t src_slot_stride, const DfcsrRegridJo
  * a wake read cells (FP64 wakes: cells, mixed: cells32), whose neighbouring
  * z lie in neighbouring cells; results are identical with or without them.
  * One plane must span at most 4 GiB (the wakes address cells within a plane
- * with 32-bit offsets). */
+ * with 32-bit offsets).
+ * dcells32 (optional, with cells32) holds the float32 time differences of
+ * cells32 in the same layout: cell (slot of slice i, x, z) = cells32 of
+ * slice i + 1 minus that of slice i, rounded to float32, and 0 for the
+ * newest slice (dfcsr_cells32_tdiff fills it).  A mixed-precision wake reads
+ * the cell-major rings only when both are present: the time lerp
+ * v0 + td (v1 - v0) then takes v1 - v0 from dcells32, the same float32
+ * value, so the results are those of the float32 rings. */
 typedef struct {
   const double *data;    /* [Xi][Zi][cap][DFCSR_HIST_REC64] or NULL */
   const float *data32;   /* [Xi][cap][DFCSR_HIST_CHUNKS32][Zi][4] or NULL */
@@ -219,7 +226,18 @@ typedef struct {
   const float *cells32;  /* [cap][Xi][Zi][DFCSR_CELL] or NULL */
   int32_t cap, head, nt, Xi, Zi;
   double min_t, min_x, min_z, dt, dx, dz;
+  const float *dcells32; /* [cap][Xi][Zi][DFCSR_CELL] or NULL */
 } DfcsrHistory;
+
+/* Fills dcells32 (layout of DfcsrHistory) for the logical slices
+ * first .. nt-1 of a ring of cap slots whose slice i lives in slot
+ * (head + i) % cap: the cells of slice i + 1 minus those of slice i in
+ * float32, and 0 for slice nt - 1.  Called after a re-grid: of the new
+ * slice alone (append_interpolant's common case) with first = nt - 2, of
+ * every slice with first = 0.  Has no counterpart in the reference: it keeps
+ * the derived ring of the mixed-precision wake current. */
+int dfcsr_cells32_tdiff(const float *cells32, float *dcells32, int Xi, int Zi, int cap, int head,
+                        int nt, int first, void *stream);
 #define DFCSR_CELL 5
 
 /* Lattice tables (lattice.py:4-110, 136-150): table[k][0..5] =

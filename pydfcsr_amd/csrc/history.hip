@@ -120,10 +120,69 @@ __global__ __launch_bounds__(kThreads) void regrid_kernel(
   }
 }
 
+// dcells32 plane of logical slice i (first <= i < nt): cells32 of slice
+// i + 1 minus that of slice i, 0 for the newest.  n: floats per plane, read
+// as float4 when kVec (n % 4 == 0, so every plane starts 16-byte aligned).
+template <bool kVec>
+__global__ __launch_bounds__(kThreads) void tdiff_kernel(const float *__restrict__ c,
+                                                         float *__restrict__ d, int64_t n,
+                                                         int cap, int head, int nt, int first) {
+  for (int i = first + (int)blockIdx.y; i < nt; i += (int)gridDim.y) {
+    const int s0 = (head + i) % cap, s1 = (head + i + 1) % cap;
+    const bool last = i == nt - 1;
+    if (kVec) {
+      const float4 *a = reinterpret_cast<const float4 *>(c + (int64_t)s0 * n);
+      const float4 *b = reinterpret_cast<const float4 *>(c + (int64_t)s1 * n);
+      float4 *o = reinterpret_cast<float4 *>(d + (int64_t)s0 * n);
+      for (int64_t e = (int64_t)blockIdx.x * kThreads + threadIdx.x; e < n / 4;
+           e += (int64_t)gridDim.x * kThreads) {
+        if (last) {
+          o[e] = make_float4(0.f, 0.f, 0.f, 0.f);
+        } else {
+          const float4 u = a[e], v = b[e];
+          o[e] = make_float4(v.x - u.x, v.y - u.y, v.z - u.z, v.w - u.w);
+        }
+      }
+    } else {
+      const float *a = c + (int64_t)s0 * n, *b = c + (int64_t)s1 * n;
+      float *o = d + (int64_t)s0 * n;
+      for (int64_t e = (int64_t)blockIdx.x * kThreads + threadIdx.x; e < n;
+           e += (int64_t)gridDim.x * kThreads)
+        o[e] = last ? 0.f : b[e] - a[e];
+    }
+  }
+}
+
 }  // namespace
 }  // namespace dfcsr
 
 using namespace dfcsr;
+
+extern "C" int dfcsr_cells32_tdiff(const float *cells32, float *dcells32, int Xi, int Zi, int cap,
+                                   int head, int nt, int first, void *stream) {
+  if (!cells32 || !dcells32) return set_error(DFCSR_EINVAL, "cells32_tdiff: NULL ring");
+  if (Xi < 1 || Zi < 1) return set_error(DFCSR_EINVAL, "cells32_tdiff: grid %d x %d", Xi, Zi);
+  if (cap < 1 || nt < 0 || nt > cap || head < 0 || head >= cap)
+    return set_error(DFCSR_EINVAL, "cells32_tdiff: cap %d head %d nt %d", cap, head, nt);
+  if (first < 0) first = 0;
+  if (first >= nt) return DFCSR_OK;
+  const int64_t n = (int64_t)Xi * Zi * DFCSR_CELL;
+  const bool vec = n % 4 == 0;
+  const int64_t per = vec ? n / 4 : n;
+  int gx = (int)((per + kThreads - 1) / kThreads);
+  if (gx > 512) gx = 512;
+  int gy = nt - first;
+  if (gy > 1024) gy = 1024;
+  hipStream_t s = as_stream(stream);
+  if (vec)
+    hipLaunchKernelGGL(tdiff_kernel<true>, dim3(gx, gy), dim3(kThreads), 0, s, cells32, dcells32,
+                       n, cap, head, nt, first);
+  else
+    hipLaunchKernelGGL(tdiff_kernel<false>, dim3(gx, gy), dim3(kThreads), 0, s, cells32, dcells32,
+                       n, cap, head, nt, first);
+  DFCSR_LAUNCH_CHECK("cells32_tdiff");
+  return DFCSR_OK;
+}
 
 extern "C" int64_t dfcsr_regrid_workspace_bytes(int njobs) {
   if (njobs < 1) njobs = 1;

@@ -295,11 +295,12 @@ __device__ inline double lookup_vx32(const DfcsrHistory &H, const HistAxes &A, d
 }
 
 // The corners (x0,z0), (x0,z1), (x1,z0), (x1,z1) of a float32 cell plane
-// from the cell at p (x0, z0) and the one dxb bytes on (x1, z0): each x
-// row's cells z0 and z1 = z0 + 1 read as 40 contiguous bytes.
-__device__ inline void load_pairs32(const char *p, uint32_t dxb, float v[4][5]) {
-  const float *a = reinterpret_cast<const float *>(p);
-  const float *b = reinterpret_cast<const float *>(p + dxb);
+// from the cell at byte oa of base (x0, z0) and the one at byte ob (x1,
+// z0): each x row's cells z0 and z1 = z0 + 1 read as 40 contiguous bytes.
+// 32-bit offsets from one base let the loads add them to its scalar address.
+__device__ inline void load_pairs32(const char *base, uint32_t oa, uint32_t ob, float v[4][5]) {
+  const float *a = reinterpret_cast<const float *>(base + (uint64_t)oa);
+  const float *b = reinterpret_cast<const float *>(base + (uint64_t)ob);
 #pragma unroll
   for (int k = 0; k < 5; ++k) {
     v[0][k] = a[k];
@@ -310,14 +311,15 @@ __device__ inline void load_pairs32(const char *p, uint32_t dxb, float v[4][5]) 
 }
 
 // Ways a lookup finds the history: the corner-packed rings, the cell-major
-// ring with a 64-bit plane address per lookup, or (mixed precision, a
-// float32 cell-major ring of at most 4 GiB) 32-bit byte offsets from the
-// ring's start, which the loads add to the ring's scalar address.
+// rings with a 64-bit plane address per lookup, or (mixed precision,
+// float32 cell-major rings of at most 4 GiB) 32-bit byte offsets from the
+// rings' starts, which the loads add to the rings' scalar addresses.
 enum CellMode { kRings = 0, kCellsPlane = 1, kCellsRing32 = 2 };
 
 // One source point in mixed precision; false when outside the history.
 // kTI: read the time-innermost FP32 ring, else the chunked one.  kCells
-// (a CellMode): read the float32 cell-major ring H.cells32 instead.
+// (a CellMode): read the float32 cell-major ring H.cells32 and its time
+// differences H.dcells32 instead.
 template <bool kTI, int kCells = kRings>
 __device__ inline bool csr_point_mixed(const DfcsrHistory &H, const HistAxes &A, double ut0,
                                        const ColumnMixed &c, double xp, float xpf, int xi0,
@@ -341,14 +343,18 @@ __device__ inline bool csr_point_mixed(const DfcsrHistory &H, const HistAxes &A,
   float f[5];
   if (kCells != kRings) {
     // float32 cell (slot, x, z) at byte ((slot Xi + x) Zi + z) * 20 of the
-    // cell-major ring; the plane of the next slot follows (slot 0 after the
-    // ring's last).  Each x row's two z cells are one 40-byte pair; at the
-    // last z (interpolate3D clamps z1 to z0) the pair is the one before,
-    // (z - 1, z), with weight 1 on its second cell, so the lerp gives that
-    // cell's value within one FP32 rounding and the loads need no branch.
+    // cell-major rings.  The time lerp v0 + td (v1 - v0) takes v1 - v0 from
+    // H.dcells32 at the same offset: the float32 difference of the next
+    // slice's cell and this one's, 0 at the newest slice, where
+    // interpolate3D clamps i1 to i0 -- so one plane of each ring serves every
+    // lookup, with no branch.  Each x row's two z cells are one 40-byte pair;
+    // at the last z (interpolate3D clamps z1 to z0) the pair is the one
+    // before, (z - 1, z), with weight 1 on its second cell, so the lerp gives
+    // that cell's value within one FP32 rounding and the loads need no branch.
     constexpr uint32_t cb = DFCSR_CELL * sizeof(float);
     const uint32_t planeB = (uint32_t)H.Xi * (uint32_t)H.Zi * cb;   // < 4 GiB (checked)
     const char *ring = reinterpret_cast<const char *>(H.cells32);
+    const char *dring = reinterpret_cast<const char *>(H.dcells32);
     int z0 = Z.i0;
     if (Z.i1 == Z.i0) {   // Zi >= 2
       z0 -= 1;
@@ -356,41 +362,25 @@ __device__ inline bool csr_point_mixed(const DfcsrHistory &H, const HistAxes &A,
     }
     const uint32_t o = rr.rb + (uint32_t)z0 * cb;
     const int s0 = hist_slot(H, T.i0);
-    const bool wrap = s0 + 1 == H.cap;
-    float v0[4][5];
-    const char *p1;
+    float v[4][5], d[4][5];
     if (kCells == kCellsRing32) {
-      const uint32_t q0 = (uint32_t)s0 * planeB + o;
-      load_pairs32(ring + (uint64_t)q0, rr.dxb, v0);
-      p1 = ring + (uint64_t)(wrap ? o : q0 + planeB);
+      const uint32_t q = (uint32_t)s0 * planeB + o;
+      load_pairs32(ring, q, q + rr.dxb, v);
+      load_pairs32(dring, q, q + rr.dxb, d);
     } else {
-      const char *pl0 = ring + (uint64_t)(uint32_t)s0 * planeB;
-      load_pairs32(pl0 + o, rr.dxb, v0);
-      p1 = (wrap ? ring : pl0 + planeB) + o;
+      const uint64_t pl = (uint64_t)(uint32_t)s0 * planeB;
+      load_pairs32(ring + pl, o, o + rr.dxb, v);
+      load_pairs32(dring + pl, o, o + rr.dxb, d);
     }
-    if (T.i1 != T.i0) {
-      float v1[4][5];
-      load_pairs32(p1, rr.dxb, v1);
 #pragma unroll
-      for (int k = 0; k < 5; ++k) {
-        const float c00 = fmaf(td, v1[0][k] - v0[0][k], v0[0][k]);
-        const float c01 = fmaf(td, v1[1][k] - v0[1][k], v0[1][k]);
-        const float c10 = fmaf(td, v1[2][k] - v0[2][k], v0[2][k]);
-        const float c11 = fmaf(td, v1[3][k] - v0[3][k], v0[3][k]);
-        const float c0 = fmaf(xd, c10 - c00, c00);
-        const float c1 = fmaf(xd, c11 - c01, c01);
-        f[k] = fmaf(zd, c1 - c0, c0);
-      }
-    } else {
-      // the last plane (interpolate3D clamps i1 to i0): the time lerp of a
-      // value with itself, fmaf(td, 0, v), is v exactly, so the cells are read
-      // once and only the (x, z) lerps remain
-#pragma unroll
-      for (int k = 0; k < 5; ++k) {
-        const float c0 = fmaf(xd, v0[2][k] - v0[0][k], v0[0][k]);
-        const float c1 = fmaf(xd, v0[3][k] - v0[1][k], v0[1][k]);
-        f[k] = fmaf(zd, c1 - c0, c0);
-      }
+    for (int k = 0; k < 5; ++k) {
+      const float c00 = fmaf(td, d[0][k], v[0][k]);
+      const float c01 = fmaf(td, d[1][k], v[1][k]);
+      const float c10 = fmaf(td, d[2][k], v[2][k]);
+      const float c11 = fmaf(td, d[3][k], v[3][k]);
+      const float c0 = fmaf(xd, c10 - c00, c00);
+      const float c1 = fmaf(xd, c11 - c01, c01);
+      f[k] = fmaf(zd, c1 - c0, c0);
     }
   } else {
     // one corner-packed record per time plane, one 16-byte chunk per field
@@ -591,8 +581,8 @@ __device__ inline void observer_point(const WakeTask &T, int i, double *__restri
 // Integrates phase ph of the mesh point whose observer data is od; every
 // thread returns the block sums of wz_k wx_j (I_z, I_x) over the phase.
 // kTI (mixed only): read the time-innermost FP32 ring.  kCells (a
-// CellMode): read the cell-major ring (H.cells for FP64, H.cells32 for
-// mixed).
+// CellMode): read the cell-major ring (H.cells for FP64; H.cells32 and its
+// time differences H.dcells32 for mixed).
 template <bool kMixed, bool kTI, int kCells = kRings>
 __device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int ph, double *red,
                            RowTable &rt, double &out_z, double &out_x) {
@@ -906,7 +896,13 @@ __host__ __device__ inline bool mixed_reads_ti(const DfcsrHistory &H, bool near)
   return near ? (H.data32 == nullptr) : (H.data32t != nullptr);
 }
 
-// Whether every byte of the float32 cell-major ring has a 32-bit offset.
+// Whether a mixed wake's near regions read the float32 cell-major ring and
+// its time differences (both must be present).
+__host__ __device__ inline bool mixed_cells(const DfcsrHistory &H) {
+  return H.cells32 != nullptr && H.dcells32 != nullptr;
+}
+
+// Whether every byte of the float32 cell-major rings has a 32-bit offset.
 __host__ __device__ inline bool cells32_ring32(const DfcsrHistory &H) {
   return (int64_t)H.cap * H.Xi * H.Zi * (int64_t)(DFCSR_CELL * sizeof(float)) <= 0xffffffffLL;
 }
@@ -967,9 +963,9 @@ __global__ __launch_bounds__(kThreads, kMixed && kNear ? 4 : DFCSR_WAKE_MINBLOCK
     return;
   }
   if constexpr (kMixed) {   // per task: its FP32 rings, cell-major or not
-    if (kNear && T.H.cells32 && cells32_ring32(T.H))
+    if (kNear && mixed_cells(T.H) && cells32_ring32(T.H))
       run_phase_block<true, kNear, false, kCellsRing32>(T, l, partials, red);
-    else if (kNear && T.H.cells32)
+    else if (kNear && mixed_cells(T.H))
       run_phase_block<true, kNear, false, kCellsPlane>(T, l, partials, red);
     else if (mixed_reads_ti(T.H, kNear))
       run_phase_block<true, kNear, true>(T, l, partials, red);
@@ -1050,7 +1046,7 @@ int check_rings(const DfcsrHistory &H, int precision, const char *who) {
     return set_error(DFCSR_EINVAL, "%s: FP64 precision needs the FP64 history ring", who);
   // the wake addresses a cell within a plane of a cell-major ring with 32-bit
   // byte offsets
-  const bool cells = precision == DFCSR_PREC_MIXED ? H.cells32 != nullptr : H.cells != nullptr;
+  const bool cells = precision == DFCSR_PREC_MIXED ? mixed_cells(H) : H.cells != nullptr;
   const int64_t cellB = DFCSR_CELL * (precision == DFCSR_PREC_MIXED ? 4 : 8);
   if (cells && (int64_t)H.Xi * H.Zi * cellB > ((int64_t)1 << 32))
     return set_error(DFCSR_EINVAL, "%s: a cell-major plane of %d x %d cells exceeds 4 GiB", who,
@@ -1147,10 +1143,10 @@ extern "C" int dfcsr_wake_mesh(const DfcsrHistory *hist, const DfcsrLattice *lat
     else
       hipLaunchKernelGGL((wake_phase_kernel<true, false, false>), ga, dim3(kThreads), 0, s, T, part);
     const bool ti = mixed_reads_ti(*hist, true);
-    if (hist->cells32 && cells32_ring32(*hist)) {
+    if (mixed_cells(*hist) && cells32_ring32(*hist)) {
       hipLaunchKernelGGL((wake_phase_kernel<true, true, false, kCellsRing32>), gb, dim3(kThreads),
                          0, s, T, part);
-    } else if (hist->cells32) {
+    } else if (mixed_cells(*hist)) {
       hipLaunchKernelGGL((wake_phase_kernel<true, true, false, kCellsPlane>), gb, dim3(kThreads),
                          0, s, T, part);
     } else if (ti) {

@@ -14,6 +14,8 @@ whole run:
   hist32 [Xi][cap][5][Zi][4]   float32  the same, chunked (optional)
   cells  [cap][Xi][Zi][5]      float64  cell-major: the five fields at (x, z)
   cells32[cap][Xi][Zi][5]      float32  the same in float32
+  dcells32[cap][Xi][Zi][5]     float32  cells32 of the next slice minus this one's (0
+                                        for the newest): the mixed wake's time lerp
 (record (x, z) holds every field at (x, z), (x, z+1), (x+1, z), (x+1, z+1),
 clamped to the last plane: one record per time plane per lookup.  hist and
 hist32t keep the slot innermost (the two planes of a lookup are neighbours,
@@ -22,7 +24,7 @@ the cell-major rings keep a plane's cells together, neighbouring z in
 neighbouring cells: what the near regions' lookups want.  hist32 stores each field's four corners as one
 16-byte chunk, z fastest; it serves the near regions when the cell-major
 rings are not kept.  keep_fp64 chooses hist (and cells), keep_fp32 hist32t
-(and cells32), keep_cells the cell-major rings, keep_chunked hist32.)
+(and cells32, dcells32), keep_cells the cell-major rings, keep_chunked hist32.)
 Appending writes slot (head + nt) % cap; popping advances head.  Nothing is
 copied per step: the reference's np.array(deque) stacking of the whole
 history (deposit.py:422-426) becomes a descriptor update.
@@ -94,6 +96,7 @@ class DF_tracker:
         self.hist32t = None   # float32 time-innermost (Xi, Zi, cap, 20)
         self.cells = None     # float64 cell-major (cap, Xi, Zi, 5)
         self.cells32 = None   # float32 cell-major (cap, Xi, Zi, 5)
+        self.dcells32 = None  # its time differences (cap, Xi, Zi, 5)
         self.cur = None
         self.n_regrid = 0
         # largest interp grid side the rings may take (None: unbounded, as the
@@ -118,10 +121,10 @@ class DF_tracker:
             return
         new_cap = max(need, 2 * self.cap, 16)
         raw = torch.empty((new_cap, self._raw_stride()), dtype=torch.float64, device=self.device)
-        rings = (None,) * 5
+        rings = (None,) * 6
         if self.Xi:
             rings = self._rings(self.Xi, self.Zi, new_cap)
-        hist, hist32, hist32t, cells, cells32 = rings
+        hist, hist32, hist32t, cells, cells32, dcells32 = rings
         if self.cap:
             # keep the ring's logical order, pending slot included (index nt)
             for i in range(min(self.nt + 1, self.cap)):
@@ -133,19 +136,21 @@ class DF_tracker:
                     hist32t[:, :, i].copy_(self.hist32t[:, :, s])
                 if hist32 is not None and self.hist32 is not None:
                     hist32[:, i].copy_(self.hist32[:, s])
-                for new, old in ((cells, self.cells), (cells32, self.cells32)):
+                for new, old in ((cells, self.cells), (cells32, self.cells32),
+                                 (dcells32, self.dcells32)):
                     if new is not None and old is not None:
                         new[i].copy_(old[s])
         self.raw, self.hist, self.hist32, self.hist32t = raw, hist, hist32, hist32t
-        self.cells, self.cells32 = cells, cells32
+        self.cells, self.cells32, self.dcells32 = cells, cells32, dcells32
         self.cap = new_cap
         self.head = 0
 
     def _rings(self, Xi, Zi, cap):
-        """(hist, hist32, hist32t, cells, cells32) for an Xi x Zi grid.  The
-        wakes address cells within a plane with 32-bit offsets: a cell-major
-        ring whose plane would pass 4 GiB is not kept (the rings serve)."""
-        hist = hist32 = hist32t = cells = cells32 = None
+        """(hist, hist32, hist32t, cells, cells32, dcells32) for an Xi x Zi
+        grid.  The wakes address cells within a plane with 32-bit offsets: a
+        cell-major ring whose plane would pass 4 GiB is not kept (the rings
+        serve)."""
+        hist = hist32 = hist32t = cells = cells32 = dcells32 = None
         dev = self.device
         if self.keep_fp64:
             hist = torch.empty((Xi, Zi, cap, _lib.HIST_REC64), dtype=torch.float64, device=dev)
@@ -155,17 +160,19 @@ class DF_tracker:
             hist32t = torch.empty((Xi, Zi, cap, _lib.HIST_REC32), dtype=torch.float32, device=dev)
             if self.keep_cells and Xi * Zi * _lib.CELL * 4 <= (1 << 32):
                 cells32 = torch.empty((cap, Xi, Zi, _lib.CELL), dtype=torch.float32, device=dev)
+                dcells32 = torch.empty_like(cells32)
             if self.keep_chunked:
                 hist32 = torch.empty((Xi, cap, _lib.HIST_CHUNKS32, Zi, 4), dtype=torch.float32,
                                      device=dev)
-        return hist, hist32, hist32t, cells, cells32
+        return hist, hist32, hist32t, cells, cells32, dcells32
 
     def _alloc_hist(self, Xi, Zi):
         if self.Xi != Xi or self.Zi != Zi or (self.hist is None and self.hist32t is None):
             # free first
-            self.hist = self.hist32 = self.hist32t = self.cells = self.cells32 = None
-            (self.hist, self.hist32, self.hist32t, self.cells,
-             self.cells32) = self._rings(Xi, Zi, self.cap)
+            self.hist = self.hist32 = self.hist32t = None
+            self.cells = self.cells32 = self.dcells32 = None
+            (self.hist, self.hist32, self.hist32t, self.cells, self.cells32,
+             self.dcells32) = self._rings(Xi, Zi, self.cap)
         self.Xi, self.Zi = Xi, Zi
 
     def raw_fields(self, i=None):
@@ -269,6 +276,10 @@ class DF_tracker:
                            self.Xi, self.Zi, self.xi[0], self.xi[1], self.zi[0], self.zi[1],
                            self.ws, hist32t=self.hist32t, cells=self.cells,
                            cells32=self.cells32)
+        if self.dcells32 is not None and idx:
+            # the re-gridded slices and each one's predecessor change differences
+            kernels.cells32_tdiff(self.cells32, self.dcells32, self.head, self.nt,
+                                  first=max(0, min(idx) - 1))
 
     def build_interpolant(self):   # deposit.py:395-426
         nt = self.nt
@@ -287,7 +298,8 @@ class DF_tracker:
         return _lib.History(p(self.hist), p(self.hist32), p(self.hist32t), p(self.cells),
                             p(self.cells32), self.cap, self.head, self.nt, self.Xi, self.Zi,
                             float(self.min_x), float(self.min_y), float(self.min_z),
-                            float(self.delta_x), float(self.delta_y), float(self.delta_z))
+                            float(self.delta_x), float(self.delta_y), float(self.delta_z),
+                            p(self.dcells32))
 
     def history_fields(self):
         """Stacked (nt, Xi, Zi) copies of the five fields in logical order
@@ -301,7 +313,7 @@ class DF_tracker:
         b = self.raw.numel() * 8 if self.raw is not None else 0
         if self.hist is not None:
             b += self.hist.numel() * 8
-        for r in (self.hist32, self.hist32t, self.cells, self.cells32):
+        for r in (self.hist32, self.hist32t, self.cells, self.cells32, self.dcells32):
             if r is not None:
                 b += r.numel() * r.element_size()
         return b

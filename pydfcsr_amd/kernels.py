@@ -253,6 +253,25 @@ def regrid(src: torch.Tensor, slot_stride: int, jobs, hist64, hist32, Xi, Zi, xi
                            stream_handle(src.device)), "regrid")
 
 
+def cells32_tdiff(cells32: torch.Tensor, dcells32: torch.Tensor, head: int, nt: int,
+                  first: int = 0):
+    """Fills dcells32 (cap, Xi, Zi, 5) float32 with the time differences of the
+    float32 cell-major ring cells32 for logical slices first .. nt-1 (slice i in
+    slot (head + i) % cap): slice i + 1 minus slice i, 0 for the newest."""
+    lib = _lib.load()
+    shape = tuple(cells32.shape)
+    if (len(shape) != 4 or shape[3] != _lib.CELL or cells32.dtype != torch.float32
+            or tuple(dcells32.shape) != shape or dcells32.dtype != torch.float32
+            or not cells32.is_contiguous() or not dcells32.is_contiguous()):
+        raise DfcsrError(f"cells32_tdiff: rings {shape} {cells32.dtype} / "
+                         f"{tuple(dcells32.shape)} {dcells32.dtype} are not two contiguous "
+                         f"float32 (cap, Xi, Zi, {_lib.CELL}) rings")
+    cap, Xi, Zi, _ = shape
+    check(lib.dfcsr_cells32_tdiff(ptr(cells32), ptr(dcells32), int(Xi), int(Zi), int(cap),
+                                  int(head), int(nt), int(first), stream_handle(cells32.device)),
+          "cells32_tdiff")
+
+
 def interpolate3D(hist: _lib.History, field: int, q: torch.Tensor, device):
     lib = _lib.load()
     out = torch.empty(q.shape[0], dtype=torch.float64, device=device)

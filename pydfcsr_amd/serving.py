@@ -91,8 +91,9 @@ def request_footprint(config: dict, steps: int, precision: str = "mixed") -> int
     grid = grid_bound(config)
     slots = steps + 2   # window and pending slice
     # per slot and grid node: the corner-packed record (160 B in float64, 80 B
-    # in float32) and the cell-major cell (40 B / 20 B)
-    rec = (80 + 4 * _lib.CELL) if precision == "mixed" else (160 + 8 * _lib.CELL)
+    # in float32) and the cell-major cell (40 B / 20 B, in float32 also its
+    # time difference, 20 B)
+    rec = (80 + 2 * 4 * _lib.CELL) if precision == "mixed" else (160 + 8 * _lib.CELL)
     hist = slots * grid * grid * rec
     raw = slots * 5 * nb * nb * 8
     particles = n * 8 * (6 * 2 + 4)           # coordinates, tracking temporary, scratch

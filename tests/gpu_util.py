@@ -60,6 +60,15 @@ def cell_ring(data, cap, head, dtype):
     return ring
 
 
+def time_diffs32(d32):
+    """(nt, ...) float32 slices -> slice i + 1 minus slice i in float32, 0 for
+    the newest: the float32 time differences a mixed wake reads beside the
+    cell-major ring."""
+    out = np.zeros_like(d32)
+    out[:-1] = d32[1:] - d32[:-1]
+    return out
+
+
 class DeviceHistory:
     """A device ring holding an oracle DFTracker's stacked history.
 
@@ -85,9 +94,12 @@ class DeviceHistory:
         self.cells = self.cells32 = None
         if "cells" in rings:
             self.cells = torch.from_numpy(cell_ring(data, cap, head, np.float64)).to(device)
+        self.dcells32 = None
         if "cells32" in rings:
-            self.cells32 = torch.from_numpy(cell_ring(data.astype(np.float32), cap, head,
-                                                      np.float32)).to(device)
+            d32 = data.astype(np.float32)
+            self.cells32 = torch.from_numpy(cell_ring(d32, cap, head, np.float32)).to(device)
+            self.dcells32 = torch.from_numpy(cell_ring(time_diffs32(d32), cap, head,
+                                                       np.float32)).to(device)
         a = tracker.axes()
 
         def p(t):
@@ -95,7 +107,7 @@ class DeviceHistory:
         self.desc = _lib.History(p(self.h64), p(self.h32), p(self.h32t), p(self.cells),
                                  p(self.cells32), cap, head, nt, Xi, Zi, float(a["min_t"]),
                                  float(a["min_x"]), float(a["min_z"]), float(a["dt"]),
-                                 float(a["dx"]), float(a["dz"]))
+                                 float(a["dx"]), float(a["dz"]), p(self.dcells32))
 
 
 class DeviceLattice:

@@ -27,7 +27,7 @@ def test_library_exports_header_symbols():
     for s in syms:
         assert hasattr(lib, s), s
     assert set(syms) <= set(_lib.EXPORTED) | {"dfcsr_savgol_operator"}
-    assert lib.dfcsr_abi_version() == _lib.ABI_VERSION == 5
+    assert lib.dfcsr_abi_version() == _lib.ABI_VERSION == 6
 
 
 def test_argument_errors_without_device():
@@ -50,6 +50,17 @@ def test_argument_errors_without_device():
     H = _lib.History(None, None, None, None, None, 4, 0, 2, 8, 8, 0, 0, 0, 1, 1, 1)
     rc = lib.dfcsr_csr_integrand(H, _lib.Lattice(), 0.0, 0.0, 0.0, 1, 1, 1, 1, 1, 1, None)
     assert rc == -1 and b"without a ring" in lib.dfcsr_last_error()
+    # the time differences of the float32 cell-major ring: both rings, a window
+    # inside the ring; a window that starts past its newest slice has no work
+    rc = lib.dfcsr_cells32_tdiff(1, None, 8, 8, 4, 0, 2, 0, None)
+    assert rc == -1 and b"NULL ring" in lib.dfcsr_last_error()
+    for cap, head, nt in ((4, 4, 2), (4, 0, 5), (0, 0, 0), (4, -1, 2)):
+        rc = lib.dfcsr_cells32_tdiff(1, 2, 8, 8, cap, head, nt, 0, None)
+        assert rc == -1 and b"cap" in lib.dfcsr_last_error(), (cap, head, nt)
+    rc = lib.dfcsr_cells32_tdiff(1, 2, 0, 8, 4, 0, 2, 0, None)
+    assert rc == -1 and b"grid" in lib.dfcsr_last_error()
+    assert lib.dfcsr_cells32_tdiff(1, 2, 8, 8, 4, 0, 2, 2, None) == 0
+    assert lib.dfcsr_cells32_tdiff(1, 2, 8, 8, 4, 0, 0, 0, None) == 0
 
 
 @pytest.mark.parametrize("window,order", [(5, 2), (7, 2), (9, 3), (5, 0), (9, 1), (21, 4)])

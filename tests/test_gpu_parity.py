@@ -212,6 +212,10 @@ def test_get_df_ragged_grids_vs_oracle(bins, window, order):
 
 # -- a8-a10 --------------------------------------------------------------------------
 def test_history_golden(golden_dir):
+    """The device history against the reference's: re-grid flags, window,
+    axes and fields.  Through the same full re-grids, window pops and ring
+    growth, the float32 time-difference ring stays slice i + 1 minus slice i
+    of the float32 cell-major ring (bit-exact), 0 for the newest slice."""
     from pydfcsr_amd.deposit import DF_tracker
     d = _g(golden_dir, "ref_history.npz")
     tr = DF_tracker(dict(xbins=40, zbins=48, filter_order=2, filter_window=7,
@@ -222,6 +226,11 @@ def test_history_golden(golden_dir):
         tr.get_DF(_t(x), _t(z), _t(px), float(d["times"][k]))
         tr.append_DF()
         flags.append(int(tr.append_interpolant(float(d["fls"][k]), float(d["n_fl"]))))
+        idx = [tr._slot(i) for i in range(tr.nt)]
+        c32 = tr.cells32.cpu().numpy()[idx]
+        want = np.zeros_like(c32)
+        want[:-1] = c32[1:] - c32[:-1]
+        assert np.array_equal(tr.dcells32.cpu().numpy()[idx], want), k
         if k >= 1:
             tr.build_interpolant()
     assert flags == list(d["regrid_flags"])
@@ -337,14 +346,36 @@ def test_wake_mixed_precision_golden(K, ws, golden_dir, golden_history):
         assert rel_max(o["x_kick"].cpu().numpy(), w[key][:, 1]) <= 1e-4
 
 
+@pytest.mark.parametrize("Xi,Zi", [(6, 8), (5, 7)])   # float4 and scalar plane paths
+def test_cells32_tdiff_matches_numpy(K, Xi, Zi):
+    """The time differences of the float32 cell-major ring: for each logical
+    slice i of a wrapped window (first .. nt-1), slice i + 1 minus slice i in
+    float32 (numpy's float32 subtraction, bit-exact), 0 for the newest; no
+    other slot is written."""
+    rng = np.random.default_rng(Xi * 31 + Zi)
+    cap, head, nt = 9, 6, 7
+    scale = 10.0 ** rng.uniform(-3, 3, (cap, 1, 1, 5))
+    c = (rng.standard_normal((cap, Xi, Zi, 5)) * scale).astype(np.float32)
+    sentinel = 12345.0
+    cd = torch.from_numpy(c).to(DEV)
+    for first in (0, 3, nt - 1, nt):
+        d = torch.full((cap, Xi, Zi, 5), sentinel, dtype=torch.float32, device=DEV)
+        K.cells32_tdiff(cd, d, head, nt, first)
+        want = np.full_like(c, sentinel)
+        for i in range(first, nt):
+            s0, s1 = (head + i) % cap, (head + i + 1) % cap
+            want[s0] = 0.0 if i == nt - 1 else c[s1] - c[s0]
+        assert np.array_equal(d.cpu().numpy(), want), first
+
+
 def test_wake_mixed_precision_ring_layouts(K, ws, golden_dir, golden_history):
     """The mixed form reads either float32 ring layout (chunked, time-innermost,
     or both: region 0 the time-innermost one, the later regions the chunked
-    one), with or without the float32 cell-major ring for the later regions
-    (slots without a slice hold NaN), with the same arithmetic, so the wakes
-    are bit-identical across layouts; without the float64 ring, vx at the
-    observer comes from a float32 ring, within 1e-4 of max|W| of the
-    reference's float64 wakes."""
+    one), with or without the float32 cell-major ring and its time
+    differences for the later regions (slots without a slice hold NaN), with
+    the same arithmetic, so the wakes are bit-identical across layouts;
+    without the float64 ring, vx at the observer comes from a float32 ring,
+    within 1e-4 of max|W| of the reference's float64 wakes."""
     w, beam, _, DL, P = _golden_wake_inputs(golden_dir, golden_history)
     _, T = golden_history
     res = {}

@@ -89,14 +89,18 @@ struct CellRow {
   uint32_t rb, dxb;
 };
 
-// The five fields of the four corner cells at byte offsets off[c] of a
-// float64 cell plane (40-byte cells).
-__device__ inline void load_cells5(const char *plane, const uint32_t off[4], double v[4][5]) {
+// The corners (x0,z), (x0,z+1), (x1,z), (x1,z+1) of a float64 cell plane
+// from byte oa (x0, z) and ob (x1, z): each x row's two cells as 80
+// contiguous bytes.
+__device__ inline void load_pairs5(const char *plane, uint32_t oa, uint32_t ob, double v[4][5]) {
+  const double *a = reinterpret_cast<const double *>(plane + oa);
+  const double *b = reinterpret_cast<const double *>(plane + ob);
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const double *p = reinterpret_cast<const double *>(plane + off[c]);
-#pragma unroll
-    for (int f = 0; f < 5; ++f) v[c][f] = p[f];
+  for (int f = 0; f < 5; ++f) {
+    v[0][f] = a[f];
+    v[1][f] = a[5 + f];
+    v[2][f] = b[f];
+    v[3][f] = b[5 + f];
   }
 }
 
@@ -123,6 +127,19 @@ __device__ inline double trilerp(double a00, double a01, double a10, double a11,
   const double c00 = lerp<kFused>(a00, b00, td), c01 = lerp<kFused>(a01, b01, td);
   const double c10 = lerp<kFused>(a10, b10, td), c11 = lerp<kFused>(a11, b11, td);
   return lerp<kFused>(lerp<kFused>(c00, c10, xd), lerp<kFused>(c01, c11, xd), zd);
+}
+
+// trilerp of a z pair read one cell low at the grid's last z (zlast): the
+// upper z value serves both z corners, as the clamped record gives them.
+template <bool kFused>
+__device__ inline double trilerp_zpair(double a00, double a01, double a10, double a11, double b00,
+                                       double b01, double b10, double b11, double td, double xd,
+                                       double zd, bool zlast) {
+  const double c00 = lerp<kFused>(a00, b00, td), c01 = lerp<kFused>(a01, b01, td);
+  const double c10 = lerp<kFused>(a10, b10, td), c11 = lerp<kFused>(a11, b11, td);
+  const double hi = lerp<kFused>(c01, c11, xd);
+  const double lo = zlast ? hi : lerp<kFused>(c00, c10, xd);
+  return lerp<kFused>(lo, hi, zd);
 }
 
 // Trilinear lookup of all five fields at (t, x, z) with interpolate3D's
@@ -154,32 +171,33 @@ __device__ inline bool hist_lookup5(const DfcsrHistory &H, const HistAxes &A, do
   }
   const double td = T.frac, xd = X.frac, zd = Z.frac;
   if (kCells) {
-    // cell (slot, x, z) at byte ((slot Xi + x) Zi + z) * 40 of the cell-major ring,
-    // per corner (x0,z0), (x0,z1), (x1,z0), (x1,z1)
+    // cell (slot, x, z) at byte ((slot Xi + x) Zi + z) * 40 of the cell-major
+    // ring; each x row's cells z0 and z0 + 1 are read as one 80-byte pair
+    // (five 16-byte loads, against three per 40-byte cell).  At the grid's
+    // last z (interpolate3D clamps z1 to z0) the pair is (z0 - 1, z0), and
+    // the z0 value serves both z corners: the clamped record's values, so
+    // the result is the corner-packed ring's exactly.
     constexpr uint32_t cb = DFCSR_CELL * sizeof(double);
     const uint32_t rowB = (uint32_t)H.Zi * cb;
     const CellRow r = rr ? *rr : CellRow{(uint32_t)X.i0 * rowB, (uint32_t)(X.i1 - X.i0) * rowB};
     const int64_t planeB = (int64_t)H.Xi * rowB;
     const char *ring = reinterpret_cast<const char *>(H.cells);
-    const uint32_t o0 = r.rb + (uint32_t)Z.i0 * cb, oz = Z.i1 != Z.i0 ? cb : 0u;
-    const uint32_t off[4] = {o0, o0 + oz, o0 + r.dxb, o0 + r.dxb + oz};
+    const bool zlast = Z.i1 == Z.i0;   // Zi >= 2: the pair (z - 1, z)
+    const uint32_t o0 = r.rb + (uint32_t)(Z.i0 - (zlast ? 1 : 0)) * cb;
     double v0[4][5];
-    load_cells5(ring + hist_slot(H, T.i0) * planeB, off, v0);
-    // at the last plane interpolate3D clamps i1 to i0: both ends of the time
-    // lerp are the same cell, read once.  Each branch does its own lerps, so
-    // the second plane's loads need not wait for the first plane's values.
+    load_pairs5(ring + hist_slot(H, T.i0) * planeB, o0, o0 + r.dxb, v0);
     if (T.i1 != T.i0) {
       double v1[4][5];
-      load_cells5(ring + hist_slot(H, T.i1) * planeB, off, v1);
+      load_pairs5(ring + hist_slot(H, T.i1) * planeB, o0, o0 + r.dxb, v1);
 #pragma unroll
       for (int f = 0; f < 5; ++f)
-        out[f] = trilerp<kRecip>(v0[0][f], v0[1][f], v0[2][f], v0[3][f], v1[0][f], v1[1][f],
-                                 v1[2][f], v1[3][f], td, xd, zd);
+        out[f] = trilerp_zpair<kRecip>(v0[0][f], v0[1][f], v0[2][f], v0[3][f], v1[0][f],
+                                       v1[1][f], v1[2][f], v1[3][f], td, xd, zd, zlast);
     } else {
 #pragma unroll
       for (int f = 0; f < 5; ++f)
-        out[f] = trilerp<kRecip>(v0[0][f], v0[1][f], v0[2][f], v0[3][f], v0[0][f], v0[1][f],
-                                 v0[2][f], v0[3][f], td, xd, zd);
+        out[f] = trilerp_zpair<kRecip>(v0[0][f], v0[1][f], v0[2][f], v0[3][f], v0[0][f],
+                                       v0[1][f], v0[2][f], v0[3][f], td, xd, zd, zlast);
     }
     return true;
   }

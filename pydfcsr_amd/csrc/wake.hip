@@ -339,7 +339,7 @@ __device__ inline bool csr_point_mixed(const DfcsrHistory &H, const HistAxes &A,
   const TruncIdx T = trunc_index(ut, H.nt), Z = trunc_index(uz, H.Zi);
   if (!(T.ok && Z.ok)) return false;
   const float td = (float)T.frac;
-  float zd = (float)Z.frac;
+  const float zd = (float)Z.frac;
   float f[5];
   if (kCells != kRings) {
     // float32 cell (slot, x, z) at byte ((slot Xi + x) Zi + z) * 20 of the
@@ -348,19 +348,15 @@ __device__ inline bool csr_point_mixed(const DfcsrHistory &H, const HistAxes &A,
     // slice's cell and this one's, 0 at the newest slice, where
     // interpolate3D clamps i1 to i0 -- so one plane of each ring serves every
     // lookup, with no branch.  Each x row's two z cells are one 40-byte pair;
-    // at the last z (interpolate3D clamps z1 to z0) the pair is the one
-    // before, (z - 1, z), with weight 1 on its second cell, so the lerp gives
-    // that cell's value within one FP32 rounding and the loads need no branch.
+    // at the grid's last z (interpolate3D clamps z1 to z0) the pair is the one
+    // before, (z - 1, z), and the z value serves both z corners: the clamped
+    // record's values, so the result is the corner-packed rings' exactly.
     constexpr uint32_t cb = DFCSR_CELL * sizeof(float);
     const uint32_t planeB = (uint32_t)H.Xi * (uint32_t)H.Zi * cb;   // < 4 GiB (checked)
     const char *ring = reinterpret_cast<const char *>(H.cells32);
     const char *dring = reinterpret_cast<const char *>(H.dcells32);
-    int z0 = Z.i0;
-    if (Z.i1 == Z.i0) {   // Zi >= 2
-      z0 -= 1;
-      zd = 1.0f;
-    }
-    const uint32_t o = rr.rb + (uint32_t)z0 * cb;
+    const bool zlast = Z.i1 == Z.i0;   // Zi >= 2 (checked)
+    const uint32_t o = rr.rb + (uint32_t)(Z.i0 - (zlast ? 1 : 0)) * cb;
     const int s0 = hist_slot(H, T.i0);
     float v[4][5], d[4][5];
     if (kCells == kCellsRing32) {
@@ -378,8 +374,8 @@ __device__ inline bool csr_point_mixed(const DfcsrHistory &H, const HistAxes &A,
       const float c01 = fmaf(td, d[1][k], v[1][k]);
       const float c10 = fmaf(td, d[2][k], v[2][k]);
       const float c11 = fmaf(td, d[3][k], v[3][k]);
-      const float c0 = fmaf(xd, c10 - c00, c00);
       const float c1 = fmaf(xd, c11 - c01, c01);
+      const float c0 = zlast ? c1 : fmaf(xd, c10 - c00, c00);
       f[k] = fmaf(zd, c1 - c0, c0);
     }
   } else {
@@ -1051,6 +1047,10 @@ int check_rings(const DfcsrHistory &H, int precision, const char *who) {
   if (cells && (int64_t)H.Xi * H.Zi * cellB > ((int64_t)1 << 32))
     return set_error(DFCSR_EINVAL, "%s: a cell-major plane of %d x %d cells exceeds 4 GiB", who,
                      H.Xi, H.Zi);
+  // a cell-major lookup reads the z pair (z0 - 1, z0) at the grid's last z
+  if (cells && H.Zi < 2)
+    return set_error(DFCSR_EINVAL, "%s: a cell-major ring needs at least 2 z cells, got %d", who,
+                     H.Zi);
   if (precision == DFCSR_PREC_MIXED && !H.data32 && !H.data32t)
     return set_error(DFCSR_EINVAL, "%s: mixed precision needs an FP32 history ring", who);
   return DFCSR_OK;

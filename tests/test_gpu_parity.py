@@ -295,6 +295,89 @@ def test_wake_golden(K, ws, golden_dir, golden_history):
         assert torch.equal(out["x_kick"], o["x_kick"]), wrap
 
 
+def _last_z_lookups(T, L, beam, t, obs):
+    """Valid near-region lookups (regions 2 and 3 of the non-chirp branch,
+    CSR.py:461-485) of the observers obs[k] = (s - t, x) whose retarded z'
+    falls in the history's last z cell, where interpolate3D clamps z1 to z0;
+    from the oracle's region grids and lattice tables."""
+    a = T.axes()
+    nt, Zi = T.data["density"].shape[0], T.data["density"].shape[2]
+
+    def i1(col, v):
+        return O.interpolate1D(v, col, L.min_x, L.delta_x)
+
+    hits = 0
+    for z, x in obs:
+        s = t + z
+        regs = O.wake_regions(s, x, t, beam["sigma_z"], beam["sigma_x"], beam["slope"][0],
+                              beam["mean_x"], 0.05, 1.0, 12, 16)
+        for xp1, sp1 in regs[1:]:
+            xp, sp = [g.ravel() for g in np.meshgrid(xp1, sp1, indexing="ij")]
+            sv = np.array([s])
+            rx = (i1(L.coords[:, 0], sv)[0] - i1(L.coords[:, 0], sp)
+                  + x * i1(L.n_vec[:, 0], sv)[0] - xp * i1(L.n_vec[:, 0], sp))
+            ry = (i1(L.coords[:, 1], sv)[0] - i1(L.coords[:, 1], sp)
+                  + x * i1(L.n_vec[:, 1], sv)[0] - xp * i1(L.n_vec[:, 1], sp))
+            R = np.sqrt(rx ** 2 + ry ** 2)
+            ut = (t - R - a["min_t"]) / a["dt"]
+            ux = (xp - a["min_x"]) / a["dx"]
+            uz = (sp - (t - R) - a["min_z"]) / a["dz"]
+            ok = (ut > -1) & (ut < nt) & (ux > -1) & (ux < T.data["density"].shape[1])
+            hits += int(np.count_nonzero(ok & (uz >= Zi - 1) & (uz < Zi)))
+    return hits
+
+
+class _RolledHistory:
+    """A history's fields rolled by k cells along z (its axes unchanged): the
+    golden bunch's core moved to the top z edge of the interp grid."""
+
+    def __init__(self, T, k):
+        self.data = {f: np.roll(v, k, axis=2) for f, v in T.data.items()}
+        self._axes = T.axes()
+
+    def axes(self):
+        return self._axes
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_wake_cells_last_z_bitexact(K, ws, golden_dir, golden_history, precision):
+    """Observers at the top z edge of a history whose bunch core lies there,
+    so that near-region lookups fall in the grid's last z cell on nonzero
+    fields (interpolate3D clamps z1 to z0 there; the cell-major rings read
+    the pair (z - 1, z) and give its upper cell to both z corners): the
+    wakes through the cell-major rings equal those through the
+    corner-packed rings bit for bit, FP64 (float64 cells) and mixed (float32
+    cells and their time differences), and match the oracle within 1e-10
+    (FP64) / 1e-4 (mixed) of max|W|."""
+    w, beam, _, DL, _ = _golden_wake_inputs(golden_dir, golden_history)
+    _, T0 = golden_history
+    Zi = T0.data["density"].shape[2]
+    T = _RolledHistory(T0, Zi // 2)
+    L = O.LatticeTables(LAT_ELEMENTS)
+    a = T.axes()
+    max_z = a["min_z"] + (Zi - 1) * a["dz"]
+    obs_np = np.array([(max_z - f * a["dz"], beam["mean_x"] + g * beam["sigma_x"])
+                       for f in (0.05, 0.3, 0.7, 1.5) for g in (-1.0, 0.0, 1.0)])
+    assert _last_z_lookups(T, L, beam, 0.3, obs_np) > 0
+    obs = _t(obs_np)   # the parameters hold its device address: keep it alive
+    P = wake_params(0.3, beam, 0.05, 1.0, 12, 16, float(w["csr_scaling"]), (0.0, 0.0, 1),
+                    (0.0, 0.0, len(obs_np)), observers=obs)
+    rings = ("fp64",) if precision == 0 else ("fp32", "fp32t")
+    cells = ("cells",) if precision == 0 else ("cells32",)
+    res = []
+    for r in (rings, rings + cells):
+        DH = DeviceHistory(T, DEV, wrap=2, rings=r)
+        o = K.wake_mesh(DH.desc, DL.desc, P, precision, DEV, ws)
+        res.append(np.stack([o["dE_dct"].cpu().numpy(), o["x_kick"].cpu().numpy()]))
+    assert np.array_equal(res[0], res[1])
+    ref = np.array([O.csr_wake(0.3 + z, x, 0.3, beam, 0.05, 1.0, 12, 16, L, T,
+                               float(w["csr_scaling"])) for z, x in obs_np]).T
+    assert np.abs(ref[0]).max() > 0
+    tol = 1e-10 if precision == 0 else 1e-4
+    for k in range(2):
+        assert rel_max(res[0][k], ref[k]) <= tol, k
+
+
 @pytest.mark.parametrize("slope0,key", [(2.5, "chirp_pos"), (-2.5, "chirp_neg")])
 def test_wake_chirp_branch_golden(K, ws, golden_dir, golden_history, slope0, key):
     w, beam, DH, DL, _ = _golden_wake_inputs(golden_dir, golden_history)

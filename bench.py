@@ -21,7 +21,9 @@ timed steps: nominal integrand points (one point = one (x', s') source for
 one mesh point; M x P per step) per second of the retarded integral.
 ``points_evaluated`` counts the points the kernels actually evaluate (the
 others lie outside the history, where the reference's integrand is exactly
-0); ``roofline`` prices those at the cost model's 214 FP64 ops per point.
+0, or gather a density and gradients that the history's span table shows to
+be 0, which gives exactly 0 too: ``points_zero_skipped``); ``roofline``
+prices the evaluated points at the cost model's 214 FP64 ops per point.
 
 The JSON line also carries the roofline of the dominant work (the wake
 integral: its four launches timed together with HIP events on the stream
@@ -508,6 +510,12 @@ def main():
     evaluated = {name: kernels.wake_points_evaluated(H, Ld, Pw, prec, sim.device, sim.ws)
                  for name, prec in (("fp64", _lib.DFCSR_PREC_FP64),
                                     ("mixed", _lib.DFCSR_PREC_MIXED))}
+    # the points inside the history whose gathered density and gradients the
+    # span table shows to be 0 (an integrand of exactly 0): skipped
+    H_all = _lib.History.from_buffer_copy(H)
+    H_all.zspan = None
+    zero_skipped = kernels.wake_points_evaluated(H_all, Ld, Pw, _lib.DFCSR_PREC_FP64, sim.device,
+                                                 sim.ws) - evaluated["fp64"]
     nominal = sim.integrand_points()
     frac_eval = evaluated["fp64"] / nominal
     prec_name = "fp64" if args.precision == "fp64" else "mixed"
@@ -545,6 +553,7 @@ def main():
         "wake_ms": round(wake_avg_ms, 3),
         "points_nominal_per_wake": nominal,
         "points_evaluated_per_wake": evaluated,
+        "points_zero_skipped_per_wake": zero_skipped,
         "history": {"slices": tr.nt, "Xi": tr.Xi, "Zi": tr.Zi,
                     "bytes": int(tr.footprint_bytes()), "regrids": tr.n_regrid,
                     "cell_major": [k for k in ("cells", "cells32")

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define DFCSR_ABI_VERSION 6
+#define DFCSR_ABI_VERSION 7
 
 enum {
   DFCSR_OK = 0,
@@ -227,7 +227,29 @@ typedef struct {
   int32_t cap, head, nt, Xi, Zi;
   double min_t, min_x, min_z, dt, dx, dz;
   const float *dcells32; /* [cap][Xi][Zi][DFCSR_CELL] or NULL */
+  /* zspan (optional, Zi <= 65535): [Xi][cap] z spans, zlo | zhi << 16, from
+   * dfcsr_history_zspan.  Entry (x, slot of slice i) bounds the z of every
+   * cell of rows x and min(x + 1, Xi - 1), in slice i and (when i < nt - 1)
+   * slice i + 1, that can contribute: density, density_x or density_z
+   * nonzero, or vx or vx_x not finite (zlo > zhi: none).  A lookup whose
+   * corners lie outside the span of its x row and first time plane gathers
+   * fields whose integrand is exactly 0 (every term carries a factor
+   * density, density_x or density_z, and 0 times a finite value is 0): the
+   * wakes skip it, as they skip lookups outside the history. */
+  const uint32_t *zspan;
 } DfcsrHistory;
+
+/* Fills zspan (layout of DfcsrHistory) for the logical slices first .. nt-1
+ * of a ring of cap slots (slice i in slot (head + i) % cap) from its
+ * float64 cell-major ring cells, or when that is NULL from the float32 one
+ * cells32.  From cells, vx and vx_x beyond the float32 range count as not
+ * finite, so the spans hold for wakes of either precision; from cells32
+ * they hold for mixed-precision wakes only (a float64 value may round to a
+ * float32 zero).  Call after dfcsr_regrid with first as for
+ * dfcsr_cells32_tdiff.  Zi <= 65535.  Has no counterpart in the reference:
+ * it lets the wakes skip lookups whose integrand is exactly 0. */
+int dfcsr_history_zspan(const double *cells, const float *cells32, uint32_t *zspan, int Xi, int Zi,
+                        int cap, int head, int nt, int first, void *stream);
 
 /* Fills dcells32 (layout of DfcsrHistory) for the logical slices
  * first .. nt-1 of a ring of cap slots whose slice i lives in slot

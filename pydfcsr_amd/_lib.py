@@ -20,7 +20,7 @@ c_i32 = ctypes.c_int32
 c_i64 = ctypes.c_int64
 c_f64 = ctypes.c_double
 
-ABI_VERSION = 6   # DFCSR_ABI_VERSION
+ABI_VERSION = 7   # DFCSR_ABI_VERSION
 DFCSR_PREC_FP64 = 0
 DFCSR_PREC_MIXED = 1
 MOMENTS_COUNT = 10
@@ -53,7 +53,8 @@ class History(ctypes.Structure):
     _fields_ = [("data", c_dp), ("data32", c_dp), ("data32t", c_dp), ("cells", c_dp),
                 ("cells32", c_dp), ("cap", c_i32), ("head", c_i32), ("nt", c_i32), ("Xi", c_i32),
                 ("Zi", c_i32), ("min_t", c_f64), ("min_x", c_f64), ("min_z", c_f64),
-                ("dt", c_f64), ("dx", c_f64), ("dz", c_f64), ("dcells32", c_dp)]
+                ("dt", c_f64), ("dx", c_f64), ("dz", c_f64), ("dcells32", c_dp),
+                ("zspan", c_dp)]
 
 
 class Lattice(ctypes.Structure):
@@ -111,6 +112,8 @@ _SIGNATURES = [
                              c_dp, c_dp, c_i32, c_i32, c_i32, c_f64, c_f64, c_f64, c_f64, c_dp,
                              c_i64, c_dp]),
     ("dfcsr_cells32_tdiff", c_i32, [c_dp, c_dp, c_i32, c_i32, c_i32, c_i32, c_i32, c_i32, c_dp]),
+    ("dfcsr_history_zspan", c_i32, [c_dp, c_dp, c_dp, c_i32, c_i32, c_i32, c_i32, c_i32, c_i32,
+                                    c_dp]),
     ("dfcsr_interpolate3d", c_i32, [ctypes.POINTER(History), c_i32, c_dp, c_i64, c_dp, c_dp]),
     ("dfcsr_interpolate1d", c_i32, [ctypes.POINTER(Lattice), c_i32, c_dp, c_i64, c_dp, c_dp]),
     ("dfcsr_csr_integrand", c_i32, [ctypes.POINTER(History), ctypes.POINTER(Lattice), c_f64,

@@ -77,6 +77,23 @@ __device__ inline int hist_slot(const DfcsrHistory &H, int ti) {
   return slot >= H.cap ? slot - H.cap : slot;
 }
 
+// The newest slices whose span entries a near-region workgroup stages in
+// LDS for each of its x rows.
+constexpr int kSpanPlanes = 16;
+
+// Whether a valid lookup at x row xi, first time plane ti and z indices Z
+// may gather a contributing cell: whether its z corners reach the span of
+// (xi, slot of ti), which covers rows xi, xi + 1 and planes ti, ti + 1.
+// zrow: that row's entries for slices zt0 .. zt0 + kSpanPlanes - 1 (LDS);
+// older slices read the table itself.  When false, every gathered density,
+// density_x and density_z is 0 and vx, vx_x are finite, so the integrand is
+// exactly 0.
+__device__ inline bool zspan_holds(const DfcsrHistory &H, const uint32_t *zrow, int zt0, int xi,
+                                   int ti, const TruncIdx &Z) {
+  const uint32_t e = ti >= zt0 ? zrow[ti - zt0] : H.zspan[(int64_t)xi * H.cap + hist_slot(H, ti)];
+  return Z.i1 >= (int)(e & 0xffffu) && Z.i0 <= (int)(e >> 16);
+}
+
 // FP64 record (x, z, slot of ti): [x][z][slot][20] layout.
 __device__ inline const double *hist_rec(const DfcsrHistory &H, int ti, int xi, int zi) {
   return H.data + (((int64_t)xi * H.Zi + zi) * H.cap + hist_slot(H, ti)) * DFCSR_HIST_REC64;
@@ -150,11 +167,15 @@ __device__ inline double trilerp_zpair(double a00, double a01, double a10, doubl
 // one; the values, and so the result, are the same.
 // Xpre: the x index of xq when the caller has it already (the same value);
 // rr: that row's offsets in a cell plane (computed here when absent).
+// zrow (source points of the near regions, with a span table): that row's
+// staged span entries (zspan_holds); a lookup they show to gather no
+// contributing cell returns false (zeros) -- its integrand would be exactly 0.
 template <bool kRecip, bool kCells = false>
 __device__ inline bool hist_lookup5(const DfcsrHistory &H, const HistAxes &A, double tq,
                                     double xq, double zq, double out[5],
                                     const TruncIdx *Xpre = nullptr,
-                                    const CellRow *rr = nullptr) {
+                                    const CellRow *rr = nullptr,
+                                    const uint32_t *zrow = nullptr, int zt0 = 0) {
   double ut = kRecip ? (tq - A.min_t) * A.it : (tq - A.min_t) / A.dt;
   double uz = kRecip ? (zq - A.min_z) * A.iz : (zq - A.min_z) / A.dz;
   TruncIdx X;
@@ -165,7 +186,7 @@ __device__ inline bool hist_lookup5(const DfcsrHistory &H, const HistAxes &A, do
     X = trunc_index(ux, H.Xi);
   }
   TruncIdx T = trunc_index(ut, H.nt), Z = trunc_index(uz, H.Zi);
-  if (!(T.ok && X.ok && Z.ok)) {
+  if (!(T.ok && X.ok && Z.ok) || (zrow && !zspan_holds(H, zrow, zt0, X.i0, T.i0, Z))) {
     for (int f = 0; f < 5; ++f) out[f] = 0.0;
     return false;
   }
@@ -287,13 +308,16 @@ __device__ inline double recip_newton(double a) {
 // One integrand point (I_z, I_x) at source (x', s'), CSR.py:645-775 in the
 // reference's operation order.  Returns false (I = 0) when the retarded
 // lookup falls outside the history, where every gathered field is zero and
-// the reference's formula gives exactly 0 -- except at R == 0, where its
-// divisions give 0/0: that point is evaluated (with zero fields) as it is.
+// the reference's formula gives exactly 0, and (with zrow) when the span
+// table shows that the gathered density and its gradients are all 0, which
+// gives exactly 0 too -- except at R == 0, where the reference's divisions
+// give 0/0: that point is evaluated (with zero fields) as it is.
 template <bool kRecip, bool kCells = false>
 __device__ inline bool csr_point(const DfcsrHistory &H, const HistAxes &A, const Observer &o,
                                  const Column &c, double xp, double &Iz, double &Ix,
                                  const TruncIdx *Xpre = nullptr,
-                                 const CellRow *rr = nullptr) {
+                                 const CellRow *rr = nullptr,
+                                 const uint32_t *zrow = nullptr, int zt0 = 0) {
   const double rx = c.ax - xp * c.L.nx;
   const double ry = c.ay - xp * c.L.ny;
   double R, iR = 0.0;
@@ -306,7 +330,8 @@ __device__ inline bool csr_point(const DfcsrHistory &H, const HistAxes &A, const
   }
   const double t_ret = o.t - R;
   double f[5];
-  if (!hist_lookup5<kRecip, kCells>(H, A, t_ret, xp, c.sp - t_ret, f, Xpre, rr) && R != 0.0) {
+  if (!hist_lookup5<kRecip, kCells>(H, A, t_ret, xp, c.sp - t_ret, f, Xpre, rr, zrow, zt0) &&
+      R != 0.0) {
     Iz = 0.0;
     Ix = 0.0;
     return false;

@@ -153,10 +153,100 @@ __global__ __launch_bounds__(kThreads) void tdiff_kernel(const float *__restrict
   }
 }
 
+// Whether a cell can contribute to a wake integrand: density, density_x or
+// density_z nonzero, or vx or vx_x not finite (beyond the float32 range for
+// a float64 cell, so that a span holds for float32 copies too).
+template <typename T>
+__device__ inline bool cell_contributes(const T *c) {
+  const T big = (T)3.4028234663852886e38;   // FLT_MAX
+  const T vx = c[DFCSR_F_VX], vxx = c[DFCSR_F_VX_X];
+  return c[DFCSR_F_DENSITY] != (T)0 || c[DFCSR_F_DENSITY_X] != (T)0 ||
+         c[DFCSR_F_DENSITY_Z] != (T)0 || !(vx <= big && vx >= -big) ||
+         !(vxx <= big && vxx >= -big);
+}
+
+constexpr int kSpanRows = 16;   // x rows per workgroup of zspan_kernel
+
+// zspan entries (x, slot of slice i) for x in one block of kSpanRows rows:
+// each wave finds the z span of whole rows (its own cells) of slices i and
+// i + 1 into LDS, then each entry joins rows x, x + 1 of both slices.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void zspan_kernel(const T *__restrict__ cells,
+                                                         uint32_t *__restrict__ zspan, int Xi,
+                                                         int Zi, int cap, int head, int nt,
+                                                         int first) {
+  __shared__ int lo_s[2][kSpanRows + 1], hi_s[2][kSpanRows + 1];
+  const int xb = blockIdx.x * kSpanRows;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = kThreads / 64;
+  const int nrow = min(kSpanRows + 1, Xi - xb);   // rows xb .. xb + nrow - 1 (x + 1 clamped)
+  const int64_t plane = (int64_t)Xi * Zi * DFCSR_CELL;
+  for (int i = first + (int)blockIdx.y; i < nt; i += (int)gridDim.y) {
+    const int np = i + 1 < nt ? 2 : 1;
+    for (int w = wave; w < np * nrow; w += nwave) {
+      const int p = w / nrow, r = w - p * nrow;
+      const T *row = cells + (int64_t)((head + i + p) % cap) * plane +
+                     (int64_t)(xb + r) * Zi * DFCSR_CELL;
+      int lo = 0x7fffffff, hi = -1;
+      for (int z = lane; z < Zi; z += 64) {
+        if (cell_contributes(row + (int64_t)z * DFCSR_CELL)) {
+          lo = min(lo, z);
+          hi = max(hi, z);
+        }
+      }
+#pragma unroll
+      for (int d = 32; d >= 1; d >>= 1) {
+        lo = min(lo, __shfl_xor(lo, d, 64));
+        hi = max(hi, __shfl_xor(hi, d, 64));
+      }
+      if (lane == 0) {
+        lo_s[p][r] = lo;
+        hi_s[p][r] = hi;
+      }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < min(kSpanRows, Xi - xb)) {
+      const int r = threadIdx.x, r1 = min(r + 1, nrow - 1);
+      int lo = min(lo_s[0][r], lo_s[0][r1]), hi = max(hi_s[0][r], hi_s[0][r1]);
+      if (np == 2) {
+        lo = min(lo, min(lo_s[1][r], lo_s[1][r1]));
+        hi = max(hi, max(hi_s[1][r], hi_s[1][r1]));
+      }
+      // no contributing cell: zlo = 0xffff > zhi = 0
+      const uint32_t e = hi < 0 ? 0xffffu : ((uint32_t)lo | ((uint32_t)hi << 16));
+      zspan[(int64_t)(xb + r) * cap + (head + i) % cap] = e;
+    }
+    __syncthreads();   // before the next slice overwrites the row spans
+  }
+}
+
 }  // namespace
 }  // namespace dfcsr
 
 using namespace dfcsr;
+
+extern "C" int dfcsr_history_zspan(const double *cells, const float *cells32, uint32_t *zspan,
+                                   int Xi, int Zi, int cap, int head, int nt, int first,
+                                   void *stream) {
+  if ((!cells && !cells32) || !zspan) return set_error(DFCSR_EINVAL, "history_zspan: NULL ring");
+  if (Xi < 1 || Zi < 1 || Zi > 0xffff)
+    return set_error(DFCSR_EINVAL, "history_zspan: grid %d x %d (Zi <= 65535)", Xi, Zi);
+  if (cap < 1 || nt < 0 || nt > cap || head < 0 || head >= cap)
+    return set_error(DFCSR_EINVAL, "history_zspan: cap %d head %d nt %d", cap, head, nt);
+  if (first < 0) first = 0;
+  if (first >= nt) return DFCSR_OK;
+  const int gx = (Xi + kSpanRows - 1) / kSpanRows;
+  int gy = nt - first;
+  if (gy > 1024) gy = 1024;
+  hipStream_t s = as_stream(stream);
+  if (cells)
+    hipLaunchKernelGGL(zspan_kernel<double>, dim3(gx, gy), dim3(kThreads), 0, s, cells, zspan,
+                       Xi, Zi, cap, head, nt, first);
+  else
+    hipLaunchKernelGGL(zspan_kernel<float>, dim3(gx, gy), dim3(kThreads), 0, s, cells32, zspan,
+                       Xi, Zi, cap, head, nt, first);
+  DFCSR_LAUNCH_CHECK("history_zspan");
+  return DFCSR_OK;
+}
 
 extern "C" int dfcsr_cells32_tdiff(const float *cells32, float *dcells32, int Xi, int Zi, int cap,
                                    int head, int nt, int first, void *stream) {

@@ -316,14 +316,17 @@ __device__ inline void load_pairs32(const char *base, uint32_t oa, uint32_t ob, 
 // rings' starts, which the loads add to the rings' scalar addresses.
 enum CellMode { kRings = 0, kCellsPlane = 1, kCellsRing32 = 2 };
 
-// One source point in mixed precision; false when outside the history.
+// One source point in mixed precision; false when outside the history or
+// (zrow: the x row's staged span entries) when the span table shows an
+// integrand of exactly 0 (zspan_holds).
 // kTI: read the time-innermost FP32 ring, else the chunked one.  kCells
 // (a CellMode): read the float32 cell-major ring H.cells32 and its time
 // differences H.dcells32 instead.
 template <bool kTI, int kCells = kRings>
 __device__ inline bool csr_point_mixed(const DfcsrHistory &H, const HistAxes &A, double ut0,
                                        const ColumnMixed &c, double xp, float xpf, int xi0,
-                                       int xi1, CellRow rr, float xd, float &Iz, float &Ix) {
+                                       int xi1, CellRow rr, float xd, float &Iz, float &Ix,
+                                       const uint32_t *zrow = nullptr, int zt0 = 0) {
   const double rx = c.ax - xp * c.npx;
   const double ry = c.ay - xp * c.npy;
   const double R2 = rx * rx + ry * ry;
@@ -337,7 +340,7 @@ __device__ inline bool csr_point_mixed(const DfcsrHistory &H, const HistAxes &A,
   const double ut = ut0 - R * A.it;               // (t - R - min_t) / dt
   const double uz = c.uz0 + R * A.iz;             // (s' - (t - R) - min_z) / dz
   const TruncIdx T = trunc_index(ut, H.nt), Z = trunc_index(uz, H.Zi);
-  if (!(T.ok && Z.ok)) return false;
+  if (!(T.ok && Z.ok) || (zrow && !zspan_holds(H, zrow, zt0, xi0, T.i0, Z))) return false;
   const float td = (float)T.frac;
   const float zd = (float)Z.frac;
   float f[5];
@@ -579,7 +582,7 @@ __device__ inline void observer_point(const WakeTask &T, int i, double *__restri
 // kTI (mixed only): read the time-innermost FP32 ring.  kCells (a
 // CellMode): read the cell-major ring (H.cells for FP64; H.cells32 and its
 // time differences H.dcells32 for mixed).
-template <bool kMixed, bool kTI, int kCells = kRings>
+template <bool kMixed, bool kTI, int kCells = kRings, bool kSkip = false>
 __device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int ph, double *red,
                            RowTable &rt, double &out_z, double &out_x) {
   const DfcsrHistory &H = T.H;
@@ -633,6 +636,18 @@ __device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int
   o.L.tx = od[6];
   o.L.ty = od[7];
   o.vx = od[8];
+  // kSkip (the near regions, whose lookups fall on the newest slices): the
+  // span entries of the phase's rows for the newest kSpanPlanes slices
+  __shared__ uint32_t zs[kSkip ? kMaxRows : 1][kSpanPlanes];
+  const bool skip = kSkip && H.zspan != nullptr;   // uniform
+  const int zt0 = max(0, H.nt - kSpanPlanes);
+  if (skip) {
+    for (int e = threadIdx.x; e < (v1 - v0) * kSpanPlanes; e += blockDim.x) {
+      const int r = e / kSpanPlanes, j = e - r * kSpanPlanes;
+      const int xi = trunc_index((xa.at(v0 + r) - A.min_x) * A.ix, H.Xi).i0;
+      zs[r][j] = zt0 + j < H.nt ? H.zspan[(int64_t)xi * H.cap + hist_slot(H, zt0 + j)] : 0xffffu;
+    }
+  }
   for (int r = threadIdx.x; r < v1 - v0; r += blockDim.x) {
     const double xp = xa.at(v0 + r);
     const TruncIdx X = trunc_index((xp - A.min_x) * A.ix, H.Xi);
@@ -675,7 +690,8 @@ __device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int
         if (!rt.ok[r]) continue;
         float Iz, Ix;
         if (!csr_point_mixed<kTI, kCells>(H, A, ut0, col, rt.xp[r], rt.xpf[r], rt.i0[r],
-                                           rt.i1[r], rt.rr[r], rt.fracf[r], Iz, Ix))
+                                           rt.i1[r], rt.rr[r], rt.fracf[r], Iz, Ix,
+                                           skip ? zs[r] : nullptr, zt0))
           continue;
         ++evaluated;
         const float wx = rt.wf[r];
@@ -691,7 +707,8 @@ __device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int
         if (!rt.ok[r]) continue;
         const TruncIdx X = {rt.i0[r], rt.i1[r], rt.frac[r], true};
         double Iz, Ix;
-        if (!csr_point<true, kCells != kRings>(H, A, o, col, rt.xp[r], Iz, Ix, &X, &rt.rr[r]))
+        if (!csr_point<true, kCells != kRings>(H, A, o, col, rt.xp[r], Iz, Ix, &X, &rt.rr[r],
+                                                skip ? zs[r] : nullptr, zt0))
           continue;
         ++evaluated;
         const double wx = rt.w[r];
@@ -731,8 +748,8 @@ __device__ inline void run_phase_block(const WakeTask &T, int l, double *__restr
   ph = kNear ? T.nA + q : q;
   __shared__ RowTable rt;
   double sz, sx;
-  wake_phase<kMixed, kTI, kCells>(T, partials + T.obs + (int64_t)i * kObsStride, ph, red, rt, sz,
-                                   sx);
+  wake_phase<kMixed, kTI, kCells, kNear>(T, partials + T.obs + (int64_t)i * kObsStride, ph, red,
+                                         rt, sz, sx);
   if (threadIdx.x == 0) {
     double *p = partials + T.part + ((int64_t)i * (T.nA + T.nB) + ph) * 2;
     p[0] = sz;

@@ -16,6 +16,9 @@ whole run:
   cells32[cap][Xi][Zi][5]      float32  the same in float32
   dcells32[cap][Xi][Zi][5]     float32  cells32 of the next slice minus this one's (0
                                         for the newest): the mixed wake's time lerp
+  zspan  [Xi][cap]             int32    per x row and slot the z span of the cells that
+                                        can contribute (rows x, x+1; slices i, i+1):
+                                        the wakes skip lookups outside it (integrand 0)
 (record (x, z) holds every field at (x, z), (x, z+1), (x+1, z), (x+1, z+1),
 clamped to the last plane: one record per time plane per lookup.  hist and
 hist32t keep the slot innermost (the two planes of a lookup are neighbours,
@@ -97,6 +100,7 @@ class DF_tracker:
         self.cells = None     # float64 cell-major (cap, Xi, Zi, 5)
         self.cells32 = None   # float32 cell-major (cap, Xi, Zi, 5)
         self.dcells32 = None  # its time differences (cap, Xi, Zi, 5)
+        self.zspan = None     # z spans of contributing cells (Xi, cap), uint32 as int32
         self.cur = None
         self.n_regrid = 0
         # largest interp grid side the rings may take (None: unbounded, as the
@@ -121,10 +125,10 @@ class DF_tracker:
             return
         new_cap = max(need, 2 * self.cap, 16)
         raw = torch.empty((new_cap, self._raw_stride()), dtype=torch.float64, device=self.device)
-        rings = (None,) * 6
+        rings = (None,) * 7
         if self.Xi:
             rings = self._rings(self.Xi, self.Zi, new_cap)
-        hist, hist32, hist32t, cells, cells32, dcells32 = rings
+        hist, hist32, hist32t, cells, cells32, dcells32, zspan = rings
         if self.cap:
             # keep the ring's logical order, pending slot included (index nt)
             for i in range(min(self.nt + 1, self.cap)):
@@ -140,17 +144,21 @@ class DF_tracker:
                                  (dcells32, self.dcells32)):
                     if new is not None and old is not None:
                         new[i].copy_(old[s])
+                if zspan is not None and self.zspan is not None:
+                    zspan[:, i].copy_(self.zspan[:, s])
         self.raw, self.hist, self.hist32, self.hist32t = raw, hist, hist32, hist32t
         self.cells, self.cells32, self.dcells32 = cells, cells32, dcells32
+        self.zspan = zspan
         self.cap = new_cap
         self.head = 0
 
     def _rings(self, Xi, Zi, cap):
-        """(hist, hist32, hist32t, cells, cells32, dcells32) for an Xi x Zi
-        grid.  The wakes address cells within a plane with 32-bit offsets: a
+        """(hist, hist32, hist32t, cells, cells32, dcells32, zspan) for an Xi x
+        Zi grid.  The wakes address cells within a plane with 32-bit offsets: a
         cell-major ring whose plane would pass 4 GiB is not kept (the rings
-        serve)."""
-        hist = hist32 = hist32t = cells = cells32 = dcells32 = None
+        serve).  The span table comes with a cell-major ring (it is derived
+        from one) when z indices fit 16 bits."""
+        hist = hist32 = hist32t = cells = cells32 = dcells32 = zspan = None
         dev = self.device
         if self.keep_fp64:
             hist = torch.empty((Xi, Zi, cap, _lib.HIST_REC64), dtype=torch.float64, device=dev)
@@ -164,15 +172,19 @@ class DF_tracker:
             if self.keep_chunked:
                 hist32 = torch.empty((Xi, cap, _lib.HIST_CHUNKS32, Zi, 4), dtype=torch.float32,
                                      device=dev)
-        return hist, hist32, hist32t, cells, cells32, dcells32
+        # spans from float32 cells hold for mixed wakes only (a float64 value
+        # may round to a float32 zero): from cells32 only without float64 rings
+        if (cells is not None or (cells32 is not None and not self.keep_fp64)) and Zi <= 0xffff:
+            zspan = torch.empty((Xi, cap), dtype=torch.int32, device=dev)
+        return hist, hist32, hist32t, cells, cells32, dcells32, zspan
 
     def _alloc_hist(self, Xi, Zi):
         if self.Xi != Xi or self.Zi != Zi or (self.hist is None and self.hist32t is None):
             # free first
             self.hist = self.hist32 = self.hist32t = None
-            self.cells = self.cells32 = self.dcells32 = None
+            self.cells = self.cells32 = self.dcells32 = self.zspan = None
             (self.hist, self.hist32, self.hist32t, self.cells, self.cells32,
-             self.dcells32) = self._rings(Xi, Zi, self.cap)
+             self.dcells32, self.zspan) = self._rings(Xi, Zi, self.cap)
         self.Xi, self.Zi = Xi, Zi
 
     def raw_fields(self, i=None):
@@ -276,10 +288,14 @@ class DF_tracker:
                            self.Xi, self.Zi, self.xi[0], self.xi[1], self.zi[0], self.zi[1],
                            self.ws, hist32t=self.hist32t, cells=self.cells,
                            cells32=self.cells32)
-        if self.dcells32 is not None and idx:
-            # the re-gridded slices and each one's predecessor change differences
-            kernels.cells32_tdiff(self.cells32, self.dcells32, self.head, self.nt,
-                                  first=max(0, min(idx) - 1))
+        # the re-gridded slices and each one's predecessor change differences
+        # and spans (a span covers the slice and the next)
+        first = max(0, min(idx) - 1) if idx else self.nt
+        if self.dcells32 is not None:
+            kernels.cells32_tdiff(self.cells32, self.dcells32, self.head, self.nt, first=first)
+        if self.zspan is not None:
+            kernels.history_zspan(self.cells, self.cells32, self.zspan, self.head, self.nt,
+                                  first=first)
 
     def build_interpolant(self):   # deposit.py:395-426
         nt = self.nt
@@ -299,7 +315,7 @@ class DF_tracker:
                             p(self.cells32), self.cap, self.head, self.nt, self.Xi, self.Zi,
                             float(self.min_x), float(self.min_y), float(self.min_z),
                             float(self.delta_x), float(self.delta_y), float(self.delta_z),
-                            p(self.dcells32))
+                            p(self.dcells32), p(self.zspan))
 
     def history_fields(self):
         """Stacked (nt, Xi, Zi) copies of the five fields in logical order
@@ -313,7 +329,8 @@ class DF_tracker:
         b = self.raw.numel() * 8 if self.raw is not None else 0
         if self.hist is not None:
             b += self.hist.numel() * 8
-        for r in (self.hist32, self.hist32t, self.cells, self.cells32, self.dcells32):
+        for r in (self.hist32, self.hist32t, self.cells, self.cells32, self.dcells32,
+                  self.zspan):
             if r is not None:
                 b += r.numel() * r.element_size()
         return b

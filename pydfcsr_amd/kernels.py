@@ -272,6 +272,31 @@ def cells32_tdiff(cells32: torch.Tensor, dcells32: torch.Tensor, head: int, nt: 
           "cells32_tdiff")
 
 
+def history_zspan(cells, cells32, zspan: torch.Tensor, head: int, nt: int, first: int = 0):
+    """Fills zspan (Xi, cap) int32 -- read as uint32 zlo | zhi << 16 -- for the
+    logical slices first .. nt-1 from the float64 cell-major ring cells (cap,
+    Xi, Zi, 5), or when that is None from the float32 one cells32: the z span
+    of the cells of rows x, x + 1 in slices i, i + 1 whose density or
+    gradients are nonzero or whose velocities are not finite."""
+    lib = _lib.load()
+    ring = cells if cells is not None else cells32
+    if ring is None:
+        raise DfcsrError("history_zspan: no cell-major ring")
+    want = torch.float64 if cells is not None else torch.float32
+    if (ring.dim() != 4 or ring.shape[3] != _lib.CELL or ring.dtype != want
+            or not ring.is_contiguous()):
+        raise DfcsrError(f"history_zspan: ring {tuple(ring.shape)} {ring.dtype} is not a "
+                         f"contiguous (cap, Xi, Zi, {_lib.CELL}) {want} ring")
+    cap, Xi, Zi, _ = ring.shape
+    if (tuple(zspan.shape) != (Xi, cap) or zspan.dtype != torch.int32
+            or not zspan.is_contiguous()):
+        raise DfcsrError(f"history_zspan: table {tuple(zspan.shape)} {zspan.dtype} is not a "
+                         f"contiguous ({Xi}, {cap}) int32 table")
+    check(lib.dfcsr_history_zspan(ptr(cells), ptr(cells32) if cells is None else None,
+                                  ptr(zspan), int(Xi), int(Zi), int(cap), int(head), int(nt),
+                                  int(first), stream_handle(ring.device)), "history_zspan")
+
+
 def interpolate3D(hist: _lib.History, field: int, q: torch.Tensor, device):
     lib = _lib.load()
     out = torch.empty(q.shape[0], dtype=torch.float64, device=device)

@@ -60,6 +60,32 @@ def cell_ring(data, cap, head, dtype):
     return ring
 
 
+def zspan_table(data, cap, head):
+    """(nt, Xi, Zi, 5) float64 slices -> (Xi, cap) int32 span table, read as
+    uint32 zlo | zhi << 16: for x row x and the slot of slice i, the z span of
+    the cells of rows x, min(x + 1, Xi - 1) in slices i and i + 1 (when
+    there is one) with density, density_x or density_z nonzero or vx, vx_x
+    beyond the float32 range or NaN; 0xffff (zlo > zhi) when there is none,
+    and for slots that hold no slice."""
+    nt, Xi, Zi, _ = data.shape
+    big = float(np.finfo(np.float32).max)
+    c = ((data[..., 0] != 0) | (data[..., 1] != 0) | (data[..., 2] != 0)
+         | ~(np.abs(data[..., 3]) <= big) | ~(np.abs(data[..., 4]) <= big))
+    x1 = np.minimum(np.arange(Xi) + 1, Xi - 1)
+    rows = c | c[:, x1]
+    both = rows.copy()
+    both[:-1] |= rows[1:]
+    out = np.full((Xi, cap), 0xFFFF, dtype=np.uint32)
+    anyz = both.any(axis=2)
+    lo = np.argmax(both, axis=2)
+    hi = Zi - 1 - np.argmax(both[:, :, ::-1], axis=2)
+    for i in range(nt):
+        e = np.where(anyz[i], lo[i].astype(np.uint32) | (hi[i].astype(np.uint32) << 16),
+                     np.uint32(0xFFFF))
+        out[:, (head + i) % cap] = e
+    return out.view(np.int32)
+
+
 def time_diffs32(d32):
     """(nt, ...) float32 slices -> slice i + 1 minus slice i in float32, 0 for
     the newest: the float32 time differences a mixed wake reads beside the
@@ -74,8 +100,8 @@ class DeviceHistory:
 
     wrap=k places logical slice 0 at slot cap-k so the ring wraps around.
     rings: which of the float64, float32 chunked and float32 time-innermost
-    rings and of the float64 / float32 cell-major rings ("cells",
-    "cells32") to build."""
+    rings, of the float64 / float32 cell-major rings ("cells", "cells32":
+    with its time differences) and of the span table ("zspan") to build."""
 
     def __init__(self, tracker, device="cuda", wrap=0, rings=("fp64", "fp32", "fp32t")):
         data = np.stack([tracker.data[f] for f in FIELDS], axis=-1)   # (nt, Xi, Zi, 5)
@@ -94,6 +120,9 @@ class DeviceHistory:
         self.cells = self.cells32 = None
         if "cells" in rings:
             self.cells = torch.from_numpy(cell_ring(data, cap, head, np.float64)).to(device)
+        self.zspan = None
+        if "zspan" in rings:
+            self.zspan = torch.from_numpy(zspan_table(data, cap, head)).to(device)
         self.dcells32 = None
         if "cells32" in rings:
             d32 = data.astype(np.float32)
@@ -107,7 +136,8 @@ class DeviceHistory:
         self.desc = _lib.History(p(self.h64), p(self.h32), p(self.h32t), p(self.cells),
                                  p(self.cells32), cap, head, nt, Xi, Zi, float(a["min_t"]),
                                  float(a["min_x"]), float(a["min_z"]), float(a["dt"]),
-                                 float(a["dx"]), float(a["dz"]), p(self.dcells32))
+                                 float(a["dx"]), float(a["dz"]), p(self.dcells32),
+                                 p(self.zspan))
 
 
 class DeviceLattice:

@@ -27,7 +27,7 @@ def test_library_exports_header_symbols():
     for s in syms:
         assert hasattr(lib, s), s
     assert set(syms) <= set(_lib.EXPORTED) | {"dfcsr_savgol_operator"}
-    assert lib.dfcsr_abi_version() == _lib.ABI_VERSION == 6
+    assert lib.dfcsr_abi_version() == _lib.ABI_VERSION == 7
 
 
 def test_argument_errors_without_device():
@@ -61,6 +61,16 @@ def test_argument_errors_without_device():
     assert rc == -1 and b"grid" in lib.dfcsr_last_error()
     assert lib.dfcsr_cells32_tdiff(1, 2, 8, 8, 4, 0, 2, 2, None) == 0
     assert lib.dfcsr_cells32_tdiff(1, 2, 8, 8, 4, 0, 0, 0, None) == 0
+    # the span table: a cell-major ring and the table, 16-bit z, a window in the ring
+    rc = lib.dfcsr_history_zspan(None, None, 3, 8, 8, 4, 0, 2, 0, None)
+    assert rc == -1 and b"NULL ring" in lib.dfcsr_last_error()
+    rc = lib.dfcsr_history_zspan(1, None, None, 8, 8, 4, 0, 2, 0, None)
+    assert rc == -1 and b"NULL ring" in lib.dfcsr_last_error()
+    rc = lib.dfcsr_history_zspan(1, None, 3, 8, 70000, 4, 0, 2, 0, None)
+    assert rc == -1 and b"65535" in lib.dfcsr_last_error()
+    rc = lib.dfcsr_history_zspan(None, 2, 3, 8, 8, 4, 4, 2, 0, None)
+    assert rc == -1 and b"cap" in lib.dfcsr_last_error()
+    assert lib.dfcsr_history_zspan(1, None, 3, 8, 8, 4, 0, 2, 2, None) == 0
 
 
 @pytest.mark.parametrize("window,order", [(5, 2), (7, 2), (9, 3), (5, 0), (9, 1), (21, 4)])

@@ -215,8 +215,10 @@ def test_history_golden(golden_dir):
     """The device history against the reference's: re-grid flags, window,
     axes and fields.  Through the same full re-grids, window pops and ring
     growth, the float32 time-difference ring stays slice i + 1 minus slice i
-    of the float32 cell-major ring (bit-exact), 0 for the newest slice."""
+    of the float32 cell-major ring (bit-exact), 0 for the newest slice, and
+    the span table the one the float64 cell-major ring gives."""
     from pydfcsr_amd.deposit import DF_tracker
+    from tests.gpu_util import zspan_table
     d = _g(golden_dir, "ref_history.npz")
     tr = DF_tracker(dict(xbins=40, zbins=48, filter_order=2, filter_window=7,
                          velocity_threhold=1000.0, upper_limit=40), device=DEV, capacity=3)
@@ -231,6 +233,9 @@ def test_history_golden(golden_dir):
         want = np.zeros_like(c32)
         want[:-1] = c32[1:] - c32[:-1]
         assert np.array_equal(tr.dcells32.cpu().numpy()[idx], want), k
+        c64 = tr.cells.cpu().numpy()[idx]
+        zs = tr.zspan.cpu().numpy()[:, idx]
+        assert np.array_equal(zs, zspan_table(c64, len(idx), 0)), k
         if k >= 1:
             tr.build_interpolant()
     assert flags == list(d["regrid_flags"])
@@ -449,6 +454,67 @@ def test_cells32_tdiff_matches_numpy(K, Xi, Zi):
             s0, s1 = (head + i) % cap, (head + i + 1) % cap
             want[s0] = 0.0 if i == nt - 1 else c[s1] - c[s0]
         assert np.array_equal(d.cpu().numpy(), want), first
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_history_zspan_matches_numpy(K, fp32):
+    """The span table from the float64 (or float32) cell-major ring: per x
+    row and slot of each slice first .. nt-1 of a wrapped window, the z span
+    of cells of rows x, x + 1 in slices i, i + 1 with a nonzero density or
+    gradient or a velocity that is not finite; no other slot written.  Slices
+    with rows of zeros, a NaN vx and values that are float32-exact."""
+    from tests.gpu_util import zspan_table, cell_ring
+    rng = np.random.default_rng(7 + fp32)
+    cap, head, nt, Xi, Zi = 9, 6, 7, 37, 53
+    data = np.zeros((nt, Xi, Zi, 5))
+    for i in range(nt):   # a band of contributing cells that moves, some rows empty
+        z0, z1 = 3 + 4 * i, 20 + 3 * i
+        data[i, 2:30, z0:z1, :3] = rng.standard_normal((28, z1 - z0, 3))
+        data[i, :, :, 3:] = rng.standard_normal((Xi, Zi, 2))
+    data[2, 5:9] = 0.0
+    data[3, 11, 40, 3] = np.nan       # vx not finite outside the band
+    data[4, 1:3, 50, 1] = 1.0         # lone gradients past the band
+    data = data.astype(np.float32).astype(np.float64)
+    dt = np.float32 if fp32 else np.float64
+    ring = torch.from_numpy(cell_ring(data.astype(dt), cap, head, dt)).to(DEV)
+    sentinel = np.int32(0x12345)
+    for first in (0, 3, nt - 1, nt):
+        zs = torch.full((Xi, cap), int(sentinel), dtype=torch.int32, device=DEV)
+        K.history_zspan(None if fp32 else ring, ring if fp32 else None, zs, head, nt, first)
+        want = zspan_table(data, cap, head)
+        for i in range(first):
+            want[:, (head + i) % cap] = sentinel
+        for s in range(cap):
+            if (s - head) % cap >= nt:
+                want[:, s] = sentinel
+        assert np.array_equal(zs.cpu().numpy(), want), first
+
+
+def test_wake_zspan_skips_exact_zeros(K, ws, golden_dir, golden_history):
+    """With the span table the wakes skip the lookups whose gathered density
+    and gradients are all 0 (an integrand of exactly 0): fewer points are
+    evaluated and the wakes are bit-identical, FP64 (corner-packed and
+    cell-major rings) and mixed (corner-packed and cell-major float32), on
+    the golden mesh and on the explicit observers of the chirp branch."""
+    w, beam, _, DL, P = _golden_wake_inputs(golden_dir, golden_history)
+    _, T = golden_history
+    b = dict(beam, slope=np.array([2.5, w["slope"][1]]))
+    rows = w["chirp_rows"]
+    obs = _t(np.stack([w["zmesh"][rows], w["xmesh"][rows]], axis=1))
+    Pc = wake_params(0.3, b, 0.05, 1.0, 12, 16, float(w["csr_scaling"]), (0.0, 0.0, 1),
+                     (0.0, 0.0, len(rows)), observers=obs)
+    cases = [(0, ("fp64",)), (0, ("fp64", "cells")), (1, ("fp32", "fp32t")),
+             (1, ("fp32", "fp32t", "cells32"))]
+    for prec, rings in cases:
+        for params in (P, Pc):
+            res, ev = [], []
+            for extra in ((), ("zspan",)):
+                DH = DeviceHistory(T, DEV, wrap=2, rings=rings + extra)
+                o = K.wake_mesh(DH.desc, DL.desc, params, prec, DEV, ws)
+                res.append(np.stack([o["dE_dct"].cpu().numpy(), o["x_kick"].cpu().numpy()]))
+                ev.append(K.wake_points_evaluated(DH.desc, DL.desc, params, prec, DEV, ws))
+            assert np.array_equal(res[0], res[1]), (prec, rings)
+            assert ev[1] < ev[0], (prec, rings, ev)
 
 
 def test_wake_mixed_precision_ring_layouts(K, ws, golden_dir, golden_history):

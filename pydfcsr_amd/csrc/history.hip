@@ -165,7 +165,9 @@ __device__ inline bool cell_contributes(const T *c) {
          !(vxx <= big && vxx >= -big);
 }
 
-constexpr int kSpanRows = 16;   // x rows per workgroup of zspan_kernel
+// x rows per workgroup of zspan_kernel: a step refreshes two slices, so 4
+// rows give 2 x 125 workgroups on a 500-row grid (16 gave 64: 49 us per step)
+constexpr int kSpanRows = 4;
 
 // zspan entries (x, slot of slice i) for x in one block of kSpanRows rows:
 // each wave finds the z span of whole rows (its own cells) of slices i and

@@ -148,6 +148,9 @@ __global__ __launch_bounds__(kThreads) void stats_partial(
 #pragma unroll
       for (int q = p; q < 6; ++q) a[6 + tri_index(p, q)] += v[p] * v[q];
   }
+  // unrolled: constant indices keep a[] in registers (a loop index would
+  // place the array in scratch memory)
+#pragma unroll
   for (int k = 0; k < kNCov; ++k) {
     const double v = block_sum(a[k], red);
     if (threadIdx.x == 0) st[kStatOff + blockIdx.x * kNCov + k] = v;
@@ -176,7 +179,8 @@ __global__ __launch_bounds__(kThreads) void stats_final(int64_t n, int nblk, dou
                                                         double mc2, const double *__restrict__ st,
                                                         double *__restrict__ out) {
   __shared__ double red[kThreads / 64];
-  double tot[kNCov];
+  double tot[kNCov];   // constant indices throughout (unrolled loops): registers
+#pragma unroll
   for (int k = 0; k < kNCov; ++k) {
     double v = 0.0;
     for (int b = threadIdx.x; b < nblk; b += kThreads) v += st[kStatOff + b * kNCov + k];
@@ -186,8 +190,11 @@ __global__ __launch_bounds__(kThreads) void stats_final(int64_t n, int nblk, dou
   const double dn = (double)n;
   const double c[6] = {st[0], st[1], st[2], st[3], st[4], st[5]};
   double *cov = out + DFCSR_STAT_COV;
+#pragma unroll
   for (int k = 0; k < 6; ++k) out[DFCSR_STAT_MEAN + k] = c[k] + tot[k] / dn;
+#pragma unroll
   for (int p = 0; p < 6; ++p)
+#pragma unroll
     for (int q = p; q < 6; ++q) {
       const int t = tri_index(p, q);
       cov[t] = (tot[6 + t] - tot[p] * tot[q] / dn) / (dn - 1.0);

@@ -249,6 +249,22 @@ def test_large_run_to_steady_state_vs_oracle():
     trapz = getattr(np, "trapezoid", None) or np.trapz
     integral = trapz(trapz(d, xg, axis=0), zg)
     assert abs(integral - 1.0) <= 1e-12, integral
+    # with and without the span table (the near regions' lookup skip):
+    # bit-identical wakes, fewer points evaluated
+    from pydfcsr_amd import _lib, kernels
+    H = tr.descriptor()
+    H0 = _lib.History.from_buffer_copy(H)
+    H0.zspan = None
+    P = sim.wake_params()
+    L = sim.lattice.device_tables(sim.device)
+    for prec in (_lib.DFCSR_PREC_FP64, _lib.DFCSR_PREC_MIXED):
+        a = kernels.wake_mesh(H, L, P, prec, sim.device, sim.ws, want_mesh=False)
+        b = kernels.wake_mesh(H0, L, P, prec, sim.device, sim.ws, want_mesh=False)
+        for k in ("dE_dct", "x_kick"):
+            same = (a[k] == b[k]) | (torch.isnan(a[k]) & torch.isnan(b[k]))
+            assert bool(same.all()), (prec, k)
+        ev = [kernels.wake_points_evaluated(h, L, P, prec, sim.device, sim.ws) for h in (H, H0)]
+        assert ev[0] < ev[1], (prec, ev)
     torch.cuda.synchronize()
 
 

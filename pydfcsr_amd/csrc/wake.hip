@@ -763,8 +763,10 @@ __device__ inline void run_phase_block(const WakeTask &T, int l, double *__restr
 // over every row of region 0 inside the history, and writes that partial sum
 // (NaN where the reference's integrand is 0/0 over the region, as
 // wake_phase).  Summation: per column over the rows in order, weighted sums
-// over the block's columns in order.
-template <bool kMixed, bool kTI>
+// over the block's columns in order.  kSkip: lookups the history's span
+// table shows to gather zero density and gradients are skipped (an integrand
+// of exactly 0, as in the near regions).
+template <bool kMixed, bool kTI, bool kSkip = false>
 __device__ void far_obs_block(const WakeTask &T, int l, double *__restrict__ partials,
                               RowTable &rt) {
   const DfcsrHistory &H = T.H;
@@ -865,7 +867,8 @@ __device__ void far_obs_block(const WakeTask &T, int l, double *__restrict__ par
           if (!rt.ok[r]) continue;
           float Iz, Ix;
           if (!csr_point_mixed<kTI, kRings>(H, A, ut0, col, rt.xp[r], rt.xpf[r], rt.i0[r],
-                                            rt.i1[r], CellRow{0u, 0u}, rt.fracf[r], Iz, Ix))
+                                            rt.i1[r], CellRow{0u, 0u}, rt.fracf[r], Iz, Ix,
+                                            kSkip ? H.zspan : nullptr, H.nt))
             continue;
           ++evaluated;
           const float wx = rt.wf[r];
@@ -878,7 +881,9 @@ __device__ void far_obs_block(const WakeTask &T, int l, double *__restrict__ par
           if (!rt.ok[r]) continue;
           const TruncIdx X = {rt.i0[r], rt.i1[r], rt.frac[r], true};
           double Iz, Ix;
-          if (!csr_point<true, false>(H, A, o, col, rt.xp[r], Iz, Ix, &X)) continue;
+          if (!csr_point<true, false>(H, A, o, col, rt.xp[r], Iz, Ix, &X, nullptr,
+                                      kSkip ? H.zspan : nullptr, H.nt))
+            continue;
           ++evaluated;
           const double wx = rt.w[r];
           cz += wx * Iz;
@@ -933,12 +938,23 @@ __global__ __launch_bounds__(kThreads, kCells ? (kMixed ? 4 : 3) : DFCSR_WAKE_MI
 
 // The observer-major region-0 launch: each XCD a contiguous range of z lines.
 // Three workgroups per CU (FP64: 168 VGPRs and a few spilled, against 191 at
-// two; large run FP64 far phases -10 %).
-template <bool kMixed, bool kTI>
+// two; large run FP64 far phases -10 %).  kSkip: the span-table skip
+// (far_skip).
+template <bool kMixed, bool kTI, bool kSkip = false>
 __global__ __launch_bounds__(kThreads, 3) void wake_far_kernel(
     WakeTask T, double *__restrict__ partials) {
   __shared__ RowTable rt;
-  far_obs_block<kMixed, kTI>(T, xcd_remap(blockIdx.x, gridDim.x), partials, rt);
+  far_obs_block<kMixed, kTI, kSkip>(T, xcd_remap(blockIdx.x, gridDim.x), partials, rt);
+}
+
+// Whether a wake's observer-major region 0 skips lookups on the span table:
+// on the chirp branch only.  Measured, bit-identical either way: the chicane
+// (chirp branch, 1e6 particles, 64 x 64 mesh) FP64 wake 4.60 -> 4.14 ms and
+// mixed 2.52 -> 2.35 ms; off the chirp branch the 1e7-particle 128 x 128 run
+// loses (FP64 wake 11.73 -> 12.13 ms): its skipped points read cached zero
+// records, and the span read lengthens every other point's path.
+__host__ inline bool far_skip(const DfcsrHistory &H, const PhasePlan &pl) {
+  return H.zspan != nullptr && pl.nreg == 4;
 }
 
 // Last task whose first index (workgroup or observer) is <= b.
@@ -1151,8 +1167,13 @@ extern "C" int dfcsr_wake_mesh(const DfcsrHistory *hist, const DfcsrLattice *lat
   hipLaunchKernelGGL(wake_observer_kernel, gf, dim3(kThreads), 0, s, T, part);
   if (precision == DFCSR_PREC_MIXED) {
     const bool ti_far = mixed_reads_ti(*hist, false);
-    if (pl.obs_major && ti_far)
+    const bool skip = far_skip(*hist, pl);
+    if (pl.obs_major && ti_far && skip)
+      hipLaunchKernelGGL((wake_far_kernel<true, true, true>), ga, dim3(kThreads), 0, s, T, part);
+    else if (pl.obs_major && ti_far)
       hipLaunchKernelGGL((wake_far_kernel<true, true>), ga, dim3(kThreads), 0, s, T, part);
+    else if (pl.obs_major && skip)
+      hipLaunchKernelGGL((wake_far_kernel<true, false, true>), ga, dim3(kThreads), 0, s, T, part);
     else if (pl.obs_major)
       hipLaunchKernelGGL((wake_far_kernel<true, false>), ga, dim3(kThreads), 0, s, T, part);
     else if (ti_far)
@@ -1172,7 +1193,9 @@ extern "C" int dfcsr_wake_mesh(const DfcsrHistory *hist, const DfcsrLattice *lat
       hipLaunchKernelGGL((wake_phase_kernel<true, true, false>), gb, dim3(kThreads), 0, s, T, part);
     }
   } else {
-    if (pl.obs_major)
+    if (pl.obs_major && far_skip(*hist, pl))
+      hipLaunchKernelGGL((wake_far_kernel<false, false, true>), ga, dim3(kThreads), 0, s, T, part);
+    else if (pl.obs_major)
       hipLaunchKernelGGL((wake_far_kernel<false, false>), ga, dim3(kThreads), 0, s, T, part);
     else
       hipLaunchKernelGGL((wake_phase_kernel<false, false, false>), ga, dim3(kThreads), 0, s, T, part);

@@ -517,6 +517,41 @@ def test_wake_zspan_skips_exact_zeros(K, ws, golden_dir, golden_history):
             assert ev[1] < ev[0], (prec, rings, ev)
 
 
+def test_wake_far_zspan_skip_chirp_mesh(K, ws, golden_dir, golden_history):
+    """On the chirp branch the observer-major region 0 skips, on the span
+    table, the lookups that gather zero density and gradients (an integrand of
+    exactly 0): the wakes are bit-identical with and without the table (FP64
+    and mixed), and fewer points are evaluated than by the same observers
+    given explicitly (region 0 then runs by row chunks, with no such skip;
+    the golden history holds 335 such points of region 0 there), whose
+    wakes agree to 1e-12 (FP64) / 1e-10 (mixed: the same float32 points,
+    summed in float64 in another order)."""
+    w, beam, _, DL, _ = _golden_wake_inputs(golden_dir, golden_history)
+    _, T = golden_history
+    b = dict(beam, slope=np.array([2.5, w["slope"][1]]))
+    xr, zr = w["xrange"], w["zrange"]
+    P = wake_params(0.3, b, 0.05, 1.0, 12, 16, float(w["csr_scaling"]),
+                    (xr[0], xr[-1], xr.size), (zr[0], zr[-1], zr.size))
+    for prec, rings, tol in ((0, ("fp64", "cells"), 1e-12),
+                             (1, ("fp32", "fp32t", "cells32"), 1e-10)):
+        res, ev = [], []
+        for extra in ((), ("zspan",)):
+            DH = DeviceHistory(T, DEV, wrap=2, rings=rings + extra)
+            o = K.wake_mesh(DH.desc, DL.desc, P, prec, DEV, ws)
+            res.append(np.stack([o["dE_dct"].cpu().numpy(), o["x_kick"].cpu().numpy()]))
+            ev.append(K.wake_points_evaluated(DH.desc, DL.desc, P, prec, DEV, ws))
+        assert np.array_equal(res[0], res[1]), (prec, rings)
+        assert np.isfinite(res[1]).all() and np.abs(res[1]).max() > 0
+        obs = torch.stack([o["zmesh"], o["xmesh"]], dim=1).contiguous()
+        Po = wake_params(0.3, b, 0.05, 1.0, 12, 16, float(w["csr_scaling"]), (0.0, 0.0, 1),
+                         (0.0, 0.0, obs.shape[0]), observers=obs)
+        oo = K.wake_mesh(DH.desc, DL.desc, Po, prec, DEV, ws)
+        ev_obs = K.wake_points_evaluated(DH.desc, DL.desc, Po, prec, DEV, ws)
+        assert ev[1] < ev_obs <= ev[0], (prec, ev, ev_obs)
+        for f, k in enumerate(("dE_dct", "x_kick")):
+            assert rel_max(oo[k].cpu().numpy(), res[1][f]) <= tol, (prec, k)
+
+
 def test_wake_mixed_precision_ring_layouts(K, ws, golden_dir, golden_history):
     """The mixed form reads either float32 ring layout (chunked, time-innermost,
     or both: region 0 the time-innermost one, the later regions the chunked

@@ -953,7 +953,7 @@ __global__ __launch_bounds__(kThreads, 3) void wake_far_kernel(
 // mixed 2.52 -> 2.35 ms; off the chirp branch the 1e7-particle 128 x 128 run
 // loses (FP64 wake 11.73 -> 12.13 ms): its skipped points read cached zero
 // records, and the span read lengthens every other point's path.
-__host__ inline bool far_skip(const DfcsrHistory &H, const PhasePlan &pl) {
+__host__ __device__ inline bool far_skip(const DfcsrHistory &H, const PhasePlan &pl) {
   return H.zspan != nullptr && pl.nreg == 4;
 }
 
@@ -981,11 +981,19 @@ __global__ __launch_bounds__(kThreads, kMixed && kNear ? 4 : DFCSR_WAKE_MINBLOCK
   const int l = b - (kNear ? T.blockB0 : T.blockA0);
   if (!kNear && T.P.observers == nullptr) {   // observer-major region 0
     __shared__ RowTable rt;
+    const bool skip = far_skip(T.H, make_plan(T.P));   // per task, as dfcsr_wake_mesh
     if constexpr (kMixed) {
-      if (mixed_reads_ti(T.H, false))
+      const bool ti = mixed_reads_ti(T.H, false);
+      if (ti && skip)
+        far_obs_block<true, true, true>(T, l, partials, rt);
+      else if (ti)
         far_obs_block<true, true>(T, l, partials, rt);
+      else if (skip)
+        far_obs_block<true, false, true>(T, l, partials, rt);
       else
         far_obs_block<true, false>(T, l, partials, rt);
+    } else if (skip) {
+      far_obs_block<false, false, true>(T, l, partials, rt);
     } else {
       far_obs_block<false, false>(T, l, partials, rt);
     }

@@ -525,7 +525,8 @@ def test_wake_far_zspan_skip_chirp_mesh(K, ws, golden_dir, golden_history):
     given explicitly (region 0 then runs by row chunks, with no such skip;
     the golden history holds 335 such points of region 0 there), whose
     wakes agree to 1e-12 (FP64) / 1e-10 (mixed: the same float32 points,
-    summed in float64 in another order)."""
+    summed in float64 in another order).  The batched launch, beside a task
+    off the chirp branch, is bit-identical with and without the table too."""
     w, beam, _, DL, _ = _golden_wake_inputs(golden_dir, golden_history)
     _, T = golden_history
     b = dict(beam, slope=np.array([2.5, w["slope"][1]]))
@@ -534,14 +535,32 @@ def test_wake_far_zspan_skip_chirp_mesh(K, ws, golden_dir, golden_history):
                     (xr[0], xr[-1], xr.size), (zr[0], zr[-1], zr.size))
     for prec, rings, tol in ((0, ("fp64", "cells"), 1e-12),
                              (1, ("fp32", "fp32t", "cells32"), 1e-10)):
-        res, ev = [], []
+        res, ev, hs = [], [], []
         for extra in ((), ("zspan",)):
             DH = DeviceHistory(T, DEV, wrap=2, rings=rings + extra)
+            hs.append(DH)
             o = K.wake_mesh(DH.desc, DL.desc, P, prec, DEV, ws)
             res.append(np.stack([o["dE_dct"].cpu().numpy(), o["x_kick"].cpu().numpy()]))
             ev.append(K.wake_points_evaluated(DH.desc, DL.desc, P, prec, DEV, ws))
         assert np.array_equal(res[0], res[1]), (prec, rings)
         assert np.isfinite(res[1]).all() and np.abs(res[1]).max() > 0
+        # the batched launch (serving) makes the same choice per task, here beside a task
+        # off the chirp branch (the golden slope): bit-identical with and without the
+        # table, and equal to the single launch to rounding (another kernel's code)
+        P0 = wake_params(0.3, beam, 0.05, 1.0, 12, 16, float(w["csr_scaling"]),
+                         (xr[0], xr[-1], xr.size), (zr[0], zr[-1], zr.size))
+        bat = []
+        for H in hs:
+            ob = [{k: torch.empty(xr.size * zr.size, dtype=torch.float64, device=DEV)
+                   for k in ("dE_dct", "x_kick")} for _ in range(2)]
+            K.wake_mesh_batched([H.desc, H.desc], [DL.desc, DL.desc], [P, P0], prec, ob, DEV, ws)
+            bat.append([[ob[j][k].cpu().numpy() for k in ("dE_dct", "x_kick")] for j in (0, 1)])
+        o0 = K.wake_mesh(DH.desc, DL.desc, P0, prec, DEV, ws)
+        for f, k in enumerate(("dE_dct", "x_kick")):
+            for j in (0, 1):
+                assert np.array_equal(bat[0][j][f], bat[1][j][f]), (prec, j, k)
+            assert rel_max(bat[1][0][f], res[1][f]) <= 1e-13, (prec, k)
+            assert rel_max(bat[1][1][f], o0[k].cpu().numpy()) <= 1e-13, (prec, k)
         obs = torch.stack([o["zmesh"], o["xmesh"]], dim=1).contiguous()
         Po = wake_params(0.3, b, 0.05, 1.0, 12, 16, float(w["csr_scaling"]), (0.0, 0.0, 1),
                          (0.0, 0.0, obs.shape[0]), observers=obs)

@@ -235,7 +235,8 @@ typedef struct {
    * corners lie outside the span of its x row and first time plane gathers
    * fields whose integrand is exactly 0 (every term carries a factor
    * density, density_x or density_z, and 0 times a finite value is 0): the
-   * wakes skip it, as they skip lookups outside the history. */
+   * wakes skip it, as they skip lookups outside the history (the near regions
+   * always, region 0 on the chirp branch, |slope| > 1). */
   const uint32_t *zspan;
 } DfcsrHistory;
 

@@ -85,7 +85,8 @@ constexpr int kSpanPlanes = 16;
 // may gather a contributing cell: whether its z corners reach the span of
 // (xi, slot of ti), which covers rows xi, xi + 1 and planes ti, ti + 1.
 // zrow: that row's entries for slices zt0 .. zt0 + kSpanPlanes - 1 (LDS);
-// older slices read the table itself.  When false, every gathered density,
+// older slices read the table itself (with zt0 = nt, every slice does: the
+// region-0 form, which keeps no LDS copy).  When false, every gathered density,
 // density_x and density_z is 0 and vx, vx_x are finite, so the integrand is
 // exactly 0.
 __device__ inline bool zspan_holds(const DfcsrHistory &H, const uint32_t *zrow, int zt0, int xi,

@@ -765,7 +765,8 @@ __device__ inline void run_phase_block(const WakeTask &T, int l, double *__restr
 // wake_phase).  Summation: per column over the rows in order, weighted sums
 // over the block's columns in order.  kSkip: lookups the history's span
 // table shows to gather zero density and gradients are skipped (an integrand
-// of exactly 0, as in the near regions).
+// of exactly 0, as in the near regions); every entry is read from the table
+// itself (zrow = the table, zt0 = nt: no LDS copy of the newest slices).
 template <bool kMixed, bool kTI, bool kSkip = false>
 __device__ void far_obs_block(const WakeTask &T, int l, double *__restrict__ partials,
                               RowTable &rt) {

@@ -289,7 +289,8 @@ def test_chicane_mid_run_chirp_branch_vs_oracle():
     """The chicane at step 200 (192 slices on the full interp grid, chirp
     branch: four regions): FP64 and mixed wakes against the oracle as in
     _deep_run, and the mixed wake without the float64 ring (vx at the
-    observer from float32) within 1e-4 as well."""
+    observer from float32) within 1e-4 as well; the wakes with and without
+    the span table bit-identical."""
     import torch
     from bench import chicane_config
     from pydfcsr_amd import _lib, kernels
@@ -307,3 +308,15 @@ def test_chicane_mid_run_chirp_branch_vs_oracle():
     torch.cuda.synchronize()
     for k in ("dE_dct", "x_kick"):
         assert _max_rms(_np(mixed32[k]), _np(ref[k]))[0] <= 1e-4
+    # with and without the span table (on the chirp branch region 0 skips on it as well
+    # as the near regions): bit-identical wakes, fewer points evaluated
+    H0 = _lib.History.from_buffer_copy(H)
+    H0.zspan = None
+    for prec in (_lib.DFCSR_PREC_FP64, _lib.DFCSR_PREC_MIXED):
+        a = kernels.wake_mesh(H, L, P, prec, sim.device, sim.ws, want_mesh=False)
+        b = kernels.wake_mesh(H0, L, P, prec, sim.device, sim.ws, want_mesh=False)
+        for k in ("dE_dct", "x_kick"):
+            same = (a[k] == b[k]) | (torch.isnan(a[k]) & torch.isnan(b[k]))
+            assert bool(same.all()), (prec, k)
+        ev = [kernels.wake_points_evaluated(h, L, P, prec, sim.device, sim.ws) for h in (H, H0)]
+        assert ev[0] < ev[1], (prec, ev)

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define DFCSR_ABI_VERSION 7
+#define DFCSR_ABI_VERSION 8
 
 enum {
   DFCSR_OK = 0,
@@ -204,9 +204,12 @@ int dfcsr_regrid(const double *src, int64_t src_slot_stride, const DfcsrRegridJo
  * needs at least one FP32 ring (with both, its region-0 phases read the
  * time-innermost one and the other phases the chunked one).
  * The cell-major rings (optional) hold the same slices plane by plane:
- * cell (slot, x, z) at ((slot Xi + x) Zi + z) DFCSR_CELL, the five fields
- * of that grid node (40 B in float64, 20 B in float32, against 160 B / 80 B
- * corner-packed records).  With them the near-region phases of
+ * cell (slot, x, z) at ((slot Xi + x) DFCSR_CELL_ROW(Zi) + z) DFCSR_CELL,
+ * the five fields of that grid node (40 B in float64, 20 B in float32,
+ * against 160 B / 80 B corner-packed records).  Each x row holds Zi + 1
+ * cells: z = 0 .. Zi - 1 and, at z = Zi, a copy of cell Zi - 1, so that the
+ * z pair (z, z + 1) a lookup reads is interpolate3D's (clamped at the last
+ * z) everywhere.  With them the near-region phases of
  * a wake read cells (FP64 wakes: cells, mixed: cells32), whose neighbouring
  * z lie in neighbouring cells; results are identical with or without them.
  * One plane must span at most 4 GiB (the wakes address cells within a plane
@@ -222,11 +225,11 @@ typedef struct {
   const double *data;    /* [Xi][Zi][cap][DFCSR_HIST_REC64] or NULL */
   const float *data32;   /* [Xi][cap][DFCSR_HIST_CHUNKS32][Zi][4] or NULL */
   const float *data32t;  /* [Xi][Zi][cap][DFCSR_HIST_REC32] or NULL */
-  const double *cells;   /* [cap][Xi][Zi][DFCSR_CELL] or NULL */
-  const float *cells32;  /* [cap][Xi][Zi][DFCSR_CELL] or NULL */
+  const double *cells;   /* [cap][Xi][Zi + 1][DFCSR_CELL] or NULL */
+  const float *cells32;  /* [cap][Xi][Zi + 1][DFCSR_CELL] or NULL */
   int32_t cap, head, nt, Xi, Zi;
   double min_t, min_x, min_z, dt, dx, dz;
-  const float *dcells32; /* [cap][Xi][Zi][DFCSR_CELL] or NULL */
+  const float *dcells32; /* [cap][Xi][Zi + 1][DFCSR_CELL] or NULL */
   /* zspan (optional, Zi <= 65535): [Xi][cap] z spans, zlo | zhi << 16, from
    * dfcsr_history_zspan.  Entry (x, slot of slice i) bounds the z of every
    * cell of rows x and min(x + 1, Xi - 1), in slice i and (when i < nt - 1)
@@ -262,6 +265,8 @@ int dfcsr_history_zspan(const double *cells, const float *cells32, uint32_t *zsp
 int dfcsr_cells32_tdiff(const float *cells32, float *dcells32, int Xi, int Zi, int cap, int head,
                         int nt, int first, void *stream);
 #define DFCSR_CELL 5
+/* cells per x row of the cell-major rings: the grid's Zi and a copy of the last */
+#define DFCSR_CELL_ROW(Zi) ((Zi) + 1)
 
 /* Lattice tables (lattice.py:4-110, 136-150): table[k][0..5] =
  * coords_x, coords_y, n_x, n_y, tau_x, tau_y at s = min_s + k*ds, k < ns;

@@ -20,7 +20,7 @@ c_i32 = ctypes.c_int32
 c_i64 = ctypes.c_int64
 c_f64 = ctypes.c_double
 
-ABI_VERSION = 7   # DFCSR_ABI_VERSION
+ABI_VERSION = 8   # DFCSR_ABI_VERSION
 DFCSR_PREC_FP64 = 0
 DFCSR_PREC_MIXED = 1
 MOMENTS_COUNT = 10

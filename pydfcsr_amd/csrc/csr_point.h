@@ -148,51 +148,33 @@ __device__ inline double trilerp(double a00, double a01, double a10, double a11,
   return lerp<kFused>(lerp<kFused>(c00, c10, xd), lerp<kFused>(c01, c11, xd), zd);
 }
 
-// trilerp of a z pair read one cell low at the grid's last z (zlast): the
-// upper z value serves both z corners, as the clamped record gives them.
-template <bool kFused>
-__device__ inline double trilerp_zpair(double a00, double a01, double a10, double a11, double b00,
-                                       double b01, double b10, double b11, double td, double xd,
-                                       double zd, bool zlast) {
-  const double c00 = lerp<kFused>(a00, b00, td), c01 = lerp<kFused>(a01, b01, td);
-  const double c10 = lerp<kFused>(a10, b10, td), c11 = lerp<kFused>(a11, b11, td);
-  const double hi = lerp<kFused>(c01, c11, xd);
-  const double lo = zlast ? hi : lerp<kFused>(c00, c10, xd);
-  return lerp<kFused>(lo, hi, zd);
-}
-
-// The wake kernels' trilinear value.  The weights of the four (t, x) corner
-// columns -- (t0,x0), (t0,x1), (t1,x0), (t1,x1) -- are computed once per
-// lookup; each field takes its lower and upper z value as a weighted sum of
-// the columns and then one z lerp: 10 operations per field against 14 for
-// t, x and z lerps, and one code path for every lookup (at the last time
-// plane or x row the two planes or rows are the same cells, as
-// interpolate3D's clamped indices give them, and every corner is weighted as
-// there).  zlast: the z pair was read one cell low at the grid's last z; its
-// upper value serves both z corners and the lower one, another cell's, is
-// not used.
-struct TxWeights {
-  double w00, w01, w10, w11;
+// The wake kernels' trilinear value: the eight corner weights, products of
+// the t, x and z weights computed once per lookup, and one weighted sum per
+// field (8 operations against 14 for t, x and z lerps).  Every corner is
+// weighted, as in interpolate3D, so a non-finite corner gives a non-finite
+// value there too; at the last plane, x row or z the two planes, rows or z
+// cells are the same values, as its clamped indices give them.
+struct TriWeights {
+  double w[8];   // (t0|t1) x (x0,z0), (x0,z1), (x1,z0), (x1,z1)
 };
 
-__device__ inline TxWeights tx_weights(double td, double xd) {
-  const double t0 = 1.0 - td, x0 = 1.0 - xd;
-  return {t0 * x0, t0 * xd, td * x0, td * xd};
+__device__ inline TriWeights tri_weights(double td, double xd, double zd) {
+  const double t0 = 1.0 - td, x0 = 1.0 - xd, z0 = 1.0 - zd;
+  const double p00 = t0 * x0, p01 = t0 * xd, p10 = td * x0, p11 = td * xd;
+  return {{p00 * z0, p00 * zd, p01 * z0, p01 * zd, p10 * z0, p10 * zd, p11 * z0, p11 * zd}};
 }
 
 // a: plane t0, b: plane t1; corners (x0,z0), (x0,z1), (x1,z0), (x1,z1)
-__device__ inline double trilerp_w(const TxWeights &w, double a00, double a01, double a10,
-                                   double a11, double b00, double b01, double b10, double b11,
-                                   double zd, bool zlast) {
-  double lo = w.w00 * a00;
-  lo = fma(w.w01, a10, lo);
-  lo = fma(w.w10, b00, lo);
-  lo = fma(w.w11, b10, lo);
-  double hi = w.w00 * a01;
-  hi = fma(w.w01, a11, hi);
-  hi = fma(w.w10, b01, hi);
-  hi = fma(w.w11, b11, hi);
-  return lerp<true>(zlast ? hi : lo, hi, zd);
+__device__ inline double trilerp_w(const TriWeights &w, double a00, double a01, double a10,
+                                   double a11, double b00, double b01, double b10, double b11) {
+  double acc = w.w[0] * a00;
+  acc = fma(w.w[1], a01, acc);
+  acc = fma(w.w[2], a10, acc);
+  acc = fma(w.w[3], a11, acc);
+  acc = fma(w.w[4], b00, acc);
+  acc = fma(w.w[5], b01, acc);
+  acc = fma(w.w[6], b10, acc);
+  return fma(w.w[7], b11, acc);
 }
 
 // Trilinear lookup of all five fields at (t, x, z) with interpolate3D's
@@ -228,19 +210,18 @@ __device__ inline bool hist_lookup5(const DfcsrHistory &H, const HistAxes &A, do
   }
   const double td = T.frac, xd = X.frac, zd = Z.frac;
   if (kCells) {
-    // cell (slot, x, z) at byte ((slot Xi + x) Zi + z) * 40 of the cell-major
-    // ring; each x row's cells z0 and z0 + 1 are read as one 80-byte pair
-    // (five 16-byte loads, against three per 40-byte cell).  At the grid's
-    // last z (interpolate3D clamps z1 to z0) the pair is (z0 - 1, z0), and
-    // the z0 value serves both z corners: the clamped record's values, so
-    // the result is the corner-packed ring's exactly.
+    // cell (slot, x, z) at byte ((slot Xi + x) (Zi + 1) + z) * 40 of the
+    // cell-major ring; each x row's cells z0 and z0 + 1 are read as one
+    // 80-byte pair (five 16-byte loads, against three per 40-byte cell).  At
+    // the grid's last z the pair is that cell and the row's copy of it, the
+    // clamped corners of interpolate3D, so the result is the corner-packed
+    // ring's exactly.
     constexpr uint32_t cb = DFCSR_CELL * sizeof(double);
-    const uint32_t rowB = (uint32_t)H.Zi * cb;
+    const uint32_t rowB = (uint32_t)DFCSR_CELL_ROW(H.Zi) * cb;
     const CellRow r = rr ? *rr : CellRow{(uint32_t)X.i0 * rowB, (uint32_t)(X.i1 - X.i0) * rowB};
     const int64_t planeB = (int64_t)H.Xi * rowB;
     const char *ring = reinterpret_cast<const char *>(H.cells);
-    const bool zlast = Z.i1 == Z.i0;   // Zi >= 2: the pair (z - 1, z)
-    const uint32_t o0 = r.rb + (uint32_t)(Z.i0 - (zlast ? 1 : 0)) * cb;
+    const uint32_t o0 = r.rb + (uint32_t)Z.i0 * cb;
     // both planes read on every lookup (at the last plane the second is the
     // first again, a cache hit): no branch between the two forms
     const char *p0 = ring + hist_slot(H, T.i0) * planeB;
@@ -249,16 +230,16 @@ __device__ inline bool hist_lookup5(const DfcsrHistory &H, const HistAxes &A, do
     load_pairs5(p0, o0, o0 + r.dxb, v0);
     load_pairs5(p1, o0, o0 + r.dxb, v1);
     if constexpr (kRecip) {
-      const TxWeights w = tx_weights(td, xd);
+      const TriWeights w = tri_weights(td, xd, zd);
 #pragma unroll
       for (int f = 0; f < 5; ++f)
         out[f] = trilerp_w(w, v0[0][f], v0[1][f], v0[2][f], v0[3][f], v1[0][f], v1[1][f],
-                           v1[2][f], v1[3][f], zd, zlast);
+                           v1[2][f], v1[3][f]);
     } else {
 #pragma unroll
       for (int f = 0; f < 5; ++f)
-        out[f] = trilerp_zpair<false>(v0[0][f], v0[1][f], v0[2][f], v0[3][f], v1[0][f],
-                                      v1[1][f], v1[2][f], v1[3][f], td, xd, zd, zlast);
+        out[f] = trilerp<false>(v0[0][f], v0[1][f], v0[2][f], v0[3][f], v1[0][f], v1[1][f],
+                                v1[2][f], v1[3][f], td, xd, zd);
     }
     return true;
   }
@@ -280,13 +261,11 @@ __device__ inline bool hist_lookup5(const DfcsrHistory &H, const HistAxes &A, do
 #pragma unroll
   for (int k = 0; k < DFCSR_HIST_REC64 / 2; ++k) b[k] = p1[k];
   if constexpr (kRecip) {
-    // at the grid's last z the record's z1 corners are its z0 ones, so the
-    // lower and upper z values are equal already
-    const TxWeights w = tx_weights(td, xd);
+    const TriWeights w = tri_weights(td, xd, zd);
 #pragma unroll
     for (int f = 0; f < 5; ++f)
       out[f] = trilerp_w(w, a[2 * f].x, a[2 * f].y, a[2 * f + 1].x, a[2 * f + 1].y, b[2 * f].x,
-                         b[2 * f].y, b[2 * f + 1].x, b[2 * f + 1].y, zd, false);
+                         b[2 * f].y, b[2 * f + 1].x, b[2 * f + 1].y);
   } else {
 #pragma unroll
     for (int f = 0; f < 5; ++f)

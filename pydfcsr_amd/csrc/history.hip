@@ -68,7 +68,8 @@ __device__ inline void rgi5(const DfcsrRegridJob &J, const double *__restrict__ 
 
 // One thread per target record: evaluates the four corner points of the
 // record and writes the corner-packed record into each ring present, and
-// the record's own grid node (its first corner) into each cell-major ring.
+// the record's own grid node (its first corner) into each cell-major ring
+// (at the last z also into the row's extra cell, its copy).
 __global__ __launch_bounds__(kThreads) void regrid_kernel(
     const double *__restrict__ src, int64_t slot_stride, const DfcsrRegridJob *__restrict__ jobs,
     const double *__restrict__ means, double *__restrict__ h64, float *__restrict__ h32,
@@ -111,11 +112,19 @@ __global__ __launch_bounds__(kThreads) void regrid_kernel(
         s4[f * (int64_t)Zi] = make_float4((float)v[0][f], (float)v[1][f], (float)v[2][f],
                                           (float)v[3][f]);
     }
-    const int64_t cell = ((int64_t)J.dst_slot * cells + c) * DFCSR_CELL;
+    const int64_t cell =
+        (((int64_t)J.dst_slot * Xi + ix) * DFCSR_CELL_ROW(Zi) + iz) * DFCSR_CELL;
+    const bool pad = iz == Zi - 1;   // also the row's copy of its last cell
 #pragma unroll
     for (int f = 0; f < 5; ++f) {
-      if (c64) c64[cell + f] = v[0][f];
-      if (c32) c32[cell + f] = (float)v[0][f];
+      if (c64) {
+        c64[cell + f] = v[0][f];
+        if (pad) c64[cell + DFCSR_CELL + f] = v[0][f];
+      }
+      if (c32) {
+        c32[cell + f] = (float)v[0][f];
+        if (pad) c32[cell + DFCSR_CELL + f] = (float)v[0][f];
+      }
     }
   }
 }
@@ -181,13 +190,13 @@ __global__ __launch_bounds__(kThreads) void zspan_kernel(const T *__restrict__ c
   const int xb = blockIdx.x * kSpanRows;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = kThreads / 64;
   const int nrow = min(kSpanRows + 1, Xi - xb);   // rows xb .. xb + nrow - 1 (x + 1 clamped)
-  const int64_t plane = (int64_t)Xi * Zi * DFCSR_CELL;
+  const int64_t plane = (int64_t)Xi * DFCSR_CELL_ROW(Zi) * DFCSR_CELL;
   for (int i = first + (int)blockIdx.y; i < nt; i += (int)gridDim.y) {
     const int np = i + 1 < nt ? 2 : 1;
     for (int w = wave; w < np * nrow; w += nwave) {
       const int p = w / nrow, r = w - p * nrow;
       const T *row = cells + (int64_t)((head + i + p) % cap) * plane +
-                     (int64_t)(xb + r) * Zi * DFCSR_CELL;
+                     (int64_t)(xb + r) * DFCSR_CELL_ROW(Zi) * DFCSR_CELL;
       int lo = 0x7fffffff, hi = -1;
       for (int z = lane; z < Zi; z += 64) {
         if (cell_contributes(row + (int64_t)z * DFCSR_CELL)) {
@@ -258,7 +267,7 @@ extern "C" int dfcsr_cells32_tdiff(const float *cells32, float *dcells32, int Xi
     return set_error(DFCSR_EINVAL, "cells32_tdiff: cap %d head %d nt %d", cap, head, nt);
   if (first < 0) first = 0;
   if (first >= nt) return DFCSR_OK;
-  const int64_t n = (int64_t)Xi * Zi * DFCSR_CELL;
+  const int64_t n = (int64_t)Xi * DFCSR_CELL_ROW(Zi) * DFCSR_CELL;   // the row copies too
   const bool vec = n % 4 == 0;
   const int64_t per = vec ? n / 4 : n;
   int gx = (int)((per + kThreads - 1) / kThreads);

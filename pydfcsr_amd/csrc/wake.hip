@@ -345,21 +345,21 @@ __device__ inline bool csr_point_mixed(const DfcsrHistory &H, const HistAxes &A,
   const float zd = (float)Z.frac;
   float f[5];
   if (kCells != kRings) {
-    // float32 cell (slot, x, z) at byte ((slot Xi + x) Zi + z) * 20 of the
-    // cell-major rings.  The time lerp v0 + td (v1 - v0) takes v1 - v0 from
-    // H.dcells32 at the same offset: the float32 difference of the next
+    // float32 cell (slot, x, z) at byte ((slot Xi + x) (Zi + 1) + z) * 20 of
+    // the cell-major rings.  The time lerp v0 + td (v1 - v0) takes v1 - v0
+    // from H.dcells32 at the same offset: the float32 difference of the next
     // slice's cell and this one's, 0 at the newest slice, where
     // interpolate3D clamps i1 to i0 -- so one plane of each ring serves every
     // lookup, with no branch.  Each x row's two z cells are one 40-byte pair;
-    // at the grid's last z (interpolate3D clamps z1 to z0) the pair is the one
-    // before, (z - 1, z), and the z value serves both z corners: the clamped
-    // record's values, so the result is the corner-packed rings' exactly.
+    // at the grid's last z the pair is that cell and the row's copy of it,
+    // interpolate3D's clamped corners, so the result is the corner-packed
+    // rings' exactly.
     constexpr uint32_t cb = DFCSR_CELL * sizeof(float);
-    const uint32_t planeB = (uint32_t)H.Xi * (uint32_t)H.Zi * cb;   // < 4 GiB (checked)
+    const uint32_t planeB =
+        (uint32_t)H.Xi * (uint32_t)DFCSR_CELL_ROW(H.Zi) * cb;   // < 4 GiB (checked)
     const char *ring = reinterpret_cast<const char *>(H.cells32);
     const char *dring = reinterpret_cast<const char *>(H.dcells32);
-    const bool zlast = Z.i1 == Z.i0;   // Zi >= 2 (checked)
-    const uint32_t o = rr.rb + (uint32_t)(Z.i0 - (zlast ? 1 : 0)) * cb;
+    const uint32_t o = rr.rb + (uint32_t)Z.i0 * cb;
     const int s0 = hist_slot(H, T.i0);
     float v[4][5], d[4][5];
     if (kCells == kCellsRing32) {
@@ -377,8 +377,8 @@ __device__ inline bool csr_point_mixed(const DfcsrHistory &H, const HistAxes &A,
       const float c01 = fmaf(td, d[1][k], v[1][k]);
       const float c10 = fmaf(td, d[2][k], v[2][k]);
       const float c11 = fmaf(td, d[3][k], v[3][k]);
+      const float c0 = fmaf(xd, c10 - c00, c00);
       const float c1 = fmaf(xd, c11 - c01, c01);
-      const float c0 = zlast ? c1 : fmaf(xd, c10 - c00, c00);
       f[k] = fmaf(zd, c1 - c0, c0);
     }
   } else {
@@ -661,7 +661,7 @@ __device__ void wake_phase(const WakeTask &T, const double *__restrict__ od, int
     rt.wf[r] = (float)rt.w[r];
     rt.fracf[r] = (float)X.frac;
     if (kCells != kRings) {
-      const uint32_t rowB = (uint32_t)H.Zi * (uint32_t)(DFCSR_CELL *
+      const uint32_t rowB = (uint32_t)DFCSR_CELL_ROW(H.Zi) * (uint32_t)(DFCSR_CELL *
                                                         (kMixed ? sizeof(float) : sizeof(double)));
       rt.rr[r].rb = (uint32_t)X.i0 * rowB;
       rt.rr[r].dxb = (uint32_t)(X.i1 - X.i0) * rowB;
@@ -923,7 +923,8 @@ __host__ __device__ inline bool mixed_cells(const DfcsrHistory &H) {
 
 // Whether every byte of the float32 cell-major rings has a 32-bit offset.
 __host__ __device__ inline bool cells32_ring32(const DfcsrHistory &H) {
-  return (int64_t)H.cap * H.Xi * H.Zi * (int64_t)(DFCSR_CELL * sizeof(float)) <= 0xffffffffLL;
+  return (int64_t)H.cap * H.Xi * DFCSR_CELL_ROW(H.Zi) * (int64_t)(DFCSR_CELL * sizeof(float)) <=
+         0xffffffffLL;
 }
 
 // Workgroups per CU the register allocation must allow: the near phases on
@@ -1086,13 +1087,9 @@ int check_rings(const DfcsrHistory &H, int precision, const char *who) {
   // byte offsets
   const bool cells = precision == DFCSR_PREC_MIXED ? mixed_cells(H) : H.cells != nullptr;
   const int64_t cellB = DFCSR_CELL * (precision == DFCSR_PREC_MIXED ? 4 : 8);
-  if (cells && (int64_t)H.Xi * H.Zi * cellB > ((int64_t)1 << 32))
+  if (cells && (int64_t)H.Xi * DFCSR_CELL_ROW(H.Zi) * cellB > ((int64_t)1 << 32))
     return set_error(DFCSR_EINVAL, "%s: a cell-major plane of %d x %d cells exceeds 4 GiB", who,
-                     H.Xi, H.Zi);
-  // a cell-major lookup reads the z pair (z0 - 1, z0) at the grid's last z
-  if (cells && H.Zi < 2)
-    return set_error(DFCSR_EINVAL, "%s: a cell-major ring needs at least 2 z cells, got %d", who,
-                     H.Zi);
+                     H.Xi, DFCSR_CELL_ROW(H.Zi));
   if (precision == DFCSR_PREC_MIXED && !H.data32 && !H.data32t)
     return set_error(DFCSR_EINVAL, "%s: mixed precision needs an FP32 history ring", who);
   return DFCSR_OK;

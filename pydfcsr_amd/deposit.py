@@ -12,9 +12,10 @@ whole run:
   hist   [Xi][Zi][cap][20]     float64  re-gridded corner-packed records
   hist32t[Xi][Zi][cap][20]     float32  the same records (mixed precision)
   hist32 [Xi][cap][5][Zi][4]   float32  the same, chunked (optional)
-  cells  [cap][Xi][Zi][5]      float64  cell-major: the five fields at (x, z)
-  cells32[cap][Xi][Zi][5]      float32  the same in float32
-  dcells32[cap][Xi][Zi][5]     float32  cells32 of the next slice minus this one's (0
+  cells  [cap][Xi][Zi+1][5]    float64  cell-major: the five fields at (x, z), and
+                                        per x row a copy of its last cell
+  cells32[cap][Xi][Zi+1][5]    float32  the same in float32
+  dcells32[cap][Xi][Zi+1][5]   float32  cells32 of the next slice minus this one's (0
                                         for the newest): the mixed wake's time lerp
   zspan  [Xi][cap]             int32    per x row and slot the z span of the cells that
                                         can contribute (rows x, x+1; slices i, i+1):
@@ -97,9 +98,9 @@ class DF_tracker:
         self.hist = None      # float64 (Xi, Zi, cap, 20)
         self.hist32 = None    # float32 chunked (Xi, cap, 5, Zi, 4)
         self.hist32t = None   # float32 time-innermost (Xi, Zi, cap, 20)
-        self.cells = None     # float64 cell-major (cap, Xi, Zi, 5)
-        self.cells32 = None   # float32 cell-major (cap, Xi, Zi, 5)
-        self.dcells32 = None  # its time differences (cap, Xi, Zi, 5)
+        self.cells = None     # float64 cell-major (cap, Xi, Zi + 1, 5)
+        self.cells32 = None   # float32 cell-major (cap, Xi, Zi + 1, 5)
+        self.dcells32 = None  # its time differences (cap, Xi, Zi + 1, 5)
         self.zspan = None     # z spans of contributing cells (Xi, cap), uint32 as int32
         self.cur = None
         self.n_regrid = 0
@@ -162,12 +163,14 @@ class DF_tracker:
         dev = self.device
         if self.keep_fp64:
             hist = torch.empty((Xi, Zi, cap, _lib.HIST_REC64), dtype=torch.float64, device=dev)
-            if self.keep_cells and Xi * Zi * _lib.CELL * 8 <= (1 << 32):
-                cells = torch.empty((cap, Xi, Zi, _lib.CELL), dtype=torch.float64, device=dev)
+            if self.keep_cells and Xi * (Zi + 1) * _lib.CELL * 8 <= (1 << 32):
+                cells = torch.empty((cap, Xi, Zi + 1, _lib.CELL), dtype=torch.float64,
+                                    device=dev)
         if self.keep_fp32:
             hist32t = torch.empty((Xi, Zi, cap, _lib.HIST_REC32), dtype=torch.float32, device=dev)
-            if self.keep_cells and Xi * Zi * _lib.CELL * 4 <= (1 << 32):
-                cells32 = torch.empty((cap, Xi, Zi, _lib.CELL), dtype=torch.float32, device=dev)
+            if self.keep_cells and Xi * (Zi + 1) * _lib.CELL * 4 <= (1 << 32):
+                cells32 = torch.empty((cap, Xi, Zi + 1, _lib.CELL), dtype=torch.float32,
+                                      device=dev)
                 dcells32 = torch.empty_like(cells32)
             if self.keep_chunked:
                 hist32 = torch.empty((Xi, cap, _lib.HIST_CHUNKS32, Zi, 4), dtype=torch.float32,

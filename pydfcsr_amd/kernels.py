@@ -219,7 +219,8 @@ def regrid(src: torch.Tensor, slot_stride: int, jobs, hist64, hist32, Xi, Zi, xi
     """jobs: list of (src_slot, dst_slot, nbx, nbz, x0, x1, z0, z1).  Rings (any
     may be None, not all): hist64 float64 (Xi, Zi, cap, 20), hist32 float32
     chunked (Xi, cap, 5, Zi, 4), hist32t float32 time-innermost (Xi, Zi, cap, 20),
-    cells / cells32 cell-major float64 / float32 (cap, Xi, Zi, 5)."""
+    cells / cells32 cell-major float64 / float32 (cap, Xi, Zi + 1, 5): each x
+    row's Zi cells and a copy of its last."""
     lib = _lib.load()
     n = len(jobs)
     if n == 0:
@@ -238,8 +239,8 @@ def regrid(src: torch.Tensor, slot_stride: int, jobs, hist64, hist32, Xi, Zi, xi
     want = {id(hist64): ((Xi, Zi, cap, _lib.HIST_REC64), torch.float64),
             id(hist32t): ((Xi, Zi, cap, _lib.HIST_REC32), torch.float32),
             id(hist32): ((Xi, cap, _lib.HIST_CHUNKS32, Zi, 4), torch.float32),
-            id(cells): ((cap, Xi, Zi, _lib.CELL), torch.float64),
-            id(cells32): ((cap, Xi, Zi, _lib.CELL), torch.float32)}
+            id(cells): ((cap, Xi, Zi + 1, _lib.CELL), torch.float64),
+            id(cells32): ((cap, Xi, Zi + 1, _lib.CELL), torch.float32)}
     for r in (hist64, hist32, hist32t, cells, cells32):
         if r is None:
             continue
@@ -255,9 +256,9 @@ def regrid(src: torch.Tensor, slot_stride: int, jobs, hist64, hist32, Xi, Zi, xi
 
 def cells32_tdiff(cells32: torch.Tensor, dcells32: torch.Tensor, head: int, nt: int,
                   first: int = 0):
-    """Fills dcells32 (cap, Xi, Zi, 5) float32 with the time differences of the
-    float32 cell-major ring cells32 for logical slices first .. nt-1 (slice i in
-    slot (head + i) % cap): slice i + 1 minus slice i, 0 for the newest."""
+    """Fills dcells32 (cap, Xi, Zi + 1, 5) float32 with the time differences of
+    the float32 cell-major ring cells32 for logical slices first .. nt-1 (slice i
+    in slot (head + i) % cap): slice i + 1 minus slice i, 0 for the newest."""
     lib = _lib.load()
     shape = tuple(cells32.shape)
     if (len(shape) != 4 or shape[3] != _lib.CELL or cells32.dtype != torch.float32
@@ -265,8 +266,8 @@ def cells32_tdiff(cells32: torch.Tensor, dcells32: torch.Tensor, head: int, nt: 
             or not cells32.is_contiguous() or not dcells32.is_contiguous()):
         raise DfcsrError(f"cells32_tdiff: rings {shape} {cells32.dtype} / "
                          f"{tuple(dcells32.shape)} {dcells32.dtype} are not two contiguous "
-                         f"float32 (cap, Xi, Zi, {_lib.CELL}) rings")
-    cap, Xi, Zi, _ = shape
+                         f"float32 (cap, Xi, Zi + 1, {_lib.CELL}) rings")
+    cap, Xi, Zi = shape[0], shape[1], shape[2] - 1
     check(lib.dfcsr_cells32_tdiff(ptr(cells32), ptr(dcells32), int(Xi), int(Zi), int(cap),
                                   int(head), int(nt), int(first), stream_handle(cells32.device)),
           "cells32_tdiff")
@@ -275,7 +276,7 @@ def cells32_tdiff(cells32: torch.Tensor, dcells32: torch.Tensor, head: int, nt: 
 def history_zspan(cells, cells32, zspan: torch.Tensor, head: int, nt: int, first: int = 0):
     """Fills zspan (Xi, cap) int32 -- read as uint32 zlo | zhi << 16 -- for the
     logical slices first .. nt-1 from the float64 cell-major ring cells (cap,
-    Xi, Zi, 5), or when that is None from the float32 one cells32: the z span
+    Xi, Zi + 1, 5), or when that is None from the float32 one cells32: the z span
     of the cells of rows x, x + 1 in slices i, i + 1 whose density or
     gradients are nonzero or whose velocities are not finite."""
     lib = _lib.load()
@@ -286,8 +287,8 @@ def history_zspan(cells, cells32, zspan: torch.Tensor, head: int, nt: int, first
     if (ring.dim() != 4 or ring.shape[3] != _lib.CELL or ring.dtype != want
             or not ring.is_contiguous()):
         raise DfcsrError(f"history_zspan: ring {tuple(ring.shape)} {ring.dtype} is not a "
-                         f"contiguous (cap, Xi, Zi, {_lib.CELL}) {want} ring")
-    cap, Xi, Zi, _ = ring.shape
+                         f"contiguous (cap, Xi, Zi + 1, {_lib.CELL}) {want} ring")
+    cap, Xi, Zi = ring.shape[0], ring.shape[1], ring.shape[2] - 1
     if (tuple(zspan.shape) != (Xi, cap) or zspan.dtype != torch.int32
             or not zspan.is_contiguous()):
         raise DfcsrError(f"history_zspan: table {tuple(zspan.shape)} {zspan.dtype} is not a "

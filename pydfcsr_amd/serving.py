@@ -91,10 +91,10 @@ def request_footprint(config: dict, steps: int, precision: str = "mixed") -> int
     grid = grid_bound(config)
     slots = steps + 2   # window and pending slice
     # per slot and grid node: the corner-packed record (160 B in float64, 80 B
-    # in float32) and the cell-major cell (40 B / 20 B, in float32 also its
-    # time difference, 20 B)
-    rec = (80 + 2 * 4 * _lib.CELL) if precision == "mixed" else (160 + 8 * _lib.CELL)
-    hist = slots * grid * grid * rec
+    # in float32); per slot and cell of the cell-major rows (grid + 1 per x
+    # row) the cell (40 B / 20 B, in float32 also its time difference, 20 B)
+    rec, cell = (80, 2 * 4 * _lib.CELL) if precision == "mixed" else (160, 8 * _lib.CELL)
+    hist = slots * grid * (grid * rec + (grid + 1) * cell)
     raw = slots * 5 * nb * nb * 8
     particles = n * 8 * (6 * 2 + 4)           # coordinates, tracking temporary, scratch
     deposit = 4 * nb * nb * 8 * 2 + 3 * nb * nb * 8

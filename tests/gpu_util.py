@@ -50,13 +50,15 @@ def device_ring(records, cap, head, width=4):
 
 
 def cell_ring(data, cap, head, dtype):
-    """(nt, Xi, Zi, 5) field stacks -> a cell-major ring (cap, Xi, Zi, 5): the
-    five fields of node (x, z), logical slice i in slot (head + i) % cap;
-    slots holding no slice are NaN."""
+    """(nt, Xi, Zi, 5) field stacks -> a cell-major ring (cap, Xi, Zi + 1, 5):
+    the five fields of node (x, z), and per x row a copy of its last cell at
+    z = Zi; logical slice i in slot (head + i) % cap; slots holding no slice
+    are NaN."""
     nt, Xi, Zi, _ = data.shape
-    ring = np.full((cap, Xi, Zi, _lib.CELL), np.nan, dtype=dtype)
+    ring = np.full((cap, Xi, Zi + 1, _lib.CELL), np.nan, dtype=dtype)
     for i in range(nt):
-        ring[(head + i) % cap] = data[i]
+        ring[(head + i) % cap, :, :Zi] = data[i]
+        ring[(head + i) % cap, :, Zi] = data[i][:, Zi - 1]
     return ring
 
 

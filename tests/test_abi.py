@@ -27,7 +27,7 @@ def test_library_exports_header_symbols():
     for s in syms:
         assert hasattr(lib, s), s
     assert set(syms) <= set(_lib.EXPORTED) | {"dfcsr_savgol_operator"}
-    assert lib.dfcsr_abi_version() == _lib.ABI_VERSION == 7
+    assert lib.dfcsr_abi_version() == _lib.ABI_VERSION == 8
 
 
 def test_argument_errors_without_device():

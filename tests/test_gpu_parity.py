@@ -215,8 +215,10 @@ def test_history_golden(golden_dir):
     """The device history against the reference's: re-grid flags, window,
     axes and fields.  Through the same full re-grids, window pops and ring
     growth, the float32 time-difference ring stays slice i + 1 minus slice i
-    of the float32 cell-major ring (bit-exact), 0 for the newest slice, and
-    the span table the one the float64 cell-major ring gives."""
+    of the float32 cell-major ring (bit-exact), 0 for the newest slice, the
+    span table the one the float64 cell-major ring gives, and each x row of
+    the cell-major rings ends with a copy of its last cell; the float64 cells
+    equal the corner-packed ring's node values."""
     from pydfcsr_amd.deposit import DF_tracker
     from tests.gpu_util import zspan_table
     d = _g(golden_dir, "ref_history.npz")
@@ -234,8 +236,11 @@ def test_history_golden(golden_dir):
         want[:-1] = c32[1:] - c32[:-1]
         assert np.array_equal(tr.dcells32.cpu().numpy()[idx], want), k
         c64 = tr.cells.cpu().numpy()[idx]
+        # each x row ends with a copy of its last cell
+        assert np.array_equal(c64[:, :, -1], c64[:, :, -2]), k
+        assert np.array_equal(c32[:, :, -1], c32[:, :, -2]), k
         zs = tr.zspan.cpu().numpy()[:, idx]
-        assert np.array_equal(zs, zspan_table(c64, len(idx), 0)), k
+        assert np.array_equal(zs, zspan_table(c64[:, :, :-1], len(idx), 0)), k
         if k >= 1:
             tr.build_interpolant()
     assert flags == list(d["regrid_flags"])
@@ -245,6 +250,10 @@ def test_history_golden(golden_dir):
     hf = tr.history_fields()
     for f in ("density", "density_x", "density_z", "vx", "vx_x"):
         assert rel_max(hf[f].cpu(), d[f]) <= 1e-9, f
+    # the float64 cells are the corner-packed ring's grid-node values, exactly
+    c64 = tr.cells.cpu().numpy()[[tr._slot(i) for i in range(tr.nt)]]
+    for k, f in enumerate(("density", "density_x", "density_z", "vx", "vx_x")):
+        assert np.array_equal(c64[:, :, :-1, k], hf[f].cpu().numpy()), f
 
 
 # -- a13 -----------------------------------------------------------------------------
